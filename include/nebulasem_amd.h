@@ -1,0 +1,92 @@
+/*
+ * nebulasem_amd.h — C-ABI of the MI355X-native NebulaSEM hot path.
+ *
+ * Two paths of NebulaSEM (dshawul/NebulaSEM) run here on one MI355X (gfx950):
+ *   1. the explicit dGSEM residual + update of the compressible Euler solver
+ *      (apps/euler/euler.cpp:179-287: divf -> rusanov -> div_flux + volume
+ *      term -> addTemporal -> Solve), and
+ *   2. the FV pressure Poisson solve (lap -> MeshMatrix -> SolveT Jacobi-PCG,
+ *      src/field/field.h:3571-3704, src/solvers/solve.cpp:39-559).
+ * Plain pointers and sizes only.  Every function returns an nsem_status; on
+ * failure nsem_last_error() describes the cause.  Node arrays are indexed as
+ * the reference indexes them: node = cell*NP + i*NPY*NPZ + j*NPZ + k
+ * (src/field/dg.h:43-44), interior nodes first.
+ */
+#ifndef NEBULASEM_AMD_H
+#define NEBULASEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NSEM_OK = 0,
+    NSEM_EINVAL = 1,        /* bad argument                                  */
+    NSEM_EUNSUPPORTED = 2,  /* input outside what this build implements     */
+    NSEM_EIO = 3,           /* file could not be read or written             */
+    NSEM_EDEVICE = 4,       /* HIP runtime error / no device                 */
+    NSEM_ENONFINITE = 5,    /* a NaN or Inf appeared in the state            */
+    NSEM_EBREAKDOWN = 6,    /* PCG breakdown (p.Ap <= 0)                      */
+    NSEM_ENOTCONVERGED = 7  /* iteration limit reached before the tolerance  */
+} nsem_status;
+
+/* Message of the last failure on this thread (never NULL). */
+const char* nsem_last_error(void);
+/* Library version string. */
+const char* nsem_version(void);
+
+/* ------------------------------------------------------------------ mesh --
+ * Replaces the reference's mesh tool (apps/mesh/meshApp.cpp:16-283) and
+ * Mesh::LoadMesh + initGeomMeshFields + DG::init_geom
+ * (src/field/field.cpp:95-384, src/field/dg.cpp:147-477).  Host only.     */
+typedef struct nsem_mesh nsem_mesh;
+
+typedef struct {
+    uint32_t npx, npy, npz, np, npf; /* LGL points per direction, per element, per face */
+    uint32_t nbcs;                   /* interior cells (gBCS)                 */
+    uint32_t ncells;                 /* interior + ghost cells (gNCells)      */
+    uint32_t nfacets;                /* gNFacets                              */
+    uint32_t nvertices;
+    uint64_t nbcs_field;             /* interior nodes (gBCSfield)            */
+    uint64_t nall_field;             /* all nodes (gALLfield)                 */
+    uint32_t nboundaries;            /* named boundary patches                */
+} nsem_mesh_info;
+
+/* Block description file -> grid file (.txt or .bin), as `mesh <spec> -o <grid>`
+ * (meshApp.cpp:16-283).  Straight-edged single blocks. */
+int nsem_grid_generate(const char* spec_path, const char* grid_path);
+/* Load a grid file and build node geometry for polynomial orders order[3]
+ * (the controls keys npx npy npz).  nonortho: 0 NONE 1 MINIMUM 2 ORTHOGONAL
+ * 3 OVER_RELAXED (FV only).  As Mesh::LoadMesh(0) (field.cpp:95-167). */
+int nsem_mesh_load(const char* grid_path, const int32_t order[3], int32_t nonortho,
+                   nsem_mesh** out);
+/* Same from a block description file, without writing the grid. */
+int nsem_mesh_from_spec(const char* spec_path, const int32_t order[3], int32_t nonortho,
+                        nsem_mesh** out);
+/* Same from one block given directly: corners[8][3] in the reference's block
+ * corner order, ndiv[3], per-edge spacing (0 linear 1 geometric 2 wall) and
+ * ratio (NULL = linear), and npatch named patches, patch i given by
+ * npoints[i] points (consecutive xyz triples in `points`). */
+int nsem_mesh_box(const double corners[24], const uint32_t ndiv[3], const int32_t spacing[12],
+                  const double ratio[12], int32_t npatch, const char* const* names,
+                  const int32_t* npoints, const double* points, const int32_t order[3],
+                  int32_t nonortho, nsem_mesh** out);
+int nsem_mesh_write_grid(const nsem_mesh* m, const char* grid_path);
+int nsem_mesh_get_info(const nsem_mesh* m, nsem_mesh_info* info);
+/* Named geometry arrays (the reference's names: FO FN fN fC fI fD cC cV Jinv
+ * allFaces faceIndices0/1 gFC gFN gCC gCV FOC FNC FMC vertices facet_off
+ * facet_vtx cell_off cell_faces faceid_off faceid boff bfaces has_boundary
+ * xgl0..2 wgl0..2 psi0..2 dpsi0..2).  kind: 0 = uint32, 1 = double. */
+int nsem_mesh_array_size(const nsem_mesh* m, const char* name, uint64_t* count, int32_t* kind);
+int nsem_mesh_get_array(const nsem_mesh* m, const char* name, void* dst, uint64_t count);
+int nsem_mesh_boundary_name(const nsem_mesh* m, uint32_t i, char* buf, uint64_t len);
+void nsem_mesh_destroy(nsem_mesh* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEBULASEM_AMD_H */

@@ -1,0 +1,5 @@
+"""MI355X-native NebulaSEM hot path: dGSEM Euler residual and FV pressure PCG.
+
+Python host mirror over the C-ABI in include/nebulasem_amd.h (ctypes); the
+compute runs in libnebulasem_amd.so (hand-written gfx950 HIP kernels)."""
+from ._lib import Mesh, NsemError, grid_generate, lib  # noqa: F401
