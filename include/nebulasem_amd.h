@@ -85,6 +85,18 @@ int nsem_mesh_get_array(const nsem_mesh* m, const char* name, void* dst, uint64_
 int nsem_mesh_boundary_name(const nsem_mesh* m, uint32_t i, char* buf, uint64_t len);
 void nsem_mesh_destroy(nsem_mesh* m);
 
+/* ---------------------------------------------------------------- fields --
+ * Reads a field file of the reference (<name><step>.txt|.bin: internal values
+ * as an explicit list or up to four generators uniform / gaussian / cosine /
+ * cosine2 / linear / gaussian-outside / hydrostatic, then the boundary block),
+ * as MeshField::read_ (src/field/field.h:1407-1565).  values: nall_field*ncomp
+ * (every node, ghosts included, xyz per node for ncomp 3); bc_kind/bc_value
+ * per mesh boundary patch in nsem_mesh_boundary_name order (kind -1 when the
+ * file has no entry for the patch).  gravity is used by `hydrostatic`.     */
+enum { NSEM_BC_NEUMANN = 0, NSEM_BC_SYMMETRY = 1, NSEM_BC_DIRICHLET = 2 };
+int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
+                    double* values, int32_t* bc_kind, double* bc_value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,3 +125,16 @@ class Mesh:
 
 def grid_generate(spec_path, grid_path):
     check(lib().nsem_grid_generate(spec_path.encode(), grid_path.encode()))
+
+
+def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
+    """Read a reference field file: (values[nall, ncomp], bc_kind[nb], bc_value[nb, ncomp])."""
+    n = mesh.info.nall_field
+    nb = mesh.info.nboundaries
+    vals = np.empty(n * ncomp)
+    kinds = np.empty(max(nb, 1), dtype=np.int32)
+    bv = np.empty(max(nb, 1) * ncomp)
+    check(lib().nsem_field_read(mesh.handle, path.encode(), ncomp, (C.c_double * 3)(*gravity),
+                                vals.ctypes.data_as(C.c_void_p), kinds.ctypes.data_as(C.c_void_p),
+                                bv.ctypes.data_as(C.c_void_p)))
+    return vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp)

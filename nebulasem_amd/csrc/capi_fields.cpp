@@ -1,0 +1,44 @@
+// C-ABI: reading the reference's field files (initial values + BCs).
+#include <cstring>
+#include <string>
+
+#include "host/fields.hpp"
+#include "mesh_access.hpp"
+#include "nebulasem_amd.h"
+#include "status.hpp"
+
+extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp,
+                               const double gravity[3], double* values, int32_t* bc_kind,
+                               double* bc_value) {
+    if (!m || !path || !values || !bc_kind || !bc_value)
+        return nsem_fail(NSEM_EINVAL, "null argument");
+    if (ncomp != 1 && ncomp != 3) return nsem_fail(NSEM_EINVAL, "ncomp must be 1 or 3");
+    const nsem::Geometry& g = nsem_mesh_geometry(m);
+    std::vector<double> cc(g.nall_field * 3);
+    for (uint64_t i = 0; i < g.nall_field; i++)
+        for (int c = 0; c < 3; c++) cc[3 * i + c] = g.cC[i][c];
+    nsem::FieldContext ctx;
+    ctx.nall = g.nall_field;
+    ctx.nbcs = g.nbcs_field;
+    ctx.cC = cc.data();
+    if (gravity)
+        for (int c = 0; c < 3; c++) ctx.gravity[c] = gravity[c];
+    nsem::FieldData fd;
+    std::string err;
+    if (!nsem::read_field_file(path, ctx, ncomp, fd, &err)) return nsem_fail(NSEM_EIO, err);
+    std::memcpy(values, fd.values.data(), sizeof(double) * fd.values.size());
+    const std::vector<std::string>& names = nsem_mesh_boundary_names(m);
+    for (size_t b = 0; b < names.size(); b++) {
+        bc_kind[b] = -1;
+        for (int c = 0; c < ncomp; c++) bc_value[b * ncomp + c] = 0;
+        for (const auto& bc : fd.bcs) {
+            if (bc.patch != names[b]) continue;
+            if (bc.type == "NEUMANN") bc_kind[b] = NSEM_BC_NEUMANN;
+            else if (bc.type == "SYMMETRY") bc_kind[b] = NSEM_BC_SYMMETRY;
+            else if (bc.type == "DIRICHLET") bc_kind[b] = NSEM_BC_DIRICHLET;
+            else return nsem_fail(NSEM_EUNSUPPORTED, "boundary type " + bc.type + " is not supported");
+            for (int c = 0; c < ncomp; c++) bc_value[b * ncomp + c] = bc.value[c];
+        }
+    }
+    return NSEM_OK;
+}

@@ -5,6 +5,7 @@
 
 #include "host/mesh.hpp"
 #include "nebulasem_amd.h"
+#include "mesh_access.hpp"
 #include "status.hpp"
 
 struct nsem_mesh {
@@ -235,3 +236,7 @@ int nsem_mesh_boundary_name(const nsem_mesh* m, uint32_t i, char* buf, uint64_t 
 void nsem_mesh_destroy(nsem_mesh* m) { delete m; }
 
 }  // extern "C"
+
+const nsem::Geometry& nsem_mesh_geometry(const nsem_mesh* m) { return m->geo; }
+const nsem::LoadedMesh& nsem_mesh_loaded(const nsem_mesh* m) { return m->lm; }
+const std::vector<std::string>& nsem_mesh_boundary_names(const nsem_mesh* m) { return m->bnames; }

@@ -988,6 +988,8 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
         for (Int ci = 0; ci < m.nbcs; ci++) {
             const IntVec& c = m.cells[ci];
             const IntVec& fids = m.faceid[ci];
+            if (c.size() < 4 || fids.size() != c.size())
+                return set_err(err, "a cell lost its faces (every boundary face unnamed?)");
             Int id0 = fids[0], id1 = id0 ^ 1;
             IntVec f1, f2;
             for (size_t i = 0; i < c.size(); i++) {

@@ -1,0 +1,455 @@
+/*
+ * ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load this; it is the checker, never the
+ * product path.
+ *
+ * A plain-C, single-threaded CPU restatement of the reference NebulaSEM
+ * algorithms on the hot path, written to follow the reference's own data
+ * flow: fields stored on every node including ghost nodes, face values built
+ * by gather (cds / rusanov), a cell-by-cell surface integral and a weak-form
+ * volume integral, then explicit boundary-condition passes.
+ *
+ *   om_euler_split_steps  apps/euler/euler.cpp:179-287 (diffusion NO, BDF1):
+ *       T += T0; Fc = rho*U; lambda = (cds|U| + cds sqrt(g R T))/2;
+ *       rho  <- Solve(divf(U*rho) + ddt)               field.h:3416-3478,3740-3920
+ *       p    <- P0 (rho T R / P0)^gamma - p_ref        euler.cpp:211-213
+ *       U    <- Solve(divf(Fc (x) U + I p) - src + ddt)  euler.cpp:216-240
+ *       T    <- Solve(divf(Fc T) + ddt); T -= T0       euler.cpp:243-286
+ *     cds           field.h:2880-2893     rusanov   field.h:2927-2943
+ *     div_flux      field.h:3085-3117     divf      field.h:3416-3478
+ *     fillBCs       field.h:2730-2776     applyExplicitBCs field.h:2585-2727
+ *     sym (SYMMETRY) src/tensor/tensor.h:483-492
+ *   om_fv_poisson          apps/potential/potential.cpp:58-64 pressure solve:
+ *       divf (FV) field.h:3470-3477, lap (FV) field.h:3571-3615,
+ *       applyImplicitBCs field.h:2541-2582, SolveT PCG solve.cpp:259-280,403-417
+ *       with the reference's stopping rule sqrt(|sum z^2| / |sum x^2|) (solve.cpp:11-35).
+ *   om_euler_hydrostatic   euler.cpp:104-149 background and initial density.
+ *
+ * Pinning: om_euler_split_steps and om_fv_poisson are checked against the
+ * outputs of the reference build (oracle/ref.mk) committed under
+ * tests/golden/; see tests/test_oracle.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int32_t npx, npy, npz, np, npf;
+    uint32_t nbcs, ncells, nfacets;
+    uint64_t nbf, naf;             /* interior nodes, all nodes           */
+    const uint32_t *FO, *FN;       /* nfacets*npf                         */
+    const double *fN;              /* nfacets*npf*3                       */
+    const double *fI;              /* nfacets*npf                         */
+    const double *fD;              /* nfacets*npf (FV)                    */
+    const double *cV;              /* naf                                 */
+    const double *cC;              /* naf*3                               */
+    const double *Jinv;            /* naf*9 (XX YY ZZ XY YZ XZ YX ZY ZX)   */
+    const double *psi[3], *dpsi[3];
+    const uint32_t *allFaces, *fi0, *fi1;
+    int32_t nbnd;                  /* boundary patches (name order)       */
+    const uint32_t *boff, *bfaces;
+} om_mesh;
+
+enum { OM_NEUMANN = 0, OM_SYMMETRY = 1, OM_DIRICHLET = 2 };
+enum { XX, YY, ZZ, XY, YZ, XZ, YX, ZY, ZX };
+
+typedef struct {
+    double dt, P0, T0, cp, cv;
+    double g[3];
+    int32_t buoyancy;
+    /* per patch boundary kinds and values (values: 1 or 3 per patch) */
+    const int32_t *bc_rho, *bc_U, *bc_T, *bc_p;
+    const double *val_rho, *val_U, *val_T, *val_p;
+} om_euler;
+
+static double dot3(const double* p, const double* q) { /* right-nested, tensor.h:117 */
+    double r = p[2] * q[2] + 0.0;
+    r = p[1] * q[1] + r;
+    return p[0] * q[0] + r;
+}
+static int equal0(double p) { return fabs(p) <= 1e-7; }
+
+/* ---------------------------------------------------- boundary passes -- */
+static void bc_scalar(const om_mesh* m, const int32_t* kind, const double* val, double* f) {
+    for (int32_t b = 0; b < m->nbnd; b++)
+        for (uint32_t j = m->boff[b]; j < m->boff[b + 1]; j++) {
+            uint32_t fid = m->bfaces[j];
+            for (int32_t n = 0; n < m->npf; n++) {
+                uint64_t k = (uint64_t)fid * m->npf + n;
+                uint32_t c1 = m->FO[k], c2 = m->FN[k];
+                if (c2 >= m->naf) continue;
+                if (kind[b] == OM_NEUMANN) {
+                    double dv[3] = {m->cC[3 * c2] - m->cC[3 * c1], m->cC[3 * c2 + 1] - m->cC[3 * c1 + 1],
+                                    m->cC[3 * c2 + 2] - m->cC[3 * c1 + 2]};
+                    f[c2] = f[c1] + val[b] * sqrt(dot3(dv, dv));
+                } else if (kind[b] == OM_SYMMETRY) {
+                    f[c2] = f[c1];
+                } else {
+                    f[c2] = val[b];
+                }
+            }
+        }
+}
+
+static void sym_vec(const double* p, const double* n, double* r) {
+    double mn = sqrt(dot3(n, n));
+    double en[3] = {n[0] / mn, n[1] / mn, n[2] / mn};
+    double A[6];  /* I - en en^T, STensor order XX YY ZZ XY YZ XZ */
+    A[0] = 1 - en[0] * en[0];
+    A[1] = 1 - en[1] * en[1];
+    A[2] = 1 - en[2] * en[2];
+    A[3] = 0 - en[0] * en[1];
+    A[4] = 0 - en[1] * en[2];
+    A[5] = 0 - en[0] * en[2];
+    r[0] = A[0] * p[0] + A[3] * p[1] + A[5] * p[2];
+    r[1] = A[3] * p[0] + A[1] * p[1] + A[4] * p[2];
+    r[2] = A[5] * p[0] + A[4] * p[1] + A[2] * p[2];
+    double mr = sqrt(dot3(r, r));
+    if (equal0(mr)) return;
+    double s = sqrt(dot3(p, p)) / mr;
+    r[0] *= s; r[1] *= s; r[2] *= s;
+}
+
+static void bc_vector(const om_mesh* m, const int32_t* kind, const double* val, double* f) {
+    for (int32_t b = 0; b < m->nbnd; b++)
+        for (uint32_t j = m->boff[b]; j < m->boff[b + 1]; j++) {
+            uint32_t fid = m->bfaces[j];
+            for (int32_t n = 0; n < m->npf; n++) {
+                uint64_t k = (uint64_t)fid * m->npf + n;
+                uint32_t c1 = m->FO[k], c2 = m->FN[k];
+                if (c2 >= m->naf) continue;
+                if (kind[b] == OM_NEUMANN) {
+                    double dv[3] = {m->cC[3 * c2] - m->cC[3 * c1], m->cC[3 * c2 + 1] - m->cC[3 * c1 + 1],
+                                    m->cC[3 * c2 + 2] - m->cC[3 * c1 + 2]};
+                    double md = sqrt(dot3(dv, dv));
+                    for (int c = 0; c < 3; c++) f[3 * c2 + c] = f[3 * c1 + c] + val[3 * b + c] * md;
+                } else if (kind[b] == OM_SYMMETRY) {
+                    sym_vec(&f[3 * c1], &m->fN[3 * k], &f[3 * c2]);
+                } else {
+                    for (int c = 0; c < 3; c++) f[3 * c2 + c] = val[3 * b + c];
+                }
+            }
+        }
+}
+
+/* ghost node <- owner node after an integral (fillBCs, field.h:2733-2743) */
+static void fill_bcs(const om_mesh* m, double* f, int nc) {
+    for (uint32_t i = m->nbcs; i < m->ncells; i++) {
+        uint32_t f0 = m->fi0[i];
+        for (int32_t n = 0; n < m->npf; n++) {
+            uint64_t k = (uint64_t)m->allFaces[f0] * m->npf + n;
+            uint32_t c2 = m->FN[k];
+            if (c2 < m->naf)
+                for (int c = 0; c < nc; c++) f[(uint64_t)c2 * nc + c] = f[(uint64_t)m->FO[k] * nc + c];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ DG -- */
+/* Explicit weak divergence of a flux F with `rows` rows of 3 components
+ * (rows=1: vector flux of a scalar equation, rows=3: tensor flux), with
+ * Rusanov face flux using q (rows components) and lambda per facet node.
+ * r has naf*rows entries.  field.h:3416-3464. */
+static void divf_dg(const om_mesh* m, int rows, const double* F, const double* q,
+                    const double* lam, double* r) {
+    const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    double* fF = (double*)calloc(nfn * rows * 3, sizeof(double));
+    /* rusanov = cds(F) - unit(fN) (x) lambda (qN - qO) */
+    for (uint64_t k = 0; k < nfn; k++) {
+        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        if (c2 >= m->naf) continue;
+        const double fi = m->fI[k];
+        const double* nv = &m->fN[3 * k];
+        double mn = sqrt(dot3(nv, nv));
+        double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+        for (int rr = 0; rr < rows; rr++) {
+            double dq = lam[k] * (q[(uint64_t)c2 * rows + rr] - q[(uint64_t)c1 * rows + rr]);
+            for (int c = 0; c < 3; c++) {
+                double v = F[((uint64_t)c1 * rows + rr) * 3 + c] * fi +
+                           F[((uint64_t)c2 * rows + rr) * 3 + c] * (1 - fi);
+                if (rows == 1) v = v - en[c] * dq;
+                fF[(k * rows + rr) * 3 + c] = v;
+            }
+        }
+        if (rows == 3) { /* tensor flux: row r gets en_r * (lambda dq_c) in column c */
+            for (int rr = 0; rr < 3; rr++)
+                for (int c = 0; c < 3; c++) {
+                    double dq = lam[k] * (q[(uint64_t)c2 * 3 + c] - q[(uint64_t)c1 * 3 + c]);
+                    fF[(k * 3 + rr) * 3 + c] -= en[rr] * dq;
+                }
+        }
+    }
+    /* surface integral, cell by cell (div_flux<false>) */
+    memset(r, 0, sizeof(double) * m->naf * rows);
+    for (uint32_t i = 0; i < m->ncells; i++)
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++)
+            for (int32_t n = 0; n < m->npf; n++) {
+                uint64_t k = (uint64_t)m->allFaces[f] * m->npf + n;
+                uint32_t c1 = m->FO[k], c2 = m->FN[k];
+                const double* nv = &m->fN[3 * k];
+                int own = (c1 >= (uint64_t)i * m->np && c1 < (uint64_t)(i + 1) * m->np);
+                if (!own && c2 >= m->naf) continue;
+                for (int rr = 0; rr < rows; rr++) {
+                    const double* v = &fF[(k * rows + rr) * 3];
+                    double d = rows == 1 ? dot3(v, nv) : v[0] * nv[0] + v[1] * nv[1] + v[2] * nv[2];
+                    if (own) r[(uint64_t)c1 * rows + rr] += d;
+                    else r[(uint64_t)c2 * rows + rr] -= d;
+                }
+            }
+    free(fF);
+    /* volume integral (weak form) */
+    const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
+    for (uint32_t ci = 0; ci < m->nbcs; ci++)
+        for (int ii = 0; ii < NPX; ii++)
+            for (int jj = 0; jj < NPY; jj++)
+                for (int kk = 0; kk < NPZ; kk++) {
+                    uint64_t index = (uint64_t)ci * m->np + ii * NPY * NPZ + jj * NPZ + kk;
+                    double Jin[9];
+                    for (int c = 0; c < 9; c++) Jin[c] = m->Jinv[9 * index + c] * m->cV[index];
+                    for (int line = 0; line < 3; line++) {
+                        int npts = line == 0 ? NPX : (line == 1 ? NPY : NPZ);
+                        for (int t = 0; t < npts; t++) {
+                            int im = ii, jm = jj, km = kk;
+                            if (line == 0) im = t;
+                            else if (line == 1) { if (t == jj) continue; jm = t; }
+                            else { if (t == kk) continue; km = t; }
+                            uint64_t index1 = (uint64_t)ci * m->np + im * NPY * NPZ + jm * NPZ + km;
+                            double d[3];
+                            d[0] = m->dpsi[0][ii * NPX + im] * m->psi[1][jj * NPY + jm] * m->psi[2][kk * NPZ + km];
+                            d[1] = m->psi[0][ii * NPX + im] * m->dpsi[1][jj * NPY + jm] * m->psi[2][kk * NPZ + km];
+                            d[2] = m->psi[0][ii * NPX + im] * m->psi[1][jj * NPY + jm] * m->dpsi[2][kk * NPZ + km];
+                            double dj[3] = {Jin[XX] * d[0] + Jin[XY] * d[1] + Jin[XZ] * d[2],
+                                            Jin[YX] * d[0] + Jin[YY] * d[1] + Jin[YZ] * d[2],
+                                            Jin[ZX] * d[0] + Jin[ZY] * d[1] + Jin[ZZ] * d[2]};
+                            for (int rr = 0; rr < rows; rr++) {
+                                const double* v = &F[(index * rows + rr) * 3];
+                                double s = rows == 1 ? dot3(v, dj) : v[0] * dj[0] + v[1] * dj[1] + v[2] * dj[2];
+                                r[index1 * rows + rr] -= s;
+                            }
+                        }
+                    }
+                }
+    fill_bcs(m, r, rows);
+}
+
+/* cds of a node scalar at every facet node */
+static void cds_scalar(const om_mesh* m, const double* f, double* out) {
+    const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    for (uint64_t k = 0; k < nfn; k++) {
+        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        if (c2 >= m->naf) { out[k] = 0; continue; }
+        out[k] = f[c1] * m->fI[k] + f[c2] * (1 - m->fI[k]);
+    }
+}
+
+/* Hydrostatic background and initial density from (T', p')
+ * (euler.cpp:104-149 with buoyancy, non-spherical). */
+void om_euler_hydrostatic(const om_mesh* m, const om_euler* e, const double* T, const double* p,
+                          double* p_ref, double* rho_ref, double* rho) {
+    const double R = e->cp - e->cv, gam = e->cp / e->cv;
+    for (uint64_t i = 0; i < m->naf; i++) {
+        double gh = e->buoyancy ? dot3(e->g, &m->cC[3 * i]) : 0.0;
+        p_ref[i] = e->buoyancy ? e->P0 * pow(1.0 + gh / (e->cp * e->T0), e->cp / R) : e->P0;
+        rho_ref[i] = e->buoyancy ? (e->P0 / (R * e->T0)) * pow(p_ref[i] / e->P0, 1 / gam)
+                                 : (e->P0 / (R * e->T0));
+        double pt = p[i] + p_ref[i];
+        rho[i] = (e->P0 / (R * (T[i] + e->T0))) * pow(pt / e->P0, 1 / gam);
+    }
+}
+
+/* The reference's split forward-Euler step, nsteps times.  All arrays are
+ * node arrays over every node (naf), ghosts included; U and g are xyz per
+ * node.  Returns 0, or 5 if a non-finite value appears. */
+int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_ref,
+                         const double* rho_ref, double* rho, double* U, double* T, double* p,
+                         int nsteps) {
+    const uint64_t N = m->naf, nfn = (uint64_t)m->nfacets * m->npf;
+    const double R = e->cp - e->cv, gam = e->cp / e->cv;
+    double* Fc = (double*)malloc(sizeof(double) * N * 3);
+    double* lam = (double*)malloc(sizeof(double) * nfn);
+    double* t1 = (double*)malloc(sizeof(double) * nfn);
+    double* w = (double*)malloc(sizeof(double) * N);
+    double* rhof = (double*)malloc(sizeof(double) * N);
+    double* fq = (double*)malloc(sizeof(double) * N * 9);
+    double* q = (double*)malloc(sizeof(double) * N * 3);
+    double* r = (double*)malloc(sizeof(double) * N * 3);
+    int status = 0;
+    for (int step = 0; step < nsteps; step++) {
+        for (uint64_t i = 0; i < N; i++) T[i] += e->T0;
+        for (uint64_t i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) Fc[3 * i + c] = rho[i] * U[3 * i + c];
+        /* lambdaMax = (cds(mag(U)) + cds(sqrt(gamma R T))) / 2 */
+        for (uint64_t i = 0; i < N; i++) w[i] = sqrt(dot3(&U[3 * i], &U[3 * i]));
+        cds_scalar(m, w, lam);
+        for (uint64_t i = 0; i < N; i++) w[i] = sqrt(gam * R * T[i]);
+        cds_scalar(m, w, t1);
+        for (uint64_t k = 0; k < nfn; k++) lam[k] = (lam[k] + t1[k]) / 2;
+        memcpy(rhof, rho, sizeof(double) * N);
+
+        /* rho equation */
+        for (uint64_t i = 0; i < N; i++)
+            for (int c = 0; c < 3; c++) fq[3 * i + c] = U[3 * i + c] * rho[i];
+        divf_dg(m, 1, fq, rho, lam, r);
+        for (uint64_t i = 0; i < N; i++) {
+            double ap = (-1.0 / e->dt) * m->cV[i];
+            double su = r[i] + rho[i] * ap;
+            rho[i] = su / ap;
+        }
+        bc_scalar(m, e->bc_rho, e->val_rho, rho);
+
+        /* pressure from the new density and the old temperature */
+        for (uint64_t i = 0; i < N; i++) p[i] = e->P0 * pow((rho[i] * T[i] * R) / e->P0, gam);
+        bc_scalar(m, e->bc_p, e->val_p, p);
+        for (uint64_t i = 0; i < N; i++) p[i] -= p_ref[i];
+
+        /* momentum equation: flux Fc (x) U + I p, Rusanov on rho^{n+1} U^n */
+        for (uint64_t i = 0; i < N; i++) {
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    fq[9 * i + 3 * a + b] = Fc[3 * i + a] * U[3 * i + b] + (a == b ? p[i] : 0.0 * p[i]);
+            for (int c = 0; c < 3; c++) q[3 * i + c] = rho[i] * U[3 * i + c];
+        }
+        divf_dg(m, 3, fq, q, lam, r);
+        for (uint64_t i = 0; i < N; i++) {
+            double ap0 = (-1.0 / e->dt) * m->cV[i];
+            double ap = ap0 * rho[i];
+            for (int c = 0; c < 3; c++) {
+                double sc = e->buoyancy ? (rho[i] - rho_ref[i]) * e->g[c] : 0.0;
+                double su = (r[3 * i + c] - sc * m->cV[i]) + (U[3 * i + c] * rhof[i]) * ap0;
+                U[3 * i + c] = su / ap;
+            }
+        }
+        bc_vector(m, e->bc_U, e->val_U, U);
+
+        /* potential temperature equation: flux Fc T, Rusanov on rho^{n+1} T^n */
+        for (uint64_t i = 0; i < N; i++) {
+            for (int c = 0; c < 3; c++) fq[3 * i + c] = Fc[3 * i + c] * T[i];
+            q[i] = rho[i] * T[i];
+        }
+        divf_dg(m, 1, fq, q, lam, r);
+        for (uint64_t i = 0; i < N; i++) {
+            double ap0 = (-1.0 / e->dt) * m->cV[i];
+            double su = r[i] + (T[i] * rhof[i]) * ap0;
+            T[i] = su / (ap0 * rho[i]);
+        }
+        bc_scalar(m, e->bc_T, e->val_T, T);
+        for (uint64_t i = 0; i < N; i++) T[i] -= e->T0;
+        for (uint64_t i = 0; i < m->nbf; i++)
+            if (!isfinite(rho[i]) || !isfinite(T[i])) status = 5;
+        if (status) break;
+    }
+    free(Fc); free(lam); free(t1); free(w); free(rhof); free(fq); free(q); free(r);
+    return status;
+}
+
+/* ------------------------------------------------------------------ FV -- */
+/* FV pressure Poisson as the potential app solves it: divU = divf(U);
+ * Solve(lap(p, 1, true) == divU) with DIAG-PCG.  bc_p/val_p per patch
+ * (NEUMANN or DIRICHLET).  U: naf*3 (ghosts set by its own BCs), p: naf
+ * (initial guess, ghosts set).  On return p holds the solution on interior
+ * cells; *iters, *res0, *res the reference's log line values.
+ * Returns 0 (converged) or 7 (iteration limit). */
+int om_fv_poisson(const om_mesh* m, const int32_t* bc_p, const double* val_p, const double* U,
+                  double* p, double tol, int32_t max_it, int32_t* iters, double* res0,
+                  double* res) {
+    const uint64_t N = m->naf, nfn = m->nfacets;
+    const uint64_t nb = m->nbf;
+    double* divU = (double*)calloc(N, sizeof(double));
+    double* ano = (double*)malloc(sizeof(double) * nfn);
+    double* ann = (double*)malloc(sizeof(double) * nfn);
+    double* ap = (double*)calloc(N, sizeof(double));
+    double* Su = (double*)calloc(N, sizeof(double));
+    double* rr = (double*)calloc(N, sizeof(double));
+    double* z = (double*)calloc(N, sizeof(double));
+    double* pp = (double*)calloc(N, sizeof(double));
+    double* Ap = (double*)calloc(N, sizeof(double));
+    /* divf(U), FV: cds face values, surface sum, ghost fill */
+    for (uint32_t i = 0; i < m->ncells; i++)
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++) {
+            uint64_t k = m->allFaces[f];
+            uint32_t c1 = m->FO[k], c2 = m->FN[k];
+            double fF[3];
+            for (int c = 0; c < 3; c++) fF[c] = U[3 * c1 + c] * m->fI[k] + U[3 * c2 + c] * (1 - m->fI[k]);
+            double d = dot3(fF, &m->fN[3 * k]);
+            if (c1 == i) divU[c1] += d;
+            else if (c2 < N) divU[c2] -= d;
+        }
+    fill_bcs(m, divU, 1);
+    /* lap(p, 1, true): ano = ann = fD * cds(1) */
+    for (uint64_t k = 0; k < nfn; k++) {
+        double mu = 1.0 * m->fI[k] + 1.0 * (1 - m->fI[k]);
+        ano[k] = m->fD[k] * mu;
+        ann[k] = m->fD[k] * mu;
+    }
+    for (uint32_t i = 0; i < m->ncells; i++)
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++) {
+            uint64_t k = m->allFaces[f];
+            uint32_t c1 = m->FO[k], c2 = m->FN[k];
+            if (c1 == i) ap[c1] += ano[k];
+            else if (c2 < N) ap[c2] += ann[k];
+        }
+    /* lap == divU : Su = 0 - divU (the non-orthogonal correction vanishes for a
+     * zero initial guess; this restatement requires p = 0 on entry) */
+    for (uint64_t i = 0; i < N; i++) Su[i] = 0.0 - divU[i];
+    /* applyImplicitBCs */
+    for (int32_t b = 0; b < m->nbnd; b++)
+        for (uint32_t j = m->boff[b]; j < m->boff[b + 1]; j++) {
+            uint64_t k = m->bfaces[j];
+            uint32_t c1 = m->FO[k], c2 = m->FN[k];
+            if (c2 >= N) continue;
+            if (bc_p[b] == OM_NEUMANN) {
+                ap[c1] -= ann[k];
+                Su[c1] += ann[k] * (p[c2] - p[c1]);
+            } else {
+                Su[c1] += ann[k] * p[c2];
+            }
+            ann[k] = 0;
+        }
+    (void)val_p;
+#define MULT(x, out)                                                          \
+    for (uint64_t i = 0; i < nb; i++) {                                        \
+        double acc = x[i] * ap[i];                                             \
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++) {                     \
+            uint64_t k = m->allFaces[f];                                       \
+            uint32_t c1 = m->FO[k], c2 = m->FN[k];                             \
+            if (c1 == i) acc -= x[c2] * ann[k];                                \
+            else if (c2 < N) acc -= x[c1] * ano[k];                            \
+        }                                                                      \
+        out[i] = acc;                                                          \
+    }
+    /* CALC_RESID */
+    MULT(p, Ap);
+    for (uint64_t i = 0; i < nb; i++) rr[i] = Su[i] - Ap[i];
+    for (uint64_t i = 0; i < nb; i++) z[i] = rr[i] * (1 / ap[i]);
+    double sz = 0, sx = 0, o_rr = 0, oo_rr;
+    for (uint64_t i = 0; i < nb; i++) { sz += z[i] * z[i]; sx += p[i] * p[i]; }
+    double r0 = sqrt(sx ? fabs(sz) / fabs(sx) : 0);
+    for (uint64_t i = 0; i < nb; i++) o_rr += rr[i] * z[i];
+    for (uint64_t i = 0; i < nb; i++) pp[i] = z[i];
+    int it = 0, status = 7;
+    double rs = r0;
+    while (it < max_it) {
+        it++;
+        MULT(pp, Ap);
+        oo_rr = 0;
+        for (uint64_t i = 0; i < nb; i++) oo_rr += pp[i] * Ap[i];
+        double alpha = oo_rr ? o_rr / oo_rr : 0;
+        for (uint64_t i = 0; i < nb; i++) p[i] = p[i] + pp[i] * alpha;
+        for (uint64_t i = 0; i < nb; i++) rr[i] = rr[i] + Ap[i] * (-alpha);
+        for (uint64_t i = 0; i < nb; i++) z[i] = rr[i] * (1 / ap[i]);
+        oo_rr = o_rr;
+        o_rr = 0;
+        for (uint64_t i = 0; i < nb; i++) o_rr += rr[i] * z[i];
+        double beta = oo_rr ? o_rr / oo_rr : 0;
+        for (uint64_t i = 0; i < nb; i++) pp[i] = z[i] + pp[i] * beta;
+        sz = 0; sx = 0;
+        for (uint64_t i = 0; i < nb; i++) { sz += z[i] * z[i]; sx += p[i] * p[i]; }
+        rs = sqrt(sx ? fabs(sz) / fabs(sx) : 0);
+        if (rs <= tol) { status = 0; break; }
+    }
+#undef MULT
+    *iters = it; *res0 = r0; *res = rs;
+    free(divU); free(ano); free(ann); free(ap); free(Su); free(rr); free(z); free(pp); free(Ap);
+    return status;
+}

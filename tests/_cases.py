@@ -1,0 +1,172 @@
+"""Test cases shared by the golden-fixture generator and the tests: each case
+is an input deck in the reference's formats (block spec for the mesh tool,
+controls file, field files)."""
+import os
+
+import numpy as np
+
+BOX_CORNERS = lambda lx, ly, lz: [(0, 0, 0), (lx, 0, 0), (lx, 0, lz), (0, 0, lz),  # noqa: E731
+                                  (0, ly, 0), (lx, ly, 0), (lx, ly, lz), (0, ly, lz)]
+
+
+def _spec(case):
+    lx, ly, lz = case["size"]
+    nx, ny, nz = case["ndiv"]
+    pts = "\n".join("%r %r %r" % c for c in BOX_CORNERS(lx, ly, lz))
+    spacing = case.get("spacing", "linear 3{%d %d %d}" % (nx, ny, nz))
+    lines = ["8", "{", pts, "}", "8{0 1 5 4 3 2 6 7} " + spacing]
+    for name, groups in case["patches"]:
+        lines.append(name + " " + " ".join("4{%s}" % " ".join(map(str, g)) for g in groups))
+    return "\n".join(lines) + "\n"
+
+
+PATCHES_2D = [("top", [(3, 2, 6, 7)]), ("bottom", [(0, 1, 5, 4)]),
+              ("delete", [(0, 1, 2, 3), (4, 5, 6, 7)]), ("sides", [(1, 5, 6, 2), (0, 4, 7, 3)])]
+PATCHES_3D = [("top", [(3, 2, 6, 7)]), ("bottom", [(0, 1, 5, 4)]), ("front", [(0, 1, 2, 3)]),
+              ("back", [(4, 5, 6, 7)]), ("sides", [(1, 5, 6, 2), (0, 4, 7, 3)])]
+PATCHES_BOX = [("xmin", [(0, 4, 7, 3)]), ("xmax", [(1, 5, 6, 2)]), ("ymin", [(0, 1, 2, 3)]),
+               ("ymax", [(4, 5, 6, 7)]), ("zmin", [(0, 1, 5, 4)]), ("zmax", [(3, 2, 6, 7)])]
+
+
+def patch_names(case):
+    return sorted(n for n, _ in case["patches"] if n != "delete")
+
+
+def euler_controls(case):
+    o = case["order"]
+    return """general
+{
+    solver                       euler
+    mesh                         grid
+    state                        TRANSIENT
+    start_step                   0
+    end_step                     %d
+    write_interval               1
+    dt                           %r
+    convection_scheme            RUSANOV
+    time_scheme                  BDF1
+    method                       PCG
+    preconditioner               DIAG
+    gravity                      0 0 -9.80606
+    npx                          %d
+    npy                          %d
+    npz                          %d
+}
+euler
+{
+    diffusion                    NO
+    buoyancy                     YES
+}
+""" % (max(case.get("check_steps", [1])), case.get("dt", 0.005), o[0], o[1], o[2])
+
+
+def field_file(ncomp, gens, bcs):
+    body = ["size %d" % ncomp, "internal %d" % len(gens), "{"] + ["    " + g for g in gens] + ["}"]
+    body += ["boundary %d" % len(bcs), "{"]
+    for name, kind in bcs:
+        body += ["    %s {" % name, "        type %s" % kind, "    }"]
+    body += ["}"]
+    return "\n".join(body) + "\n"
+
+
+def write_euler_case(d, case):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "spec"), "w") as f:
+        f.write(_spec(case))
+    with open(os.path.join(d, "controls"), "w") as f:
+        f.write(euler_controls(case))
+    names = patch_names(case)
+    fields = {"T0.txt": (1, case.get("T", ["uniform 0"]), "NEUMANN"),
+              "U0.txt": (3, ["uniform 0 0 0"], "SYMMETRY"),
+              "p0.txt": (1, ["uniform 0"], "NEUMANN"),
+              "rho0.txt": (1, ["uniform 0"], "NEUMANN")}
+    for fn, (nc, gens, kind) in fields.items():
+        with open(os.path.join(d, fn), "w") as f:
+            f.write(field_file(nc, gens, [(n, kind) for n in names]))
+
+
+def write_potential_case(d, case):
+    os.makedirs(d, exist_ok=True)
+    n = case["ndiv"]
+    c = dict(case, size=(1, 1, 1), patches=PATCHES_BOX, order=(0, 0, 0))
+    with open(os.path.join(d, "spec"), "w") as f:
+        f.write(_spec(c))
+    with open(os.path.join(d, "controls"), "w") as f:
+        f.write("""general
+{
+    solver                       potential
+    mesh                         grid
+    state                        STEADY
+    start_step                   0
+    end_step                     1
+    write_interval               1
+    convection_scheme            HYBRID
+    nonortho_scheme              OVER_RELAXED
+    method                       PCG
+    preconditioner               DIAG
+    tolerance                    %r
+    max_iterations               100000
+    gravity                      0 0 0
+    npx                          0
+    npy                          0
+    npz                          0
+}
+potential
+{
+    n_ORTHO                      0
+}
+""" % case.get("tol", 1e-10))
+    U = potential_velocity(case)
+    names = patch_names(c)
+    with open(os.path.join(d, "U0.txt"), "w") as f:
+        f.write("size 3\ninternal %d\n{\n" % len(U))
+        for u in U:
+            f.write("%.17g %.17g %.17g\n" % tuple(u))
+        f.write("}\n" + field_file(3, [], [(nm, "NEUMANN") for nm in names]).split("}\n", 1)[1])
+    with open(os.path.join(d, "p0.txt"), "w") as f:
+        f.write(field_file(1, ["uniform 0"], [(nm, "DIRICHLET" if nm == "xmin" else "NEUMANN")
+                                              for nm in names]))
+
+
+def potential_velocity(case):
+    """Seeded U ~ uniform[-1,1]^3 per interior cell, in the reference's cell order."""
+    nx, ny, nz = case["ndiv"]
+    rng = np.random.default_rng(case.get("seed", 20261202))
+    return rng.uniform(-1.0, 1.0, size=(nx * ny * nz, 3))
+
+
+# ------------------------------------------------------------------ cases --
+CFG0 = dict(name="cfg0_2d_n4", ndiv=(16, 1, 16), size=(1000, 100, 1000), order=(4, 0, 4),
+            patches=PATCHES_2D, dt=0.005, check_steps=[1, 10, 100],
+            T=["gaussian 0 0.5 500 50 350 250 1000 250"])
+
+EULER_CASES = [
+    CFG0,
+    dict(name="e3_n3", ndiv=(3, 2, 2), size=(3000, 2000, 2000), order=(3, 3, 3),
+         patches=PATCHES_3D, dt=0.01, check_steps=[1, 5], keep_fields=True,
+         T=["gaussian 0 1.0 1500 1000 700 800 800 800"]),
+    dict(name="e3_n7", ndiv=(2, 2, 2), size=(1000, 1000, 500), order=(7, 7, 7),
+         patches=PATCHES_3D, dt=0.002, check_steps=[1, 3],
+         T=["gaussian 0 0.5 400 600 200 300 300 300"]),
+    dict(name="e3_aniso_bubbles", ndiv=(4, 3, 2), size=(1000, 900, 500), order=(2, 3, 4),
+         patches=PATCHES_3D, dt=0.004, check_steps=[2],
+         T=["gaussian 0 0.7 300 300 200 250 250 250", "gaussian 0 0.3 700 600 300 200 200 200",
+            "gaussian 0 1.0 500 450 250 150 150 150"]),
+    dict(name="e3_wall_n3", ndiv=(5, 4, 6), size=(1000, 800, 600), order=(3, 3, 3),
+         spacing="wall 3{5 4 6} 3{2 3 0.5}", patches=PATCHES_3D, dt=0.002, check_steps=[2],
+         T=["gaussian 0 0.5 500 400 300 200 200 200"]),
+]
+
+GEOM_CASES = [dict(c) for c in EULER_CASES] + [
+    dict(name="g2d_n4_small", ndiv=(6, 1, 5), size=(600, 100, 500), order=(4, 0, 4),
+         patches=PATCHES_2D, keep=("FO", "FN", "faceid", "cell_faces")),
+    dict(name="g3d_geometric_n2", ndiv=(5, 4, 6), size=(1000, 800, 600), order=(2, 2, 2),
+         spacing="geometric 3{5 4 6} 3{2 3 0.5}", patches=PATCHES_3D),
+    dict(name="gfv_543", ndiv=(5, 4, 3), size=(1, 1, 1), order=(0, 0, 0), patches=PATCHES_BOX),
+]
+
+POTENTIAL_CASES = [
+    dict(name="pot_8", ndiv=(8, 8, 8), keep_fields=True),
+    dict(name="pot_16", ndiv=(16, 16, 16)),
+    dict(name="pot_12x8x6", ndiv=(12, 8, 6), seed=7),
+]

@@ -1,0 +1,137 @@
+"""ctypes front end of the oracle (oracle/liboracle.so) — test infrastructure.
+
+The oracle is the plain-C CPU restatement of the reference algorithms; it is
+the checker for the HIP path and never part of the product."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+OM_NEUMANN, OM_SYMMETRY, OM_DIRICHLET = 0, 1, 2
+
+
+def build_oracle():
+    src = os.path.join(ROOT, "oracle", "nsem_oracle.c")
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o",
+                           ORACLE_SO, src, "-lm"])
+    return ORACLE_SO
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        _lib = C.CDLL(ORACLE_SO)
+    return _lib
+
+
+P = C.POINTER
+PD = P(C.c_double)
+PU = P(C.c_uint32)
+PI = P(C.c_int32)
+
+
+class OmMesh(C.Structure):
+    _fields_ = [("npx", C.c_int32), ("npy", C.c_int32), ("npz", C.c_int32), ("np", C.c_int32),
+                ("npf", C.c_int32), ("nbcs", C.c_uint32), ("ncells", C.c_uint32),
+                ("nfacets", C.c_uint32), ("nbf", C.c_uint64), ("naf", C.c_uint64),
+                ("FO", PU), ("FN", PU), ("fN", PD), ("fI", PD), ("fD", PD), ("cV", PD),
+                ("cC", PD), ("Jinv", PD), ("psi", PD * 3), ("dpsi", PD * 3),
+                ("allFaces", PU), ("fi0", PU), ("fi1", PU), ("nbnd", C.c_int32),
+                ("boff", PU), ("bfaces", PU)]
+
+
+class OmEuler(C.Structure):
+    _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
+                ("cv", C.c_double), ("g", C.c_double * 3), ("buoyancy", C.c_int32),
+                ("bc_rho", PI), ("bc_U", PI), ("bc_T", PI), ("bc_p", PI),
+                ("val_rho", PD), ("val_U", PD), ("val_T", PD), ("val_p", PD)]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class OracleMesh:
+    """Holds the numpy arrays a Mesh exports and the OmMesh view of them."""
+
+    def __init__(self, mesh):
+        inf = mesh.info
+        self.keep = {}
+        for name in ("FO", "FN", "fN", "fI", "fD", "cV", "cC", "allFaces", "faceIndices0",
+                     "faceIndices1", "boff", "bfaces"):
+            self.keep[name] = np.ascontiguousarray(mesh.array(name))
+        if inf.np > 1:
+            self.keep["Jinv"] = np.ascontiguousarray(mesh.array("Jinv"))
+        else:
+            self.keep["Jinv"] = np.zeros(9)
+        for d in range(3):
+            self.keep[f"psi{d}"] = mesh.array(f"psi{d}")
+            self.keep[f"dpsi{d}"] = mesh.array(f"dpsi{d}")
+        k = self.keep
+        s = OmMesh()
+        s.npx, s.npy, s.npz, s.np, s.npf = inf.npx, inf.npy, inf.npz, inf.np, inf.npf
+        s.nbcs, s.ncells, s.nfacets = inf.nbcs, inf.ncells, inf.nfacets
+        s.nbf, s.naf = inf.nbcs_field, inf.nall_field
+        s.FO, s.FN = _p(k["FO"], PU), _p(k["FN"], PU)
+        s.fN, s.fI, s.fD = _p(k["fN"], PD), _p(k["fI"], PD), _p(k["fD"], PD)
+        s.cV, s.cC, s.Jinv = _p(k["cV"], PD), _p(k["cC"], PD), _p(k["Jinv"], PD)
+        for d in range(3):
+            s.psi[d] = _p(k[f"psi{d}"], PD)
+            s.dpsi[d] = _p(k[f"dpsi{d}"], PD)
+        s.allFaces, s.fi0, s.fi1 = _p(k["allFaces"], PU), _p(k["faceIndices0"], PU), _p(k["faceIndices1"], PU)
+        s.nbnd = len(mesh.boundaries)
+        s.boff, s.bfaces = _p(k["boff"], PU), _p(k["bfaces"], PU)
+        self.s = s
+        self.info = inf
+
+
+def euler_hydrostatic(om, params, T, p):
+    n = om.info.nall_field
+    p_ref, rho_ref, rho = np.empty(n), np.empty(n), np.empty(n)
+    lib().om_euler_hydrostatic(C.byref(om.s), C.byref(params), _p(T, PD), _p(p, PD),
+                               _p(p_ref, PD), _p(rho_ref, PD), _p(rho, PD))
+    return p_ref, rho_ref, rho
+
+
+def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -9.80606),
+                buoyancy=1):
+    """bcs: dict field -> (kinds int32 array, values float64 array)."""
+    e = OmEuler()
+    e.dt, e.P0, e.T0, e.cp, e.cv = dt, P0, T0, cp, cv
+    e.g[:] = list(g)
+    e.buoyancy = buoyancy
+    keep = []
+    for f in ("rho", "U", "T", "p"):
+        kinds, vals = bcs[f]
+        kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        keep += [kinds, vals]
+        setattr(e, "bc_" + f, _p(kinds, PI))
+        setattr(e, "val_" + f, _p(vals, PD))
+    e._keep = keep
+    return e
+
+
+def euler_split_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps):
+    st = lib().om_euler_split_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
+                                    _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))
+    return st
+
+
+def fv_poisson(om, bc_kind, bc_val, U, p, tol, max_it):
+    bc_kind = np.ascontiguousarray(bc_kind, dtype=np.int32)
+    bc_val = np.ascontiguousarray(bc_val, dtype=np.float64)
+    it = C.c_int32()
+    r0, r = C.c_double(), C.c_double()
+    st = lib().om_fv_poisson(C.byref(om.s), _p(bc_kind, PI), _p(bc_val, PD), _p(U, PD), _p(p, PD),
+                             C.c_double(tol), C.c_int32(max_it), C.byref(it), C.byref(r0), C.byref(r))
+    return st, it.value, r0.value, r.value

@@ -1,0 +1,134 @@
+"""Regenerate the golden fixtures from the reference build (oracle/_ref).
+
+Runs in this container only (it needs /root/reference built by
+`make -f oracle/ref.mk`).  For each case it writes the reference's input
+deck into a temporary directory, runs the reference's own executables
+(mesh, geomdump, euler, potential) there, and records:
+
+  * SHA-256 of every loaded-mesh / geometry array the reference produces
+    (geomdump), to pin the mesh pipeline bit for bit;
+  * SHA-256 plus norms of the reference's output fields after N steps
+    (euler: rho U T p; potential: p and the PCG iteration count);
+  * for the small cases, the output arrays themselves (npz), so that a
+    tolerance comparison is possible if a hash ever differs.
+
+Usage:  python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from _cases import EULER_CASES, GEOM_CASES, POTENTIAL_CASES, write_euler_case, write_potential_case  # noqa: E402
+from _fieldio import read_field_bin  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+GEOM_SKIP = {"header", "bnames"}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def read_dump(path):
+    out = {}
+    with open(path, "rb") as f:
+        while True:
+            nm = f.read(32)
+            if len(nm) < 32:
+                break
+            name = nm.split(b"\0")[0].decode()
+            kind = int(np.frombuffer(f.read(4), np.uint32)[0])
+            cnt = int(np.frombuffer(f.read(8), np.uint64)[0])
+            dt = {0: np.uint32, 1: np.float64}[kind]
+            out[name] = np.frombuffer(f.read(cnt * np.dtype(dt).itemsize), dt)
+    return out
+
+
+def run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{cmd} failed:\n{r.stdout}\n{r.stderr}")
+    return r.stdout
+
+
+def geometry_hashes(case, d, keep_arrays):
+    run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+    run([os.path.join(REF, "geomdump"), "controls", "geom.bin"], d)
+    dump = read_dump(os.path.join(d, "geom.bin"))
+    out = {"sha256": {}, "sizes": {}}
+    for k, v in dump.items():
+        if k in GEOM_SKIP:
+            continue
+        if k == "fD" and tuple(case["order"]) != (0, 0, 0):
+            continue  # DG penalty factor: only the implicit DG laplacian uses it
+        out["sha256"][k] = sha(v)
+        out["sizes"][k] = int(v.size)
+    out["bnames"] = bytes(dump["bnames"].astype(np.uint8)).decode().split()
+    arrays = {k: v for k, v in dump.items() if k in keep_arrays}
+    return out, arrays
+
+
+def main():
+    fixtures = {"geometry": {}, "euler": {}, "potential": {}}
+    arrays = {}
+    tmp = tempfile.mkdtemp(prefix="nsem_golden_")
+    try:
+        for case in GEOM_CASES:
+            d = os.path.join(tmp, "g_" + case["name"])
+            write_euler_case(d, case)
+            fx, arr = geometry_hashes(case, d, keep_arrays=case.get("keep", ()))
+            fixtures["geometry"][case["name"]] = fx
+            for k, v in arr.items():
+                arrays[f"geom_{case['name']}_{k}"] = v
+            print("geometry", case["name"], len(fx["sha256"]), "arrays")
+        for case in EULER_CASES:
+            d = os.path.join(tmp, "e_" + case["name"])
+            write_euler_case(d, case)
+            run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+            log = run([os.path.join(REF, "euler"), "controls"], d)
+            rec = {"steps": {}}
+            for s in case["check_steps"]:
+                rec["steps"][str(s)] = {}
+                for name in ("rho", "U", "T", "p"):
+                    v = read_field_bin(os.path.join(d, f"{name}{s}.bin")).reshape(-1)
+                    rec["steps"][str(s)][name] = {"sha256": sha(v), "l2": float(np.linalg.norm(v)),
+                                                  "max": float(np.max(np.abs(v))), "n": int(v.size)}
+                    if case.get("keep_fields"):
+                        arrays[f"euler_{case['name']}_{name}{s}"] = v
+            rec["log_tail"] = log.strip().splitlines()[-3:]
+            fixtures["euler"][case["name"]] = rec
+            print("euler", case["name"], "ok")
+        for case in POTENTIAL_CASES:
+            d = os.path.join(tmp, "p_" + case["name"])
+            write_potential_case(d, case)
+            run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+            log = run([os.path.join(REF, "potential"), "controls"], d)
+            line = [ln for ln in log.splitlines() if "PCG" in ln][-1]
+            toks = line.split()
+            it = int(toks[[t.endswith("Iterations") for t in toks].index(True) + 1])
+            res = float(toks[toks.index("Final") + 2])
+            p = read_field_bin(os.path.join(d, "p1.bin")).reshape(-1)
+            fixtures["potential"][case["name"]] = {
+                "iterations": it, "final_residual": res, "log_line": line.split("]", 1)[-1].strip(),
+                "p_sha256": sha(p), "p_l2": float(np.linalg.norm(p)), "n": int(p.size)}
+            if case.get("keep_fields"):
+                arrays[f"potential_{case['name']}_p"] = p
+            print("potential", case["name"], it, res)
+    finally:
+        shutil.rmtree(tmp)
+    with open(os.path.join(HERE, "reference_fixtures.json"), "w") as f:
+        json.dump(fixtures, f, indent=1, sort_keys=True)
+    np.savez_compressed(os.path.join(HERE, "reference_arrays.npz"), **arrays)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,111 @@
+"""The oracle (oracle/nsem_oracle.c) is pinned to the reference: starting
+from the same decks, its Euler split steps and its FV Poisson PCG reproduce
+the reference build's output fields bit for bit (SHA-256 of each field, with
+a norm check for diagnosis), and the same PCG iteration counts."""
+import os
+
+import numpy as np
+import pytest
+
+from _cases import EULER_CASES, POTENTIAL_CASES, write_potential_case
+from _euler_setup import bc_table, load_case
+from _golden import arrays, fixtures, sha
+from nebulasem_amd import Mesh, read_field
+
+FX = fixtures()
+
+
+def run_oracle_euler(oracle, tmp_path, case):
+    mesh, f = load_case(tmp_path, case)
+    om = oracle.OracleMesh(mesh)
+    prm = oracle.make_params(case["dt"], bc_table(f))
+    T = f["T"][0].reshape(-1).copy()
+    p = f["p"][0].reshape(-1).copy()
+    U = f["U"][0].reshape(-1).copy()
+    p_ref, rho_ref, rho = oracle.euler_hydrostatic(om, prm, T, p)
+    return mesh, om, prm, p_ref, rho_ref, rho, U, T, p
+
+
+@pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
+def test_oracle_euler_matches_reference(oracle, tmp_path, case):
+    mesh, om, prm, p_ref, rho_ref, rho, U, T, p = run_oracle_euler(oracle, tmp_path, case)
+    nb = mesh.info.nbcs_field
+    done = 0
+    fx = FX["euler"][case["name"]]["steps"]
+    for s in case["check_steps"]:
+        assert oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, s - done) == 0
+        done = s
+        for name, arr, nc in (("rho", rho, 1), ("U", U, 3), ("T", T, 1), ("p", p, 1)):
+            v = arr[:nb * nc]
+            ref = fx[str(s)][name]
+            assert abs(np.linalg.norm(v) - ref["l2"]) <= 1e-12 * max(ref["l2"], 1e-300), (s, name)
+            assert sha(v) == ref["sha256"], f"step {s} {name}: not bit-identical to the reference"
+
+
+def test_oracle_euler_small_case_arrays(oracle, tmp_path):
+    case = [c for c in EULER_CASES if c.get("keep_fields")][0]
+    mesh, om, prm, p_ref, rho_ref, rho, U, T, p = run_oracle_euler(oracle, tmp_path, case)
+    arr = arrays()
+    nb = mesh.info.nbcs_field
+    s = case["check_steps"][-1]
+    oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, s)
+    for name, a, nc in (("rho", rho, 1), ("U", U, 3), ("T", T, 1), ("p", p, 1)):
+        assert np.array_equal(a[:nb * nc], arr[f"euler_{case['name']}_{name}{s}"]), name
+
+
+def potential_inputs(tmp_path, case):
+    d = os.path.join(str(tmp_path), case["name"])
+    write_potential_case(d, case)
+    mesh = Mesh.from_spec(os.path.join(d, "spec"), order=(0, 0, 0))
+    U, kU, vU = read_field(mesh, os.path.join(d, "U0.txt"), 3)
+    p, kp, vp = read_field(mesh, os.path.join(d, "p0.txt"), 1)
+    # ghost values as applyExplicitBCs sets them after reading
+    FO, FN = mesh.array("FO"), mesh.array("FN")
+    boff, bf = mesh.array("boff"), mesh.array("bfaces")
+    U, p = U.copy(), p.reshape(-1).copy()
+    for b in range(len(mesh.boundaries)):
+        fs = bf[boff[b]:boff[b + 1]]
+        U[FN[fs]] = U[FO[fs]] if kU[b] == 0 else vU[b]
+        p[FN[fs]] = vp[b, 0] if kp[b] == 2 else p[FO[fs]]
+    return mesh, U, p, kp, vp
+
+
+@pytest.mark.parametrize("case", POTENTIAL_CASES, ids=[c["name"] for c in POTENTIAL_CASES])
+def test_oracle_poisson_matches_reference(oracle, tmp_path, case):
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, case)
+    om = oracle.OracleMesh(mesh)
+    st, it, r0, r = oracle.fv_poisson(om, kp, vp.reshape(-1), U.reshape(-1).copy(), p, 1e-10, 100000)
+    ref = FX["potential"][case["name"]]
+    assert st == 0
+    assert it == ref["iterations"]
+    assert abs(r - ref["final_residual"]) <= 1e-5 * ref["final_residual"]
+    assert sha(p[:mesh.info.nbcs]) == ref["p_sha256"]
+
+
+def test_poisson_against_dense_solve(oracle, tmp_path):
+    """Brute force: the FV operator as a dense matrix; the PCG result solves it."""
+    case = dict(name="pot_4", ndiv=(4, 3, 5), seed=3)
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, case)
+    om = oracle.OracleMesh(mesh)
+    st, it, r0, r = oracle.fv_poisson(om, kp, vp.reshape(-1), U.reshape(-1).copy(), p, 1e-14, 1000)
+    n = mesh.info.nbcs
+    FO, FN, fD = mesh.array("FO"), mesh.array("FN"), mesh.array("fD")
+    fN = mesh.array("fN").reshape(-1, 3)
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    xmin = mesh.boundaries.index("xmin")
+    boff, bf = mesh.array("boff"), mesh.array("bfaces")
+    dir_faces = set(bf[boff[xmin]:boff[xmin + 1]].tolist())
+    Uc = U.reshape(-1, 3)
+    for k in range(mesh.info.nfacets):
+        c1, c2 = FO[k], FN[k]
+        flux = 0.5 * (Uc[c1] + Uc[c2]) @ fN[k] if c2 < n else Uc[c2] @ fN[k]
+        b[c1] -= flux
+        if c2 < n:
+            b[c2] += flux
+            A[c1, c1] += fD[k]; A[c2, c2] += fD[k]
+            A[c1, c2] -= fD[k]; A[c2, c1] -= fD[k]
+        elif k in dir_faces:
+            A[c1, c1] += fD[k]  # Dirichlet p = 0 on xmin
+    x = np.linalg.solve(A, b)
+    assert np.allclose(p[:n], x, rtol=1e-9, atol=1e-11 * np.abs(x).max())
