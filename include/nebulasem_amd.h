@@ -97,6 +97,78 @@ enum { NSEM_BC_NEUMANN = 0, NSEM_BC_SYMMETRY = 1, NSEM_BC_DIRICHLET = 2 };
 int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
                     double* values, int32_t* bc_kind, double* bc_value);
 
+/* -------------------------------------------------------------- DG Euler --
+ * Device-resident dGSEM compressible Euler problem on one GPU (gfx950).
+ * Constants as apps/utils/properties.cpp:14-33 (P0 T0 cp cv) and the controls
+ * keys dt / gravity; buoyancy as the euler block (euler.cpp:104-131).      */
+typedef struct {
+    double dt;
+    double P0, T0, cp, cv;
+    double gravity[3];
+    int32_t buoyancy;
+} nsem_euler_params;
+typedef struct nsem_dg nsem_dg;
+
+/* Upload geometry and build the face tables.  bc_kind[4*nb] gives, per mesh
+ * boundary patch, the kinds of (rho, U, T, p) (NSEM_BC_*; scalars must be
+ * NEUMANN or SYMMETRY), bc_value[6*nb] the values (rho, Ux, Uy, Uz, T, p).
+ * Fails with NSEM_EDEVICE when no HIP device is present. */
+int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const int32_t* bc_kind,
+                   const double* bc_value, nsem_dg** out);
+void nsem_dg_destroy(nsem_dg* d);
+/* State as the reference stores it, interior nodes only (host arrays):
+ * rho, U (xyz per node), T = theta', p = p'. */
+int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p);
+int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
+/* Hydrostatic reference p_ref, rho_ref at the interior nodes (euler.cpp:121-123). */
+int nsem_dg_get_background(const nsem_dg* d, double* p_ref, double* rho_ref);
+/* Initial density from theta' and p' as euler.cpp:136-149. */
+int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* p, double* rho);
+/* nsteps of the reference's split forward-Euler update (euler.cpp:179-287,
+ * diffusion NO): rho, then p, then U and theta, each a Rusanov dGSEM residual
+ * with the explicit ddt/Solve update (field.h:3740-3920, solve.cpp:563-570). */
+int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps);
+int nsem_dg_synchronize(nsem_dg* d);
+
+/* ------------------------------------------------------------ FV Poisson --
+ * Pressure Poisson of the FV solvers: Solve(lap(p, mu, true) == divf(U))
+ * (apps/potential/potential.cpp:58-64; apps/piso/piso.cpp:210-213 with
+ * mu = rmu) with the DIAG (Jacobi) preconditioned CG of SolveT
+ * (src/solvers/solve.cpp:39-559).  Mesh loaded with order {0,0,0}.        */
+typedef struct nsem_fv nsem_fv;
+typedef struct {
+    int32_t iterations;        /* CG iterations (all inner ones in mode 2)   */
+    int32_t outer_iterations;  /* refinement steps (mode 2), else 1          */
+    int32_t breakdown;
+    int32_t reserved;
+    double res0_ref;           /* initial residual as the reference logs it  */
+    double res_ref;            /* final sqrt(|sum z^2| / |sum x^2|)          */
+    double res_true;           /* final |b - A x| / |b| in FP64               */
+    double seconds;            /* device time of the solve                    */
+} nsem_fv_result;
+/* Assemble lap(p, mu, true) with boundary conditions folded in
+ * (field.h:3571-3615, 2541-2582).  mu: per cell including ghosts (NULL = 1);
+ * bc_kind/bc_value per boundary patch (NEUMANN or DIRICHLET). */
+int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_kind,
+                   const double* bc_value, nsem_fv** out);
+void nsem_fv_destroy(nsem_fv* fv);
+int nsem_fv_info(const nsem_fv* fv, uint64_t* ncells, int32_t* structured);
+/* Source = BC part - divf(U) (field.h:3470-3477): U interior cells xyz; the
+ * velocity boundary conditions per patch (NEUMANN, SYMMETRY or DIRICHLET). */
+int nsem_fv_set_rhs_divergence(nsem_fv* fv, const double* U, const int32_t* bc_kind,
+                               const double* bc_value, int32_t nboundaries);
+int nsem_fv_set_rhs(nsem_fv* fv, const double* su);
+int nsem_fv_get_rhs(nsem_fv* fv, double* su);
+/* y = A x for interior cells (the reference's mul, field.h:2369-2420). */
+int nsem_fv_apply(nsem_fv* fv, const double* x, double* y);
+/* Solve A x = b in place (x: initial guess in, solution out).  mode 0: FP64
+ * PCG stopped on the reference's criterion sqrt(|sum z^2|/|sum x^2|) <= tol
+ * (solve.cpp:11-35, 441-444); mode 1: FP64 PCG to |b-Ax|/|b| <= tol; mode 2:
+ * FP32 inner PCG with FP64 iterative refinement to |b-Ax|/|b| <= tol.
+ * Returns NSEM_ENOTCONVERGED when max_it is reached first. */
+int nsem_fv_solve(nsem_fv* fv, double* x, int32_t mode, double tol, int32_t max_it,
+                  nsem_fv_result* res);
+
 #ifdef __cplusplus
 }
 #endif

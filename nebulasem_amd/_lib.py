@@ -138,3 +138,163 @@ def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
                                 vals.ctypes.data_as(C.c_void_p), kinds.ctypes.data_as(C.c_void_p),
                                 bv.ctypes.data_as(C.c_void_p)))
     return vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp)
+
+
+class EulerParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
+                ("cv", C.c_double), ("gravity", C.c_double * 3), ("buoyancy", C.c_int32)]
+
+
+BC_NEUMANN, BC_SYMMETRY, BC_DIRICHLET = 0, 1, 2
+
+
+def euler_params(dt, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, gravity=(0.0, 0.0, -9.80606),
+                 buoyancy=1):
+    p = EulerParams()
+    p.dt, p.P0, p.T0, p.cp, p.cv = dt, P0, T0, cp, cv
+    p.gravity[:] = list(gravity)
+    p.buoyancy = buoyancy
+    return p
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DGEuler:
+    """Device-resident dGSEM Euler problem (nsem_dg_* in the C-ABI)."""
+
+    def __init__(self, mesh, params, bc_kind, bc_value):
+        """bc_kind: (nb, 4) kinds of rho U T p; bc_value: (nb, 6) rho Ux Uy Uz T p."""
+        self.mesh = mesh
+        self.params = params
+        k = np.ascontiguousarray(bc_kind, dtype=np.int32).reshape(-1)
+        v = np.ascontiguousarray(bc_value, dtype=np.float64).reshape(-1)
+        h = C.c_void_p()
+        check(lib().nsem_dg_create(mesh.handle, C.byref(params), k.ctypes.data_as(C.c_void_p),
+                                   v.ctypes.data_as(C.c_void_p), C.byref(h)))
+        self._h = h
+        self.nb = mesh.info.nbcs_field
+
+    @classmethod
+    def from_fields(cls, mesh, params, fields):
+        """fields: dict name -> read_field() result for rho, U, T, p."""
+        nb = len(mesh.boundaries)
+        kinds = np.stack([fields[f][1] for f in ("rho", "U", "T", "p")], axis=1)
+        vals = np.concatenate([fields["rho"][2], fields["U"][2], fields["T"][2], fields["p"][2]], axis=1)
+        assert vals.shape == (nb, 6)
+        return cls(mesh, params, kinds, vals)
+
+    def set_state(self, rho, U, T, p):
+        nb = self.nb
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (rho, U, T, p)]
+        assert a[0].size == nb and a[1].size == 3 * nb and a[2].size == nb and a[3].size == nb
+        check(lib().nsem_dg_set_state(self._h, *[_pd(x) for x in a]))
+        self._keep = a
+
+    def get_state(self):
+        nb = self.nb
+        rho, U, T, p = np.empty(nb), np.empty(3 * nb), np.empty(nb), np.empty(nb)
+        check(lib().nsem_dg_get_state(self._h, _pd(rho), _pd(U), _pd(T), _pd(p)))
+        return rho, U, T, p
+
+    def background(self):
+        pr, rr = np.empty(self.nb), np.empty(self.nb)
+        check(lib().nsem_dg_get_background(self._h, _pd(pr), _pd(rr)))
+        return pr, rr
+
+    def initial_density(self, T, p):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        rho = np.empty(self.nb)
+        check(lib().nsem_euler_initial_density(self._h, _pd(T), _pd(p), _pd(rho)))
+        return rho
+
+    def split_steps(self, n):
+        check(lib().nsem_dg_split_steps(self._h, int(n)))
+
+    def synchronize(self):
+        check(lib().nsem_dg_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nsem_dg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FvResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("breakdown", C.c_int32),
+                ("reserved", C.c_int32), ("res0_ref", C.c_double), ("res_ref", C.c_double),
+                ("res_true", C.c_double), ("seconds", C.c_double)]
+
+
+class FVPoisson:
+    """Device FV pressure Poisson (nsem_fv_* in the C-ABI)."""
+
+    MODE_REFERENCE, MODE_TRUE_RESIDUAL, MODE_MIXED = 0, 1, 2
+
+    def __init__(self, mesh, p_bc_kind, p_bc_value, mu=None):
+        k = np.ascontiguousarray(p_bc_kind, dtype=np.int32)
+        v = np.ascontiguousarray(p_bc_value, dtype=np.float64).reshape(-1)
+        mu_p = None
+        if mu is not None:
+            self._mu = np.ascontiguousarray(mu, dtype=np.float64)
+            mu_p = _pd(self._mu)
+        h = C.c_void_p()
+        check(lib().nsem_fv_create(mesh.handle, mu_p, k.ctypes.data_as(C.c_void_p),
+                                   v.ctypes.data_as(C.c_void_p), C.byref(h)))
+        self._h = h
+        self.mesh = mesh
+        n = C.c_uint64()
+        st = C.c_int32()
+        check(lib().nsem_fv_info(self._h, C.byref(n), C.byref(st)))
+        self.n = n.value
+        self.structured = bool(st.value)
+
+    def set_rhs_divergence(self, U, bc_kind, bc_value):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
+        assert U.size == 3 * self.n
+        k = np.ascontiguousarray(bc_kind, dtype=np.int32)
+        v = np.ascontiguousarray(bc_value, dtype=np.float64).reshape(-1)
+        check(lib().nsem_fv_set_rhs_divergence(self._h, _pd(U), k.ctypes.data_as(C.c_void_p), _pd(v),
+                                               len(k)))
+
+    def set_rhs(self, su):
+        su = np.ascontiguousarray(su, dtype=np.float64)
+        check(lib().nsem_fv_set_rhs(self._h, _pd(su)))
+
+    def rhs(self):
+        out = np.empty(self.n)
+        check(lib().nsem_fv_get_rhs(self._h, _pd(out)))
+        return out
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        check(lib().nsem_fv_apply(self._h, _pd(x), _pd(y)))
+        return y
+
+    def solve(self, x0=None, mode=0, tol=1e-10, max_it=100000, raise_on_error=True):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        res = FvResult()
+        rc = lib().nsem_fv_solve(self._h, _pd(x), int(mode), C.c_double(tol), int(max_it), C.byref(res))
+        if raise_on_error:
+            check(rc)
+        return x, res, rc
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nsem_fv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,139 @@
+// Device-side data of a DG Euler problem and the helpers its kernels share.
+//
+// Layout in HBM (one GPU, interior nodes only; ghost values are never stored,
+// they are produced from the owner state by the boundary condition when a
+// face flux needs them):
+//   node n = element*NP + i*NPY*NPZ + j*NPZ + k       (as the reference)
+//   state      SoA double arrays  rho, ux, uy, uz, T (theta'), p (p')
+//   metric     SoA double JcV[9][nb] = Jinv*cV, cV[nb]
+//   background SoA double p_ref[nb], rho_ref[nb]
+//   faces      per element, per face group g (0..5): EFace {facet, other, meta}
+//              per facet: area normal gFN (3 doubles)
+//   basis      per direction: LGL weights w[n], derivative D[s*n+i] = l'_i(x_s)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nsem {
+
+constexpr int kMaxN = 16;  // LGL points per direction supported
+
+// meta bits: [0:3) other face group, [3] self is owner, [4] face present,
+// [5] other is a ghost (boundary), [8:24) boundary patch index
+struct EFace {
+    uint32_t facet;
+    uint32_t other;
+    uint32_t meta;
+};
+
+enum BcKind : int32_t { BC_NEUMANN = 0, BC_SYMMETRY = 1, BC_DIRICHLET = 2 };
+
+constexpr int kMaxPatches = 64;
+
+struct BcTable {
+    int32_t kind[4][kMaxPatches];     // rho, U, T, p
+    double value[6][kMaxPatches];     // rho, Ux, Uy, Uz, T, p
+};
+
+struct DgDev {
+    int npx, npy, npz, np;
+    uint32_t nelem;                   // interior elements
+    uint64_t nb;                      // interior nodes
+    double dt, P0, T0, R, gamma, cp;
+    double g[3];
+    int buoyancy;
+    const double* JcV;                // [9][nb]
+    const double* cV;                 // [nb]
+    const double* p_ref;              // [nb]
+    const double* rho_ref;            // [nb]
+    const EFace* eface;               // [nelem*6]
+    const double* gFN;                // [nfacets*3]
+    const double* D[3];               // [n*n] per direction
+    const double* w[3];               // [n] per direction
+    const BcTable* bc;
+};
+
+__device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,
+                                        double b2) {
+    // right-nested like the reference's unrolled vector dot product
+    double r = a2 * b2 + 0.0;
+    r = a1 * b1 + r;
+    return a0 * b0 + r;
+}
+
+// SYMMETRY ghost velocity: tangential projection rescaled to |U|
+// (src/tensor/tensor.h:483-492).
+__device__ __forceinline__ void sym_vector(const double* p, const double* n, double* r) {
+    double mn = sqrt(dot3r(n[0], n[1], n[2], n[0], n[1], n[2]));
+    double e0 = n[0] / mn, e1 = n[1] / mn, e2 = n[2] / mn;
+    double A0 = 1 - e0 * e0, A1 = 1 - e1 * e1, A2 = 1 - e2 * e2;
+    double A3 = 0 - e0 * e1, A4 = 0 - e1 * e2, A5 = 0 - e0 * e2;
+    r[0] = A0 * p[0] + A3 * p[1] + A5 * p[2];
+    r[1] = A3 * p[0] + A1 * p[1] + A4 * p[2];
+    r[2] = A5 * p[0] + A4 * p[1] + A2 * p[2];
+    double mr = sqrt(dot3r(r[0], r[1], r[2], r[0], r[1], r[2]));
+    if (fabs(mr) <= 1e-7) return;
+    double s = sqrt(dot3r(p[0], p[1], p[2], p[0], p[1], p[2])) / mr;
+    r[0] *= s;
+    r[1] *= s;
+    r[2] *= s;
+}
+
+// Ghost velocity of a boundary face node (applyExplicitBCs, field.h:2660-2690):
+// SYMMETRY reflects to the tangential plane, DIRICHLET is the patch value,
+// NEUMANN copies (|dv| is zero for DG ghost nodes).  Scalars are always
+// zero-gradient copies on this path.
+__device__ __forceinline__ void ghost_velocity(const BcTable& bt, int pi, const double* own,
+                                               const double* n, double* out) {
+    const int kind = bt.kind[1][pi];
+    if (kind == BC_SYMMETRY) {
+        sym_vector(own, n, out);
+    } else if (kind == BC_DIRICHLET) {
+        out[0] = bt.value[1][pi]; out[1] = bt.value[2][pi]; out[2] = bt.value[3][pi];
+    } else {
+        out[0] = own[0] + bt.value[1][pi] * 0.0;
+        out[1] = own[1] + bt.value[2][pi] * 0.0;
+        out[2] = own[2] + bt.value[3][pi] * 0.0;
+    }
+}
+
+// Face membership of node (i,j,k) for face group g (dg.cpp face_map).
+__device__ __forceinline__ bool on_face(int g, int i, int j, int k, int NX, int NY, int NZ) {
+    switch (g) {
+        case 0: return k == 0;
+        case 1: return k == NZ - 1;
+        case 2: return j == 0;
+        case 3: return j == NY - 1;
+        case 4: return i == 0;
+        default: return i == NX - 1;
+    }
+}
+
+// Face-local coordinates (a,b) of node (i,j,k) on face group g.
+__device__ __forceinline__ void face_ab(int g, int i, int j, int k, int& a, int& b) {
+    if (g < 2) { a = i; b = j; }
+    else if (g < 4) { a = i; b = k; }
+    else { a = j; b = k; }
+}
+
+// Node of element `cell` on face group g at face-local (a,b).
+__device__ __forceinline__ uint64_t face_node(uint32_t cell, int g, int a, int b, int NX, int NY,
+                                              int NZ) {
+    const int fm[6] = {0, NZ - 1, 0, NY - 1, 0, NX - 1};
+    int v = fm[g];
+    int i, j, k;
+    if (g < 2) { i = a; j = b; k = v; }
+    else if (g < 4) { i = a; j = v; k = b; }
+    else { i = v; j = a; k = b; }
+    return (uint64_t)cell * (NX * NY * NZ) + (uint64_t)(i * NY * NZ + j * NZ + k);
+}
+
+// Facet-node weight w_a w_b / 4 in the owner's face layout (dg.cpp:368-404).
+__device__ __forceinline__ double face_weight(const DgDev& d, int owner_g, int a, int b) {
+    if (owner_g < 2) return d.w[0][a] * d.w[1][b] / 4;
+    if (owner_g < 4) return d.w[0][a] * d.w[2][b] / 4;
+    return d.w[1][a] * d.w[2][b] / 4;
+}
+
+}  // namespace nsem

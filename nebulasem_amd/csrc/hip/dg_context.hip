@@ -1,0 +1,292 @@
+// DG Euler problem on the device: set-up from a loaded mesh, state transfer,
+// and the C-ABI entry points that run steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../mesh_access.hpp"
+#include "../status.hpp"
+#include "dg_common.hpp"
+#include "dg_kernels.hpp"
+#include "nebulasem_amd.h"
+
+using namespace nsem;
+
+#define HIP_TRY(x)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            return nsem_fail(NSEM_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t n) { bytes = n; return hipMalloc(&p, n ? n : 8); }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+}  // namespace
+
+struct nsem_dg {
+    DgDev dev{};
+    nsem_euler_params prm{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t nb = 0;
+    // geometry
+    DevBuf JcV, cV, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
+    // double-buffered state: rho ux uy uz T p
+    DevBuf st[2][6];
+    int cur = 0;
+    // profiling of the dominant kernels
+    bool profile = false;
+    double prof_ms[4] = {0, 0, 0, 0};
+    uint64_t prof_n[4] = {0, 0, 0, 0};
+    std::vector<double> host_p_ref, host_rho_ref;
+    ~nsem_dg() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const int32_t* bc_kind,
+                   const double* bc_value, nsem_dg** out) {
+    if (!mesh || !prm || !bc_kind || !bc_value || !out) return nsem_fail(NSEM_EINVAL, "null argument");
+    const Geometry& g = nsem_mesh_geometry(mesh);
+    const LoadedMesh& lm = nsem_mesh_loaded(mesh);
+    const std::vector<std::string>& names = nsem_mesh_boundary_names(mesh);
+    if (g.np < 2) return nsem_fail(NSEM_EINVAL, "DG path needs polynomial order >= 1");
+    if (g.np > 1024) return nsem_fail(NSEM_EUNSUPPORTED, "at most 1024 nodes per element");
+    if (names.size() > (size_t)kMaxPatches) return nsem_fail(NSEM_EUNSUPPORTED, "too many boundary patches");
+    if (12 * (size_t)g.np * sizeof(double) > 65536)
+        return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the LDS staging (order <= 7 in 3D)");
+    if (!(prm->dt > 0)) return nsem_fail(NSEM_EINVAL, "dt must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return nsem_fail(NSEM_EDEVICE, "no HIP device available");
+
+    auto d = std::make_unique<nsem_dg>();
+    d->prm = *prm;
+    const uint64_t nb = g.nbcs_field;
+    d->nb = nb;
+    HIP_TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&d->ev0));
+    HIP_TRY(hipEventCreate(&d->ev1));
+    hipStream_t s = d->stream;
+
+    // boundary table: scalars must be zero-gradient on this path
+    BcTable bt;
+    std::memset(&bt, 0, sizeof(bt));
+    for (size_t b = 0; b < names.size(); b++) {
+        for (int f = 0; f < 4; f++) {
+            int k = bc_kind[4 * b + f];
+            if (k < 0) return nsem_fail(NSEM_EINVAL, "no boundary condition for patch " + names[b]);
+            if (f != 1 && k == BC_DIRICHLET)
+                return nsem_fail(NSEM_EUNSUPPORTED, "DIRICHLET on rho/T/p is not supported by the DG path");
+            bt.kind[f][b] = k;
+        }
+        for (int v = 0; v < 6; v++) bt.value[v][b] = bc_value[6 * b + v];
+    }
+    // facet -> patch
+    std::vector<int32_t> patch_of(g.nfacets, -1);
+    {
+        int b = 0;
+        for (const auto& kv : lm.boundaries) {
+            for (Int f : kv.second) patch_of[f] = b;
+            b++;
+        }
+    }
+    // element face table
+    std::vector<EFace> ef((size_t)g.nbcs * 6);
+    for (Int ci = 0; ci < g.nbcs; ci++) {
+        const IntVec& c = lm.cells[ci];
+        for (size_t mm = 0; mm < c.size(); mm++) {
+            Int fi = c[mm];
+            Int grp = lm.faceid[ci][mm];
+            if (grp > 5) return nsem_fail(NSEM_EUNSUPPORTED, "bad face group");
+            EFace& e = ef[(size_t)ci * 6 + grp];
+            if ((e.meta >> 4) & 1) return nsem_fail(NSEM_EUNSUPPORTED, "non-conforming faces are not supported yet");
+            bool own = lm.FOC[fi] == ci;
+            Int other = own ? lm.FNC[fi] : lm.FOC[fi];
+            Int og = grp ^ 1;
+            bool ghost = other >= g.nbcs;
+            if (!ghost) {
+                const IntVec& cn = lm.cells[other];
+                bool found = false;
+                for (size_t t = 0; t < cn.size(); t++)
+                    if (cn[t] == fi) { og = lm.faceid[other][t]; found = true; break; }
+                if (!found) return nsem_fail(NSEM_EINVAL, "inconsistent face connectivity");
+                // both sides must index the face with the same extents
+                auto ext = [&](Int gg, Int& na, Int& nb2) {
+                    if (gg < 2) { na = g.npx; nb2 = g.npy; }
+                    else if (gg < 4) { na = g.npx; nb2 = g.npz; }
+                    else { na = g.npy; nb2 = g.npz; }
+                };
+                Int a0, b0, a1, b1;
+                ext(grp, a0, b0);
+                ext(og, a1, b1);
+                if (a0 != a1 || b0 != b1)
+                    return nsem_fail(NSEM_EUNSUPPORTED, "face extents differ across an interior face");
+            } else {
+                if (!own) return nsem_fail(NSEM_EINVAL, "boundary face not owned by an interior cell");
+                if (patch_of[fi] < 0) return nsem_fail(NSEM_EINVAL, "boundary face outside every patch");
+            }
+            e.facet = fi;
+            e.other = other;
+            e.meta = og | (own ? 8u : 0u) | 16u | (ghost ? 32u : 0u) |
+                     ((uint32_t)(ghost ? patch_of[fi] : 0) << 8);
+        }
+    }
+    // metric, volumes, background (euler.cpp:104-131)
+    std::vector<double> JcV(9 * nb), cV(nb), pref(nb), rref(nb);
+    const double R = prm->cp - prm->cv, gam = prm->cp / prm->cv;
+    for (uint64_t n = 0; n < nb; n++) {
+        for (int c = 0; c < 9; c++) JcV[c * nb + n] = g.Jinv[n][c] * g.cV[n];
+        cV[n] = g.cV[n];
+        const V3& x = g.cC[n];
+        if (prm->buoyancy) {
+            double gh = prm->gravity[2] * x[2] + 0.0;
+            gh = prm->gravity[1] * x[1] + gh;
+            gh = prm->gravity[0] * x[0] + gh;
+            pref[n] = prm->P0 * std::pow(1.0 + gh / (prm->cp * prm->T0), prm->cp / R);
+            rref[n] = (prm->P0 / (R * prm->T0)) * std::pow(pref[n] / prm->P0, 1 / gam);
+        } else {
+            pref[n] = prm->P0;
+            rref[n] = prm->P0 / (R * prm->T0);
+        }
+    }
+    std::vector<double> gfn(3 * (size_t)g.nfacets);
+    for (Int f = 0; f < g.nfacets; f++)
+        for (int c = 0; c < 3; c++) gfn[3 * (size_t)f + c] = lm.fn[f][c];
+    hipError_t e;
+    if ((e = upload(d->JcV, JcV, s)) != hipSuccess || (e = upload(d->cV, cV, s)) != hipSuccess ||
+        (e = upload(d->p_ref, pref, s)) != hipSuccess || (e = upload(d->rho_ref, rref, s)) != hipSuccess ||
+        (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess)
+        return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+    for (int dir = 0; dir < 3; dir++) {
+        if ((e = upload(d->D[dir], g.basis[dir].dpsi, s)) != hipSuccess ||
+            (e = upload(d->w[dir], g.basis[dir].w, s)) != hipSuccess)
+            return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(d->bc.alloc(sizeof(BcTable)));
+    HIP_TRY(hipMemcpyAsync(d->bc.p, &bt, sizeof(BcTable), hipMemcpyHostToDevice, s));
+    for (int a = 0; a < 2; a++)
+        for (int v = 0; v < 6; v++) {
+            HIP_TRY(d->st[a][v].alloc(nb * sizeof(double)));
+            HIP_TRY(hipMemsetAsync(d->st[a][v].p, 0, nb * sizeof(double), s));
+        }
+    HIP_TRY(hipStreamSynchronize(s));
+    d->host_p_ref = pref;
+    d->host_rho_ref = rref;
+
+    DgDev& dv = d->dev;
+    dv.npx = g.npx; dv.npy = g.npy; dv.npz = g.npz; dv.np = g.np;
+    dv.nelem = g.nbcs; dv.nb = nb;
+    dv.dt = prm->dt; dv.P0 = prm->P0; dv.T0 = prm->T0; dv.R = R; dv.gamma = gam; dv.cp = prm->cp;
+    for (int c = 0; c < 3; c++) dv.g[c] = prm->gravity[c];
+    dv.buoyancy = prm->buoyancy;
+    dv.JcV = d->JcV.as<double>(); dv.cV = d->cV.as<double>();
+    dv.p_ref = d->p_ref.as<double>(); dv.rho_ref = d->rho_ref.as<double>();
+    dv.eface = d->eface.as<EFace>(); dv.gFN = d->gFN.as<double>();
+    for (int dir = 0; dir < 3; dir++) { dv.D[dir] = d->D[dir].as<double>(); dv.w[dir] = d->w[dir].as<double>(); }
+    dv.bc = d->bc.as<BcTable>();
+    *out = d.release();
+    return NSEM_OK;
+}
+
+void nsem_dg_destroy(nsem_dg* d) { delete d; }
+
+int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p) {
+    if (!d || !rho || !U || !T || !p) return nsem_fail(NSEM_EINVAL, "null argument");
+    const uint64_t nb = d->nb;
+    std::vector<double> ux(nb), uy(nb), uz(nb);
+    for (uint64_t n = 0; n < nb; n++) { ux[n] = U[3 * n]; uy[n] = U[3 * n + 1]; uz[n] = U[3 * n + 2]; }
+    const double* src[6] = {rho, ux.data(), uy.data(), uz.data(), T, p};
+    for (int v = 0; v < 6; v++)
+        HIP_TRY(hipMemcpyAsync(d->st[d->cur][v].p, src[v], nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return NSEM_OK;
+}
+
+int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p) {
+    if (!d || !rho || !U || !T || !p) return nsem_fail(NSEM_EINVAL, "null argument");
+    const uint64_t nb = d->nb;
+    std::vector<double> ux(nb), uy(nb), uz(nb);
+    double* dst[6] = {rho, ux.data(), uy.data(), uz.data(), T, p};
+    for (int v = 0; v < 6; v++)
+        HIP_TRY(hipMemcpyAsync(dst[v], d->st[d->cur][v].p, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (uint64_t n = 0; n < nb; n++) { U[3 * n] = ux[n]; U[3 * n + 1] = uy[n]; U[3 * n + 2] = uz[n]; }
+    return NSEM_OK;
+}
+
+int nsem_dg_get_background(const nsem_dg* d, double* p_ref, double* rho_ref) {
+    if (!d || !p_ref || !rho_ref) return nsem_fail(NSEM_EINVAL, "null argument");
+    std::memcpy(p_ref, d->host_p_ref.data(), d->nb * sizeof(double));
+    std::memcpy(rho_ref, d->host_rho_ref.data(), d->nb * sizeof(double));
+    return NSEM_OK;
+}
+
+int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* p, double* rho) {
+    if (!d || !T || !p || !rho) return nsem_fail(NSEM_EINVAL, "null argument");
+    const nsem_euler_params& e = d->prm;
+    const double R = e.cp - e.cv, gam = e.cp / e.cv;
+    for (uint64_t n = 0; n < d->nb; n++) {
+        const double pt = p[n] + d->host_p_ref[n];
+        rho[n] = (e.P0 / (R * (T[n] + e.T0))) * std::pow(pt / e.P0, 1 / gam);
+    }
+    return NSEM_OK;
+}
+
+int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
+    if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    for (int32_t s = 0; s < nsteps; s++) {
+        DevBuf* A = d->st[d->cur];
+        DevBuf* B = d->st[1 - d->cur];
+        SplitState in{A[0].as<double>(), A[1].as<double>(), A[2].as<double>(), A[3].as<double>(), A[4].as<double>()};
+        SplitOut out{B[1].as<double>(), B[2].as<double>(), B[3].as<double>(), B[4].as<double>(), B[5].as<double>()};
+        if (d->profile) HIP_TRY(hipEventRecord(d->ev0, d->stream));
+        HIP_TRY(launch_split_step(d->dev, in, B[0].as<double>(), out, d->stream));
+        if (d->profile) {
+            HIP_TRY(hipEventRecord(d->ev1, d->stream));
+            HIP_TRY(hipEventSynchronize(d->ev1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+            d->prof_ms[0] += ms;
+            d->prof_n[0]++;
+        }
+        d->cur = 1 - d->cur;
+    }
+    return NSEM_OK;
+}
+
+int nsem_dg_synchronize(nsem_dg* d) {
+    if (!d) return nsem_fail(NSEM_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return NSEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+// The reference's split forward-Euler step of apps/euler/euler.cpp:179-287
+// (diffusion NO), as two fused element kernels:
+//
+//   split_rho : rho^{n+1} = (r + rho^n ap) / ap,  ap = -cV/dt,
+//               r = surface(Rusanov) - volume(weak) of F = U rho
+//   split_ut  : p' = P0 (rho^{n+1} theta R / P0)^gamma - p_ref
+//               U^{n+1}     from F = (rho^n U) (x) U + p' I, q = rho^{n+1} U,
+//                           source (rho^{n+1} - rho_ref) g
+//               theta^{n+1} from F = (rho^n U) theta,      q = rho^{n+1} theta
+//
+// One work-group holds EPB elements (EPB*NP threads, one per node).  A node
+// computes its own flux, writes its contravariant components to LDS, then
+// applies the transposed 1-D derivative along its three GLL lines (the
+// sum-factorised form of field.h:3443-3464), adds the Rusanov surface terms
+// of the faces it lies on (field.h:2927-2943, 3085-3117) reading the
+// neighbour's trace straight from global memory, and updates itself
+// (field.h:3740-3920 with solve.cpp:563-570).  Ghost traces come from the
+// boundary condition of the owner state, never from storage.
+#include "dg_kernels.hpp"
+
+namespace nsem {
+
+// ---------------------------------------------------------------------------
+template <int NXc, int NYc, int NZc>
+__global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out) {
+    const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
+    const int NP = NX * NY * NZ;
+    const int epb = blockDim.x / NP;
+    const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
+    const uint32_t e = blockIdx.x * epb + le;
+    const bool active = le < epb && e < d.nelem;
+    extern __shared__ double lds[];
+    double* G = lds + (size_t)le * 3 * NP;
+    const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
+    const uint64_t n = (uint64_t)e * NP + q;
+
+    double rho = 0, u0 = 0, u1 = 0, u2 = 0, th = 0;
+    if (active) {
+        rho = s.rho[n]; u0 = s.ux[n]; u1 = s.uy[n]; u2 = s.uz[n];
+        th = s.T[n] + d.T0;
+        // flux F = U rho -> contravariant G_c = sum_m F_m Jin[m][c]
+        const double f0 = u0 * rho, f1 = u1 * rho, f2 = u2 * rho;
+        const double cv = d.cV[n];
+        double J[9];
+        for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * d.nb + n];
+        (void)cv;
+        // Jin components: XX0 YY1 ZZ2 XY3 YZ4 XZ5 YX6 ZY7 ZX8
+        G[0 * NP + q] = f0 * J[0] + f1 * J[6] + f2 * J[8];
+        G[1 * NP + q] = f0 * J[3] + f1 * J[1] + f2 * J[7];
+        G[2 * NP + q] = f0 * J[5] + f1 * J[4] + f2 * J[2];
+    }
+    __syncthreads();
+    if (!active) return;
+
+    double vol = 0;
+    {
+        const double* D0 = d.D[0];
+        const double* D1 = d.D[1];
+        const double* D2 = d.D[2];
+        for (int a = 0; a < NX; a++) vol += D0[a * NX + i] * G[0 * NP + a * NY * NZ + j * NZ + k];
+        for (int b = 0; b < NY; b++) vol += D1[b * NY + j] * G[1 * NP + i * NY * NZ + b * NZ + k];
+        for (int c = 0; c < NZ; c++) vol += D2[c * NZ + k] * G[2 * NP + i * NY * NZ + j * NZ + c];
+    }
+
+    // surface: Rusanov flux of the faces through this node
+    double surf = 0;
+    const double gRT = d.gamma * d.R;
+    for (int g = 0; g < 6; g++) {
+        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+        const EFace ef = d.eface[(size_t)e * 6 + g];
+        if (!((ef.meta >> 4) & 1)) continue;
+        const bool own = (ef.meta >> 3) & 1;
+        const bool ghost = (ef.meta >> 5) & 1;
+        const int og = ef.meta & 7;
+        int a, b;
+        face_ab(g, i, j, k, a, b);
+        const double wgt = face_weight(d, own ? g : og, a, b);
+        const double nx = d.gFN[3 * (size_t)ef.facet] * wgt;
+        const double ny = d.gFN[3 * (size_t)ef.facet + 1] * wgt;
+        const double nz = d.gFN[3 * (size_t)ef.facet + 2] * wgt;
+        const double nv[3] = {nx, ny, nz};
+        double orho, ou[3], oth;
+        if (!ghost) {
+            const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+            orho = s.rho[m]; ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+            oth = s.T[m] + d.T0;
+        } else {
+            // scalars: zero-gradient copies; velocity by its own condition
+            const double uu[3] = {u0, u1, u2};
+            ghost_velocity(*d.bc, ef.meta >> 8, uu, nv, ou);
+            orho = rho;
+            oth = th;
+        }
+        // owner / neighbour roles
+        const double rO = own ? rho : orho, rN = own ? orho : rho;
+        const double uO[3] = {own ? u0 : ou[0], own ? u1 : ou[1], own ? u2 : ou[2]};
+        const double uN[3] = {own ? ou[0] : u0, own ? ou[1] : u1, own ? ou[2] : u2};
+        const double thO = own ? th : oth, thN = own ? oth : th;
+        const double fI = ghost ? 0.0 : 0.5;
+        const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
+        const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
+        const double cO = sqrt(gRT * thO), cN = sqrt(gRT * thN);
+        const double lam = ((magO * fI + magN * (1 - fI)) + (cO * fI + cN * (1 - fI))) / 2;
+        const double mn = sqrt(dot3r(nx, ny, nz, nx, ny, nz));
+        const double en[3] = {nx / mn, ny / mn, nz / mn};
+        const double dq = lam * (rN - rO);
+        double fF[3];
+        for (int c = 0; c < 3; c++) fF[c] = (uO[c] * rO) * fI + (uN[c] * rN) * (1 - fI) - en[c] * dq;
+        const double flux = dot3r(fF[0], fF[1], fF[2], nx, ny, nz);
+        surf += own ? flux : -flux;
+    }
+    const double r = surf - vol;
+    const double ap = (-1.0 / d.dt) * d.cV[n];
+    rho_out[n] = (r + rho * ap) / ap;
+}
+
+// ---------------------------------------------------------------------------
+template <int NXc, int NYc, int NZc>
+__global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, const double* __restrict__ rho1,
+                                                       SplitOut o) {
+    const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
+    const int NP = NX * NY * NZ;
+    const int epb = blockDim.x / NP;
+    const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
+    const uint32_t e = blockIdx.x * epb + le;
+    const bool active = le < epb && e < d.nelem;
+    extern __shared__ double lds[];
+    double* G = lds + (size_t)le * 12 * NP;  // rows: Ux Uy Uz theta, 3 directions each
+    const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
+    const uint64_t n = (uint64_t)e * NP + q;
+    const double R = d.R, gam = d.gamma;
+
+    double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
+    if (active) {
+        rho0 = s.rho[n]; rho = rho1[n];
+        u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
+        th = s.T[n] + d.T0;
+        pp = d.P0 * pow((rho * th * R) / d.P0, gam) - d.p_ref[n];
+        const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
+        double J[9];
+        for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * d.nb + n];
+        for (int row = 0; row < 4; row++) {
+            double f0, f1, f2;
+            if (row < 3) {
+                f0 = fc[row] * u[0] + (row == 0 ? pp : 0.0 * pp);
+                f1 = fc[row] * u[1] + (row == 1 ? pp : 0.0 * pp);
+                f2 = fc[row] * u[2] + (row == 2 ? pp : 0.0 * pp);
+            } else {
+                f0 = fc[0] * th; f1 = fc[1] * th; f2 = fc[2] * th;
+            }
+            G[(row * 3 + 0) * NP + q] = f0 * J[0] + f1 * J[6] + f2 * J[8];
+            G[(row * 3 + 1) * NP + q] = f0 * J[3] + f1 * J[1] + f2 * J[7];
+            G[(row * 3 + 2) * NP + q] = f0 * J[5] + f1 * J[4] + f2 * J[2];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+
+    double res[4];
+    {
+        const double* D0 = d.D[0];
+        const double* D1 = d.D[1];
+        const double* D2 = d.D[2];
+        for (int row = 0; row < 4; row++) {
+            const double* G0 = G + (row * 3 + 0) * NP;
+            const double* G1 = G + (row * 3 + 1) * NP;
+            const double* G2 = G + (row * 3 + 2) * NP;
+            double v = 0;
+            for (int a = 0; a < NX; a++) v += D0[a * NX + i] * G0[a * NY * NZ + j * NZ + k];
+            for (int b = 0; b < NY; b++) v += D1[b * NY + j] * G1[i * NY * NZ + b * NZ + k];
+            for (int c = 0; c < NZ; c++) v += D2[c * NZ + k] * G2[i * NY * NZ + j * NZ + c];
+            res[row] = -v;
+        }
+    }
+
+    const double gRT = gam * R;
+    for (int g = 0; g < 6; g++) {
+        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+        const EFace ef = d.eface[(size_t)e * 6 + g];
+        if (!((ef.meta >> 4) & 1)) continue;
+        const bool own = (ef.meta >> 3) & 1;
+        const bool ghost = (ef.meta >> 5) & 1;
+        const int og = ef.meta & 7;
+        int a, b;
+        face_ab(g, i, j, k, a, b);
+        const double wgt = face_weight(d, own ? g : og, a, b);
+        const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                              d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+        double orho0, orho, ou[3], oth, opp;
+        if (!ghost) {
+            const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+            orho0 = s.rho[m]; orho = rho1[m];
+            ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+            oth = s.T[m] + d.T0;
+            opp = d.P0 * pow((orho * oth * R) / d.P0, gam) - d.p_ref[m];
+        } else {
+            ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
+            orho0 = rho0;
+            orho = rho;
+            oth = th;
+            opp = pp;
+        }
+        const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
+        const double rO = own ? rho : orho, rN = own ? orho : rho;
+        double uO[3], uN[3];
+        for (int c = 0; c < 3; c++) { uO[c] = own ? u[c] : ou[c]; uN[c] = own ? ou[c] : u[c]; }
+        const double thO = own ? th : oth, thN = own ? oth : th;
+        const double pO = own ? pp : opp, pN = own ? opp : pp;
+        const double fI = ghost ? 0.0 : 0.5;
+        const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
+        const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
+        const double lam = ((magO * fI + magN * (1 - fI)) +
+                            (sqrt(gRT * thO) * fI + sqrt(gRT * thN) * (1 - fI))) / 2;
+        const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+        const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+        const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
+        const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
+        // momentum: tensor flux, q = rho^{n+1} U
+        for (int row = 0; row < 3; row++) {
+            double fF[3];
+            for (int c = 0; c < 3; c++) {
+                const double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
+                const double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
+                const double dq = lam * (rN * uN[c] - rO * uO[c]);
+                fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
+            }
+            const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
+            res[row] += own ? flux : -flux;
+        }
+        // potential temperature: vector flux, q = rho^{n+1} theta
+        {
+            const double dq = lam * (rN * thN - rO * thO);
+            double fF[3];
+            for (int c = 0; c < 3; c++) fF[c] = (fcO[c] * thO) * fI + (fcN[c] * thN) * (1 - fI) - en[c] * dq;
+            const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
+            res[3] += own ? flux : -flux;
+        }
+    }
+    const double cv = d.cV[n];
+    const double ap0 = (-1.0 / d.dt) * cv;
+    const double ap = ap0 * rho;
+    double* outs[3] = {o.ux, o.uy, o.uz};
+    for (int c = 0; c < 3; c++) {
+        const double sc = d.buoyancy ? (rho - d.rho_ref[n]) * d.g[c] : 0.0;
+        outs[c][n] = ((res[c] - sc * cv) + (u[c] * rho0) * ap0) / ap;
+    }
+    const double thn = (res[3] + (th * rho0) * ap0) / ap;
+    o.T[n] = thn - d.T0;
+    o.p[n] = pp;
+}
+
+// ---------------------------------------------------------------------------
+template <int X, int Y, int Z>
+static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
+                                 hipStream_t st) {
+    const int NP = d.np;
+    if (NP > 1024) return hipErrorInvalidValue;
+    int epb = NP >= 256 ? 1 : 256 / NP;
+    int threads = epb * NP;
+    threads = (threads + 63) / 64 * 64;
+    const unsigned blocks = (d.nelem + epb - 1) / epb;
+    size_t lds_a = (size_t)epb * 3 * NP * sizeof(double);
+    size_t lds_b = (size_t)epb * 12 * NP * sizeof(double);
+    hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL((split_ut_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_b, st, d, s, rho1, o);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
+                             hipStream_t st) {
+    if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_split_t<8, 8, 8>(d, s, rho1, o, st);
+    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_split_t<5, 1, 5>(d, s, rho1, o, st);
+    if (d.npx == 4 && d.npy == 4 && d.npz == 4) return launch_split_t<4, 4, 4>(d, s, rho1, o, st);
+    return launch_split_t<0, 0, 0>(d, s, rho1, o, st);
+}
+
+}  // namespace nsem
