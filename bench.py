@@ -1,0 +1,256 @@
+"""Benchmark of the MI355X dGSEM Euler residual and FV pressure PCG.
+
+Workload (one GPU): 3D dGSEM compressible Euler, 64x64x32 hex elements over
+10 x 10 x 5 km, N = 7 (512 GLL nodes per element, 67,108,864 nodes, 335.5M
+DOF), hydrostatic theta0 = 300 K plus four seeded Gaussian warm bubbles
+(dtheta in [0.1, 1] K, radii 0.5-2 km), no-flux walls, SSP-RK3.
+
+One "step" = one SSP-RK3 step (three fused residual+update stages) over the
+whole mesh, inputs resident in HBM.  value = DOF*stage/s of the
+FP32-perturbation form (FP32 residual and stage increments over the FP64
+background, FP64 base state and accumulation); the FP64 form, the error of
+FP32 against FP64 after warmup+steps steps, the reference-mode split step and
+the FV Poisson PCG time to |b-Ax|/|b| <= 1e-10 are reported beside it.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fv]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12               # MI355X HBM3E, bytes/s
+NQ = 5
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_node(s_in, s_base, s_out, s_geo, N):
+    """Bytes per node per stage of the DG residual+update (the path's
+    algorithmic figure): n_q (s_in + s_base + s_out) + 10 s_geo + 2*8
+    + 3/(N+1) (3 s_geo + 8)."""
+    return NQ * (s_in + s_base + s_out) + 10 * s_geo + 2 * 8 + 3.0 / (N + 1) * (3 * s_geo + 8)
+
+
+def box_patches(lx, ly, lz):
+    c = [(0, 0, 0), (lx, 0, 0), (lx, 0, lz), (0, 0, lz), (0, ly, 0), (lx, ly, 0), (lx, ly, lz), (0, ly, lz)]
+    planes = [("top", (3, 2, 6, 7)), ("bottom", (0, 1, 5, 4)), ("front", (0, 1, 2, 3)),
+              ("back", (4, 5, 6, 7)), ("sides", (1, 5, 6, 2)), ("sides", (0, 4, 7, 3))]
+    return c, [(n, [c[i] for i in g]) for n, g in planes]
+
+
+def bubbles(seed, lx, ly, lz, k=4):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        dth = rng.uniform(0.1, 1.0)
+        r = rng.uniform(500.0, 2000.0)
+        cx, cy, cz = rng.uniform(0, lx), rng.uniform(0, ly), rng.uniform(0, lz)
+        out.append(f"gaussian 0 {dth!r} {cx!r} {cy!r} {cz!r} {r!r} {r!r} {r!r}")
+    return out
+
+
+def field_file(path, ncomp, gens, patches, kind):
+    names = sorted(set(p for p in patches))
+    with open(path, "w") as f:
+        f.write(f"size {ncomp}\ninternal {len(gens)}\n{{\n")
+        for g in gens:
+            f.write("    " + g + "\n")
+        f.write(f"}}\nboundary {len(names)}\n{{\n")
+        for n in names:
+            f.write(f"    {n} {{\n        type {kind}\n    }}\n")
+        f.write("}\n")
+
+
+def setup_dg(ndiv, size, order, dt, seed, tmp):
+    from nebulasem_amd import DGEuler, Mesh, euler_params, read_field
+    lx, ly, lz = size
+    corners, patches = box_patches(lx, ly, lz)
+    t = time.time()
+    mesh = Mesh.box(corners, ndiv, patches=patches, order=order)
+    log(f"mesh {ndiv} order {order}: {mesh.info.nbcs_field} nodes, {time.time() - t:.1f}s")
+    names = [n for n, _ in patches]
+    g = (0.0, 0.0, -9.80606)
+    field_file(os.path.join(tmp, "T0.txt"), 1, bubbles(seed, lx, ly, lz), names, "NEUMANN")
+    field_file(os.path.join(tmp, "U0.txt"), 3, ["uniform 0 0 0"], names, "SYMMETRY")
+    field_file(os.path.join(tmp, "p0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+    field_file(os.path.join(tmp, "rho0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+    t = time.time()
+    f = {k: read_field(mesh, os.path.join(tmp, k + "0.txt"), nc, g)
+         for k, nc in (("T", 1), ("U", 3), ("p", 1), ("rho", 1))}
+    nb = mesh.info.nbcs_field
+    dg = DGEuler.from_fields(mesh, euler_params(dt), f)
+    T = np.ascontiguousarray(f["T"][0][:nb, 0])
+    p = np.ascontiguousarray(f["p"][0][:nb, 0])
+    U = np.ascontiguousarray(f["U"][0][:nb])
+    rho = dg.initial_density(T, p)
+    del f
+    log(f"fields + device set-up {time.time() - t:.1f}s")
+    return mesh, dg, (rho, U, T, p)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """The oracle (single-threaded C restatement of the reference's split
+    step) on a bounded sample: 16x16x8 elements at N = 7 (1,048,576 nodes,
+    the workload's element shape and order), timed over whole steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as oracle
+    from nebulasem_amd import Mesh, read_field
+    with tempfile.TemporaryDirectory() as tmp:
+        lx, ly, lz = 10000.0 / 4, 10000.0 / 4, 5000.0 / 4
+        corners, patches = box_patches(lx, ly, lz)
+        mesh = Mesh.box(corners, (16, 16, 8), patches=patches, order=(7, 7, 7))
+        names = [n for n, _ in patches]
+        g = (0.0, 0.0, -9.80606)
+        field_file(os.path.join(tmp, "T0.txt"), 1, bubbles(7, lx, ly, lz), names, "NEUMANN")
+        field_file(os.path.join(tmp, "U0.txt"), 3, ["uniform 0 0 0"], names, "SYMMETRY")
+        field_file(os.path.join(tmp, "p0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        field_file(os.path.join(tmp, "rho0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        f = {k: read_field(mesh, os.path.join(tmp, k + "0.txt"), nc, g)
+             for k, nc in (("T", 1), ("U", 3), ("p", 1), ("rho", 1))}
+        om = oracle.OracleMesh(mesh)
+        prm = oracle.make_params(0.005, {k: (f[k][1], f[k][2].reshape(-1)) for k in ("rho", "U", "T", "p")})
+        T = f["T"][0].reshape(-1).copy()
+        p = f["p"][0].reshape(-1).copy()
+        U = f["U"][0].reshape(-1).copy()
+        p_ref, rho_ref, rho = oracle.euler_hydrostatic(om, prm, T, p)
+        nb = mesh.info.nbcs_field
+        steps, t0 = 0, time.time()
+        while True:
+            oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, 1)
+            steps += 1
+            el = time.time() - t0
+            if el > seconds_budget or steps >= 20:
+                break
+        rate = NQ * nb * steps / el
+        return {"value": rate, "unit": "DOF*stage/s", "cores": 1, "kind": "port",
+                "sample": f"oracle split step (reference algorithm, forward Euler, 1 stage/step), "
+                          f"16x16x8 elements N=7 = {nb} nodes, {steps} steps in {el:.1f}s, 1 thread"}
+
+
+def fv_bench(n, tol=1e-10):
+    from nebulasem_amd import BC_DIRICHLET, BC_NEUMANN, FVPoisson, Mesh
+    corners = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]
+    planes = [("xmin", (0, 4, 7, 3)), ("xmax", (1, 5, 6, 2)), ("ymin", (0, 1, 2, 3)),
+              ("ymax", (4, 5, 6, 7)), ("zmin", (0, 1, 5, 4)), ("zmax", (3, 2, 6, 7))]
+    t = time.time()
+    mesh = Mesh.box(corners, (n, n, n), patches=[(nm, [corners[i] for i in g]) for nm, g in planes])
+    kinds = np.array([BC_DIRICHLET if b == "xmin" else BC_NEUMANN for b in mesh.boundaries], dtype=np.int32)
+    fv = FVPoisson(mesh, kinds, np.zeros(len(kinds)))
+    rng = np.random.default_rng(20261202)
+    U = rng.uniform(-1.0, 1.0, size=(mesh.info.nbcs, 3))
+    fv.set_rhs_divergence(U, np.zeros(len(kinds), dtype=np.int32), np.zeros((len(kinds), 3)))
+    log(f"FV {n}^3 set-up {time.time() - t:.1f}s (structured={fv.structured})")
+    out = {}
+    for mode, name in ((1, "fp64"), (2, "fp32_refined")):
+        fv.solve(mode=mode, tol=tol)  # warm-up (kernel load)
+        x, res, rc = fv.solve(mode=mode, tol=tol)
+        out[name] = {"seconds": res.seconds, "iterations": res.iterations,
+                     "outer": res.outer_iterations, "true_rel_residual": res.res_true}
+        log(f"FV {name}: {res.seconds * 1e3:.1f} ms, {res.iterations} it, |b-Ax|/|b| {res.res_true:.2e}")
+    # per-iteration bytes: SpMV+dot (p 8, 3 couplings 24, diag 8, Ap 8) + update (x p r Ap
+    # read, x r z write, idiag read: 64) + p update (z p read, p write: 24)
+    b_iter = 48 + 64 + 24
+    t_iter = out["fp64"]["seconds"] / max(out["fp64"]["iterations"], 1)
+    out["fp64"]["GBps_per_iter"] = b_iter * n ** 3 / t_iter / 1e9
+    out["cells"] = n ** 3
+    fv.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ndiv", type=str, default="64,64,32")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fv", action="store_true")
+    ap.add_argument("--fv-n", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    ndiv = tuple(int(x) for x in args.ndiv.split(","))
+    N = 7
+    dt = 0.005
+    with tempfile.TemporaryDirectory() as tmp:
+        mesh, dg, (rho, U, T, p) = setup_dg(ndiv, (10000.0, 10000.0, 5000.0), (N, N, N), dt, 20261202, tmp)
+    nb = mesh.info.nbcs_field
+    K, W = args.steps, args.warmup
+    res = {}
+    for prec in (64, 32):
+        dg.set_state(rho, U, T, p)
+        dg.rk3_steps(W, prec)
+        dg.synchronize()
+        t = dg.time_rk3(K, prec)
+        res[prec] = {"seconds": t, "rate": NQ * nb * 3 * K / t, "stage_s": t / (3 * K),
+                     "q": dg.get_conserved()}
+        log(f"rk3 fp{prec}: {t * 1e3:.1f} ms for {K} steps, {res[prec]['rate'] / 1e9:.2f} G DOF*stage/s")
+    err = {}
+    for name, a, b in zip(("rho_p", "m", "Theta_p"), res[32]["q"], res[64]["q"]):
+        err[name] = rel_l2(a, b)
+    log(f"fp32 vs fp64 after {W + K} steps: {err}")
+    # the reference's split forward-Euler step (1 stage/step)
+    dg.set_state(rho, U, T, p)
+    ks = max(5, K // 5)
+    ts = dg.time_split(ks)
+    split_rate = NQ * nb * ks / ts
+    log(f"split step fp64: {ts / ks * 1e3:.2f} ms/step, {split_rate / 1e9:.2f} G DOF*stage/s")
+    dg.close()
+    del res[64]["q"], res[32]["q"]
+
+    b32 = algorithmic_bytes_per_node(4, 8, 8, 4, N)
+    b64 = algorithmic_bytes_per_node(8, 8, 8, 8, N)
+    achieved = b32 * nb / res[32]["stage_s"]
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": "rk_stage_kernel<float,float,8,8,8>",
+                "bytes_per_node": b32, "nodes_per_launch": nb,
+                "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
+                "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK}
+    fv = None
+    if not args.no_fv:
+        fv = fv_bench(args.fv_n)
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "dGSEM DOF*stage/s (FP32-perturbation SSP-RK3, N=7)",
+        "value": res[32]["rate"],
+        "unit": "DOF*stage/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": res[32]["seconds"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 residual / f64 state",
+        "data": "synthetic: hydrostatic theta0=300K + 4 seeded Gaussian bubbles",
+        "config": {"workload": f"3D dGSEM Euler {ndiv[0]}x{ndiv[1]}x{ndiv[2]} hex, 10x10x5 km, N=7, "
+                               f"SSP-RK3 dt={dt}", "nodes": nb, "dof": NQ * nb},
+        "fp64": {"value": res[64]["rate"], "ms_per_step": res[64]["seconds"] / K * 1e3},
+        "fp32_over_fp64": res[32]["rate"] / res[64]["rate"],
+        "fp32_vs_fp64_rel_l2": err,
+        "split_reference_step_fp64": {"value": split_rate, "ms_per_step": ts / ks * 1e3},
+        "roofline": roofline,
+        "pcg": fv,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -117,7 +117,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
                    const double* bc_value, nsem_dg** out);
 void nsem_dg_destroy(nsem_dg* d);
 /* State as the reference stores it, interior nodes only (host arrays):
- * rho, U (xyz per node), T = theta', p = p'. */
+ * rho, U (xyz per node), T = theta', p = p'.  Setting it also sets the
+ * method-of-lines state; getting it returns whichever path ran last. */
 int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p);
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
 /* Hydrostatic reference p_ref, rho_ref at the interior nodes (euler.cpp:121-123). */
@@ -128,6 +129,18 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
  * diffusion NO): rho, then p, then U and theta, each a Rusanov dGSEM residual
  * with the explicit ddt/Solve update (field.h:3740-3920, solve.cpp:563-570). */
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps);
+/* nsteps of SSP-RK3 (Shu-Osher) on the method-of-lines form of the same
+ * spatial operator, state = conserved perturbations (rho', rho U,
+ * Theta' = rho theta - rho_ref T0) over the FP64 hydrostatic background.
+ * precision 64: FP64 residual; 32: FP32 residual and stage increments with
+ * FP64 base state and accumulation.  No reference counterpart (the
+ * reference's explicit schemes all reduce to forward Euler). */
+int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision);
+/* Conserved perturbations of the method-of-lines state (interior nodes). */
+int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p);
+/* Same as the step functions, returning the device time of the steps. */
+int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds);
+int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds);
 int nsem_dg_synchronize(nsem_dg* d);
 
 /* ------------------------------------------------------------ FV Poisson --

@@ -213,6 +213,25 @@ class DGEuler:
     def split_steps(self, n):
         check(lib().nsem_dg_split_steps(self._h, int(n)))
 
+    def rk3_steps(self, n, precision=64):
+        check(lib().nsem_dg_rk3_steps(self._h, int(n), int(precision)))
+
+    def time_rk3(self, n, precision=64):
+        s = C.c_double()
+        check(lib().nsem_dg_time_rk3(self._h, int(n), int(precision), C.byref(s)))
+        return s.value
+
+    def time_split(self, n):
+        s = C.c_double()
+        check(lib().nsem_dg_time_split(self._h, int(n), C.byref(s)))
+        return s.value
+
+    def get_conserved(self):
+        nb = self.nb
+        rp, m, tp = np.empty(nb), np.empty(3 * nb), np.empty(nb)
+        check(lib().nsem_dg_get_conserved(self._h, _pd(rp), _pd(m), _pd(tp)))
+        return rp, m, tp
+
     def synchronize(self):
         check(lib().nsem_dg_synchronize(self._h))
 

@@ -1,4 +1,7 @@
 // C-ABI: mesh generation, loading and geometry export (host side).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -185,7 +188,11 @@ int nsem_mesh_box(const double corners[24], const uint32_t ndiv[3], const int32_
     }
     auto m = std::make_unique<nsem_mesh>();
     std::string err;
+    auto t0 = std::chrono::steady_clock::now();
     if (!generate_block_grid(blk, patches, m->grid, &err)) return nsem_fail(NSEM_EUNSUPPORTED, err);
+    if (std::getenv("NSEM_TIMING"))
+        std::fprintf(stderr, "[nsem] %-28s %8.3f s\n", "generate_block_grid",
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     return finish_mesh(m, order, nonortho, out);
 }
 

@@ -45,6 +45,8 @@ struct DgDev {
     int buoyancy;
     const double* JcV;                // [9][nb]
     const double* cV;                 // [nb]
+    const float* JcV32;               // [9][nb] FP32 copies for the FP32 residual
+    const float* cV32;                // [nb]
     const double* p_ref;              // [nb]
     const double* rho_ref;            // [nb]
     const EFace* eface;               // [nelem*6]

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../host/parallel.hpp"
 #include "../mesh_access.hpp"
 #include "../status.hpp"
 #include "dg_common.hpp"
@@ -43,10 +44,15 @@ struct nsem_dg {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t nb = 0;
     // geometry
-    DevBuf JcV, cV, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
+    DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
     // double-buffered state: rho ux uy uz T p
     DevBuf st[2][6];
     int cur = 0;
+    // method-of-lines state: FP64 conserved perturbations (rho', m, Theta'),
+    // double-buffered, and stage increments in FP64 and FP32
+    DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
+    int mcur = 0;
+    bool mol_current = false;
     // profiling of the dominant kernels
     bool profile = false;
     double prof_ms[4] = {0, 0, 0, 0};
@@ -81,7 +87,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     if (g.np < 2) return nsem_fail(NSEM_EINVAL, "DG path needs polynomial order >= 1");
     if (g.np > 1024) return nsem_fail(NSEM_EUNSUPPORTED, "at most 1024 nodes per element");
     if (names.size() > (size_t)kMaxPatches) return nsem_fail(NSEM_EUNSUPPORTED, "too many boundary patches");
-    if (12 * (size_t)g.np * sizeof(double) > 65536)
+    if (15 * (size_t)g.np * sizeof(double) > 65536)
         return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the LDS staging (order <= 7 in 3D)");
     if (!(prm->dt > 0)) return nsem_fail(NSEM_EINVAL, "dt must be positive");
     int ndev = 0;
@@ -162,10 +168,16 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     }
     // metric, volumes, background (euler.cpp:104-131)
     std::vector<double> JcV(9 * nb), cV(nb), pref(nb), rref(nb);
+    std::vector<float> JcV32(9 * nb), cV32(nb);
     const double R = prm->cp - prm->cv, gam = prm->cp / prm->cv;
-    for (uint64_t n = 0; n < nb; n++) {
-        for (int c = 0; c < 9; c++) JcV[c * nb + n] = g.Jinv[n][c] * g.cV[n];
+    parallel_for(nb, [&](uint64_t nbeg, uint64_t nend) {
+    for (uint64_t n = nbeg; n < nend; n++) {
+        for (int c = 0; c < 9; c++) {
+            JcV[c * nb + n] = g.Jinv[n][c] * g.cV[n];
+            JcV32[c * nb + n] = (float)JcV[c * nb + n];
+        }
         cV[n] = g.cV[n];
+        cV32[n] = (float)cV[n];
         const V3& x = g.cC[n];
         if (prm->buoyancy) {
             double gh = prm->gravity[2] * x[2] + 0.0;
@@ -178,11 +190,13 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             rref[n] = prm->P0 / (R * prm->T0);
         }
     }
+    });
     std::vector<double> gfn(3 * (size_t)g.nfacets);
     for (Int f = 0; f < g.nfacets; f++)
         for (int c = 0; c < 3; c++) gfn[3 * (size_t)f + c] = lm.fn[f][c];
     hipError_t e;
     if ((e = upload(d->JcV, JcV, s)) != hipSuccess || (e = upload(d->cV, cV, s)) != hipSuccess ||
+        (e = upload(d->JcV32, JcV32, s)) != hipSuccess || (e = upload(d->cV32, cV32, s)) != hipSuccess ||
         (e = upload(d->p_ref, pref, s)) != hipSuccess || (e = upload(d->rho_ref, rref, s)) != hipSuccess ||
         (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess)
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
@@ -198,6 +212,14 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             HIP_TRY(d->st[a][v].alloc(nb * sizeof(double)));
             HIP_TRY(hipMemsetAsync(d->st[a][v].p, 0, nb * sizeof(double), s));
         }
+    for (int a = 0; a < 2; a++)
+        for (int v = 0; v < 5; v++) {
+            HIP_TRY(d->mq[a][v].alloc(nb * sizeof(double)));
+            HIP_TRY(d->d64[a][v].alloc(nb * sizeof(double)));
+            HIP_TRY(d->d32[a][v].alloc(nb * sizeof(float)));
+        }
+    HIP_TRY(d->flag.alloc(sizeof(int)));
+    HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), s));
     HIP_TRY(hipStreamSynchronize(s));
     d->host_p_ref = pref;
     d->host_rho_ref = rref;
@@ -209,6 +231,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     for (int c = 0; c < 3; c++) dv.g[c] = prm->gravity[c];
     dv.buoyancy = prm->buoyancy;
     dv.JcV = d->JcV.as<double>(); dv.cV = d->cV.as<double>();
+    dv.JcV32 = d->JcV32.as<float>(); dv.cV32 = d->cV32.as<float>();
     dv.p_ref = d->p_ref.as<double>(); dv.rho_ref = d->rho_ref.as<double>();
     dv.eface = d->eface.as<EFace>(); dv.gFN = d->gFN.as<double>();
     for (int dir = 0; dir < 3; dir++) { dv.D[dir] = d->D[dir].as<double>(); dv.w[dir] = d->w[dir].as<double>(); }
@@ -228,12 +251,55 @@ int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const doub
     for (int v = 0; v < 6; v++)
         HIP_TRY(hipMemcpyAsync(d->st[d->cur][v].p, src[v], nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
+    // conserved perturbations for the method-of-lines path
+    const double T0 = d->prm.T0;
+    std::vector<double> q[5];
+    for (int v = 0; v < 5; v++) q[v].resize(nb);
+    for (uint64_t n = 0; n < nb; n++) {
+        const double rr = d->host_rho_ref[n];
+        q[0][n] = rho[n] - rr;
+        q[1][n] = rho[n] * ux[n];
+        q[2][n] = rho[n] * uy[n];
+        q[3][n] = rho[n] * uz[n];
+        q[4][n] = rho[n] * (T[n] + T0) - rr * T0;
+    }
+    for (int v = 0; v < 5; v++)
+        HIP_TRY(hipMemcpyAsync(d->mq[d->mcur][v].p, q[v].data(), nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->mol_current = false;
+    return NSEM_OK;
+}
+
+int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p) {
+    if (!d || !rho_p || !m || !theta_p) return nsem_fail(NSEM_EINVAL, "null argument");
+    const uint64_t nb = d->nb;
+    std::vector<double> mx(nb), my(nb), mz(nb);
+    double* dst[5] = {rho_p, mx.data(), my.data(), mz.data(), theta_p};
+    for (int v = 0; v < 5; v++)
+        HIP_TRY(hipMemcpyAsync(dst[v], d->mq[d->mcur][v].p, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (uint64_t n = 0; n < nb; n++) { m[3 * n] = mx[n]; m[3 * n + 1] = my[n]; m[3 * n + 2] = mz[n]; }
     return NSEM_OK;
 }
 
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p) {
     if (!d || !rho || !U || !T || !p) return nsem_fail(NSEM_EINVAL, "null argument");
     const uint64_t nb = d->nb;
+    if (d->mol_current) {
+        std::vector<double> rp(nb), m(3 * nb), tp(nb);
+        int rc = nsem_dg_get_conserved(d, rp.data(), m.data(), tp.data());
+        if (rc) return rc;
+        const double T0 = d->prm.T0, gam = d->prm.cp / d->prm.cv;
+        for (uint64_t n = 0; n < nb; n++) {
+            const double rr = d->host_rho_ref[n];
+            rho[n] = rr + rp[n];
+            for (int c = 0; c < 3; c++) U[3 * n + c] = m[3 * n + c] / rho[n];
+            T[n] = (tp[n] + rr * T0) / rho[n] - T0;
+            p[n] = d->host_p_ref[n] * std::expm1(gam * std::log1p(tp[n] / (rr * T0)));
+        }
+        return NSEM_OK;
+    }
     std::vector<double> ux(nb), uy(nb), uz(nb);
     double* dst[6] = {rho, ux.data(), uy.data(), uz.data(), T, p};
     for (int v = 0; v < 6; v++)
@@ -280,6 +346,94 @@ int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
         }
         d->cur = 1 - d->cur;
     }
+    d->mol_current = false;
+    return NSEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename R, typename S>
+int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
+    static const double kB[3] = {1.0, 0.25, 2.0 / 3.0};
+    for (int32_t st = 0; st < nsteps; st++) {
+        DevBuf* qa = d->mq[d->mcur];
+        DevBuf* qb = d->mq[1 - d->mcur];
+        for (int stage = 0; stage < 3; stage++) {
+            RkArgs<S> a{};
+            for (int v = 0; v < 5; v++) {
+                a.q[v] = qa[v].as<double>();
+                a.din[v] = stage == 0 ? nullptr : dl[(stage - 1) & 1][v].as<S>();
+                a.dout[v] = dl[stage & 1][v].as<S>();
+                a.qout[v] = qb[v].as<double>();
+            }
+            a.b = kB[stage];
+            a.last = stage == 2;
+            a.check = (stage == 2);
+            a.flag = d->flag.as<int>();
+            HIP_TRY((launch_rk_stage<R, S>(d->dev, a, d->stream)));
+        }
+        d->mcur = 1 - d->mcur;
+    }
+    d->mol_current = true;
+    return NSEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
+    if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    int rc;
+    if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
+    else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
+    else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    if (rc) return rc;
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, d->flag.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (flag) {
+        HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        return nsem_fail(NSEM_ENONFINITE, "non-finite residual in the RK stages");
+    }
+    return NSEM_OK;
+}
+
+int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
+    if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    HIP_TRY(hipEventRecord(d->ev0, d->stream));
+    int rc;
+    if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
+    else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
+    else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(d->ev1, d->stream));
+    HIP_TRY(hipEventSynchronize(d->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    *seconds = ms * 1e-3;
+    int flag = 0;
+    HIP_TRY(hipMemcpy(&flag, d->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        HIP_TRY(hipMemset(d->flag.p, 0, sizeof(int)));
+        return nsem_fail(NSEM_ENONFINITE, "non-finite residual in the RK stages");
+    }
+    return NSEM_OK;
+}
+
+int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds) {
+    if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    HIP_TRY(hipEventRecord(d->ev0, d->stream));
+    int rc = nsem_dg_split_steps(d, nsteps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(d->ev1, d->stream));
+    HIP_TRY(hipEventSynchronize(d->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    *seconds = ms * 1e-3;
     return NSEM_OK;
 }
 

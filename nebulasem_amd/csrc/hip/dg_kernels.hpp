@@ -21,6 +21,23 @@ struct SplitOut {
     double* p;
 };
 
+// One fused residual + SSP-RK3 stage.  q: FP64 base state (5 SoA arrays:
+// rho', m_x, m_y, m_z, Theta'); din: stage increment in (null at stage 1);
+// out: b*(din + dt L(q + din)) to dout, or q + that to qout when last.
+template <typename S>
+struct RkArgs {
+    const double* q[5];
+    const S* din[5];
+    S* dout[5];
+    double* qout[5];
+    double b;
+    int last;
+    int check;
+    int* flag;
+};
+template <typename R, typename S>
+hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st);
+
 // One split forward-Euler step (two fused element kernels).
 hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
                              hipStream_t st);

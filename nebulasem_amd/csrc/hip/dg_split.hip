@@ -20,6 +20,54 @@
 
 namespace nsem {
 
+// Volume term of the reference's weak divergence (field.h:3443-3464) at
+// target node (i,j,k), subtracted from r in the reference's accumulation
+// order.  The reference walks source nodes s lexicographically and, at each,
+// subtracts dot(F_s, Jin_s grad psi_t(x_s)) from every target on the three
+// GLL lines through s.  Seen from the target the sources come as: x-line
+// a < i, y-line b < j, z-line c < k, itself, z-line c > k, y-line b > j,
+// x-line a > i.  Off the diagonal grad psi has one non-zero component, so
+// Jin grad psi is one column of Jin times a derivative value, exactly as the
+// reference's 3x3 product evaluates it.  F: LDS rows x 3 x NP, J: LDS 9 x NP.
+template <int ROWS, int VECMASK>
+__device__ __forceinline__ void volume_reference_order(double* r, const double* F, const double* J, int NP,
+                                                       int i, int j, int k, int NX, int NY, int NZ,
+                                                       const double* D0, const double* D1,
+                                                       const double* D2) {
+    auto sub = [&](int s, double w0, double w1, double w2) {
+        for (int row = 0; row < ROWS; row++) {
+            const double f0 = F[(row * 3 + 0) * NP + s];
+            const double f1 = F[(row * 3 + 1) * NP + s];
+            const double f2 = F[(row * 3 + 2) * NP + s];
+            // vector flux: right-nested dot; tensor flux: row dot left to right
+            const double c = ((VECMASK >> row) & 1) ? dot3r(f0, f1, f2, w0, w1, w2) : f0 * w0 + f1 * w1 + f2 * w2;
+            r[row] -= c;
+        }
+    };
+    // column d of Jin (XX0 YY1 ZZ2 XY3 YZ4 XZ5 YX6 ZY7 ZX8)
+    auto col = [&](int s, int d, double D) {
+        const int c0 = d == 0 ? 0 : (d == 1 ? 3 : 5);
+        const int c1 = d == 0 ? 6 : (d == 1 ? 1 : 4);
+        const int c2 = d == 0 ? 8 : (d == 1 ? 7 : 2);
+        sub(s, J[c0 * NP + s] * D, J[c1 * NP + s] * D, J[c2 * NP + s] * D);
+    };
+    const int syz = NY * NZ;
+    const int t = i * syz + j * NZ + k;
+    for (int a = 0; a < i; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
+    for (int b = 0; b < j; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
+    for (int c = 0; c < k; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
+    {
+        const double d0 = D0[i * NX + i], d1 = D1[j * NY + j], d2 = D2[k * NZ + k];
+        const double w0 = J[0 * NP + t] * d0 + J[3 * NP + t] * d1 + J[5 * NP + t] * d2;
+        const double w1 = J[6 * NP + t] * d0 + J[1 * NP + t] * d1 + J[4 * NP + t] * d2;
+        const double w2 = J[8 * NP + t] * d0 + J[7 * NP + t] * d1 + J[2 * NP + t] * d2;
+        sub(t, w0, w1, w2);
+    }
+    for (int c = k + 1; c < NZ; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
+    for (int b = j + 1; b < NY; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
+    for (int a = i + 1; a < NX; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
+}
+
 // ---------------------------------------------------------------------------
 template <int NXc, int NYc, int NZc>
 __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out) {
@@ -30,7 +78,8 @@ __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, 
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds[];
-    double* G = lds + (size_t)le * 3 * NP;
+    double* F = lds + (size_t)le * 12 * NP;  // flux U rho (3), then Jin (9)
+    double* J = F + 3 * NP;
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
@@ -38,31 +87,16 @@ __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, 
     if (active) {
         rho = s.rho[n]; u0 = s.ux[n]; u1 = s.uy[n]; u2 = s.uz[n];
         th = s.T[n] + d.T0;
-        // flux F = U rho -> contravariant G_c = sum_m F_m Jin[m][c]
-        const double f0 = u0 * rho, f1 = u1 * rho, f2 = u2 * rho;
-        const double cv = d.cV[n];
-        double J[9];
-        for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * d.nb + n];
-        (void)cv;
-        // Jin components: XX0 YY1 ZZ2 XY3 YZ4 XZ5 YX6 ZY7 ZX8
-        G[0 * NP + q] = f0 * J[0] + f1 * J[6] + f2 * J[8];
-        G[1 * NP + q] = f0 * J[3] + f1 * J[1] + f2 * J[7];
-        G[2 * NP + q] = f0 * J[5] + f1 * J[4] + f2 * J[2];
+        F[0 * NP + q] = u0 * rho;
+        F[1 * NP + q] = u1 * rho;
+        F[2 * NP + q] = u2 * rho;
+        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
     }
     __syncthreads();
     if (!active) return;
 
-    double vol = 0;
-    {
-        const double* D0 = d.D[0];
-        const double* D1 = d.D[1];
-        const double* D2 = d.D[2];
-        for (int a = 0; a < NX; a++) vol += D0[a * NX + i] * G[0 * NP + a * NY * NZ + j * NZ + k];
-        for (int b = 0; b < NY; b++) vol += D1[b * NY + j] * G[1 * NP + i * NY * NZ + b * NZ + k];
-        for (int c = 0; c < NZ; c++) vol += D2[c * NZ + k] * G[2 * NP + i * NY * NZ + j * NZ + c];
-    }
-
-    // surface: Rusanov flux of the faces through this node
+    // surface: Rusanov flux of the faces through this node, in the cell's
+    // face order (div_flux, field.h:3093-3114)
     double surf = 0;
     const double gRT = d.gamma * d.R;
     for (int g = 0; g < 6; g++) {
@@ -91,7 +125,6 @@ __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, 
             orho = rho;
             oth = th;
         }
-        // owner / neighbour roles
         const double rO = own ? rho : orho, rN = own ? orho : rho;
         const double uO[3] = {own ? u0 : ou[0], own ? u1 : ou[1], own ? u2 : ou[2]};
         const double uN[3] = {own ? ou[0] : u0, own ? ou[1] : u1, own ? ou[2] : u2};
@@ -107,11 +140,13 @@ __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, 
         double fF[3];
         for (int c = 0; c < 3; c++) fF[c] = (uO[c] * rO) * fI + (uN[c] * rN) * (1 - fI) - en[c] * dq;
         const double flux = dot3r(fF[0], fF[1], fF[2], nx, ny, nz);
-        surf += own ? flux : -flux;
+        if (own) surf += flux;
+        else surf -= flux;
     }
-    const double r = surf - vol;
+    double r[1] = {surf};
+    volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
     const double ap = (-1.0 / d.dt) * d.cV[n];
-    rho_out[n] = (r + rho * ap) / ap;
+    rho_out[n] = (r[0] + rho * ap) / ap;
 }
 
 // ---------------------------------------------------------------------------
@@ -125,117 +160,105 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds[];
-    double* G = lds + (size_t)le * 12 * NP;  // rows: Ux Uy Uz theta, 3 directions each
+    double* F = lds + (size_t)le * 15 * NP;  // two flux rows (6), then Jin (9)
+    double* J = F + 6 * NP;
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
     const double R = d.R, gam = d.gamma;
 
     double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
+    double fr[4][3];  // own flux rows: U_x U_y U_z (tensor rows), theta (vector)
     if (active) {
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
         th = s.T[n] + d.T0;
         pp = d.P0 * pow((rho * th * R) / d.P0, gam) - d.p_ref[n];
         const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
-        double J[9];
-        for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * d.nb + n];
-        for (int row = 0; row < 4; row++) {
-            double f0, f1, f2;
-            if (row < 3) {
-                f0 = fc[row] * u[0] + (row == 0 ? pp : 0.0 * pp);
-                f1 = fc[row] * u[1] + (row == 1 ? pp : 0.0 * pp);
-                f2 = fc[row] * u[2] + (row == 2 ? pp : 0.0 * pp);
+        for (int row = 0; row < 3; row++)
+            for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
+        for (int c = 0; c < 3; c++) fr[3][c] = fc[c] * th;
+        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
+    }
+
+    double res[4] = {0, 0, 0, 0};
+    if (active) {
+        const double gRT = gam * R;
+        for (int g = 0; g < 6; g++) {
+            if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+            const EFace ef = d.eface[(size_t)e * 6 + g];
+            if (!((ef.meta >> 4) & 1)) continue;
+            const bool own = (ef.meta >> 3) & 1;
+            const bool ghost = (ef.meta >> 5) & 1;
+            const int og = ef.meta & 7;
+            int a, b;
+            face_ab(g, i, j, k, a, b);
+            const double wgt = face_weight(d, own ? g : og, a, b);
+            const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                                  d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+            double orho0, orho, ou[3], oth, opp;
+            if (!ghost) {
+                const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+                orho0 = s.rho[m]; orho = rho1[m];
+                ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+                oth = s.T[m] + d.T0;
+                opp = d.P0 * pow((orho * oth * R) / d.P0, gam) - d.p_ref[m];
             } else {
-                f0 = fc[0] * th; f1 = fc[1] * th; f2 = fc[2] * th;
+                ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
+                orho0 = rho0;
+                orho = rho;
+                oth = th;
+                opp = pp;
             }
-            G[(row * 3 + 0) * NP + q] = f0 * J[0] + f1 * J[6] + f2 * J[8];
-            G[(row * 3 + 1) * NP + q] = f0 * J[3] + f1 * J[1] + f2 * J[7];
-            G[(row * 3 + 2) * NP + q] = f0 * J[5] + f1 * J[4] + f2 * J[2];
+            const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
+            const double rO = own ? rho : orho, rN = own ? orho : rho;
+            double uO[3], uN[3];
+            for (int c = 0; c < 3; c++) { uO[c] = own ? u[c] : ou[c]; uN[c] = own ? ou[c] : u[c]; }
+            const double thO = own ? th : oth, thN = own ? oth : th;
+            const double pO = own ? pp : opp, pN = own ? opp : pp;
+            const double fI = ghost ? 0.0 : 0.5;
+            const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
+            const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
+            const double lam = ((magO * fI + magN * (1 - fI)) +
+                                (sqrt(gRT * thO) * fI + sqrt(gRT * thN) * (1 - fI))) / 2;
+            const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+            const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+            const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
+            const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
+            for (int row = 0; row < 3; row++) {
+                double fF[3];
+                for (int c = 0; c < 3; c++) {
+                    const double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
+                    const double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
+                    const double dq = lam * (rN * uN[c] - rO * uO[c]);
+                    fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
+                }
+                const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
+                if (own) res[row] += flux;
+                else res[row] -= flux;
+            }
+            {
+                const double dq = lam * (rN * thN - rO * thO);
+                double fF[3];
+                for (int c = 0; c < 3; c++) fF[c] = (fcO[c] * thO) * fI + (fcN[c] * thN) * (1 - fI) - en[c] * dq;
+                const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
+                if (own) res[3] += flux;
+                else res[3] -= flux;
+            }
         }
     }
+    // volume terms in two passes of two flux rows through LDS
+    if (active)
+        for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[0][c]; F[(3 + c) * NP + q] = fr[1][c]; }
+    __syncthreads();
+    if (active)
+        volume_reference_order<2, 0>(res, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
+    __syncthreads();
+    if (active)
+        for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[2][c]; F[(3 + c) * NP + q] = fr[3][c]; }
     __syncthreads();
     if (!active) return;
+    volume_reference_order<2, 2>(res + 2, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
 
-    double res[4];
-    {
-        const double* D0 = d.D[0];
-        const double* D1 = d.D[1];
-        const double* D2 = d.D[2];
-        for (int row = 0; row < 4; row++) {
-            const double* G0 = G + (row * 3 + 0) * NP;
-            const double* G1 = G + (row * 3 + 1) * NP;
-            const double* G2 = G + (row * 3 + 2) * NP;
-            double v = 0;
-            for (int a = 0; a < NX; a++) v += D0[a * NX + i] * G0[a * NY * NZ + j * NZ + k];
-            for (int b = 0; b < NY; b++) v += D1[b * NY + j] * G1[i * NY * NZ + b * NZ + k];
-            for (int c = 0; c < NZ; c++) v += D2[c * NZ + k] * G2[i * NY * NZ + j * NZ + c];
-            res[row] = -v;
-        }
-    }
-
-    const double gRT = gam * R;
-    for (int g = 0; g < 6; g++) {
-        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
-        const EFace ef = d.eface[(size_t)e * 6 + g];
-        if (!((ef.meta >> 4) & 1)) continue;
-        const bool own = (ef.meta >> 3) & 1;
-        const bool ghost = (ef.meta >> 5) & 1;
-        const int og = ef.meta & 7;
-        int a, b;
-        face_ab(g, i, j, k, a, b);
-        const double wgt = face_weight(d, own ? g : og, a, b);
-        const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
-                              d.gFN[3 * (size_t)ef.facet + 2] * wgt};
-        double orho0, orho, ou[3], oth, opp;
-        if (!ghost) {
-            const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
-            orho0 = s.rho[m]; orho = rho1[m];
-            ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
-            oth = s.T[m] + d.T0;
-            opp = d.P0 * pow((orho * oth * R) / d.P0, gam) - d.p_ref[m];
-        } else {
-            ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
-            orho0 = rho0;
-            orho = rho;
-            oth = th;
-            opp = pp;
-        }
-        const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
-        const double rO = own ? rho : orho, rN = own ? orho : rho;
-        double uO[3], uN[3];
-        for (int c = 0; c < 3; c++) { uO[c] = own ? u[c] : ou[c]; uN[c] = own ? ou[c] : u[c]; }
-        const double thO = own ? th : oth, thN = own ? oth : th;
-        const double pO = own ? pp : opp, pN = own ? opp : pp;
-        const double fI = ghost ? 0.0 : 0.5;
-        const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
-        const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
-        const double lam = ((magO * fI + magN * (1 - fI)) +
-                            (sqrt(gRT * thO) * fI + sqrt(gRT * thN) * (1 - fI))) / 2;
-        const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
-        const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
-        const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
-        const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
-        // momentum: tensor flux, q = rho^{n+1} U
-        for (int row = 0; row < 3; row++) {
-            double fF[3];
-            for (int c = 0; c < 3; c++) {
-                const double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
-                const double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
-                const double dq = lam * (rN * uN[c] - rO * uO[c]);
-                fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
-            }
-            const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
-            res[row] += own ? flux : -flux;
-        }
-        // potential temperature: vector flux, q = rho^{n+1} theta
-        {
-            const double dq = lam * (rN * thN - rO * thO);
-            double fF[3];
-            for (int c = 0; c < 3; c++) fF[c] = (fcO[c] * thO) * fI + (fcN[c] * thN) * (1 - fI) - en[c] * dq;
-            const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
-            res[3] += own ? flux : -flux;
-        }
-    }
     const double cv = d.cV[n];
     const double ap0 = (-1.0 / d.dt) * cv;
     const double ap = ap0 * rho;
@@ -259,8 +282,8 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
     int threads = epb * NP;
     threads = (threads + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
-    size_t lds_a = (size_t)epb * 3 * NP * sizeof(double);
-    size_t lds_b = (size_t)epb * 12 * NP * sizeof(double);
+    size_t lds_a = (size_t)epb * 12 * NP * sizeof(double);
+    size_t lds_b = (size_t)epb * 15 * NP * sizeof(double);
     hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;

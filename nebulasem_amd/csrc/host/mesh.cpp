@@ -6,12 +6,16 @@
 // tensor.cpp), because mesh numbering decisions (coplanarity, corner picking,
 // normal orientation) and the node geometry are compared bit for bit.
 #include "mesh.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 namespace nsem {
 
@@ -154,6 +158,17 @@ void erase_indices(std::vector<T>& data, const IntVec& idx) {
     }
     data.swap(keep);
 }
+
+struct PhaseTimer {
+    bool on = std::getenv("NSEM_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char* what) {
+        if (!on) return;
+        auto n = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[nsem] %-28s %8.3f s\n", what, std::chrono::duration<double>(n - t).count());
+        t = n;
+    }
+};
 
 bool set_err(std::string* err, const std::string& msg) {
     if (err) *err = msg;
@@ -544,6 +559,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
     std::vector<IntVec>& F = m.facets;
     std::vector<IntVec>& C = m.cells;
 
+    PhaseTimer tm;
     // ---- addBoundaryCells (mesh.cpp:55-109)
     m.nbcs = C.size();
     m.FOC.assign(F.size(), kMaxInt);
@@ -574,6 +590,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
                 if (faceInB[fi]) m.has_boundary[m.FOC[fi]] = 1;
             }
 
+    tm.mark("addBoundaryCells");
     // ---- fixHexCells, conforming hexahedra (mesh.cpp:161-446)
     MeshWork w{V, F};
     m.FMC.assign(F.size(), 0);
@@ -679,6 +696,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
         m.faceid.push_back(b);
     }
 
+    tm.mark("fixHexCells");
     // ---- calcGeometry (mesh.cpp:450-577), planar case
     m.fc.assign(F.size(), V3());
     m.cc.assign(C.size(), V3());
@@ -732,6 +750,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
         m.cc[i] = m.fc[fi];
     }
 
+    tm.mark("calcGeometry");
     // ---- removeBoundary("delete") (mesh.cpp:581-669)
     {
         IntVec fs = m.boundaries["delete"];
@@ -771,6 +790,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
         // has_boundary is indexed by interior cell; interior cells are never
         // removed here, so it keeps its meaning.
     }
+    tm.mark("removeBoundary");
     // ---- removeUnusedVertices (mesh.cpp:674-721)
     {
         IntVec used(V.size(), 0);
@@ -799,6 +819,7 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
             erase_indices(V, rv);
         }
     }
+    tm.mark("removeUnusedVertices");
     // ---- drop empty and "interior" patches (field.cpp:148-155)
     for (auto it = m.boundaries.begin(); it != m.boundaries.end();) {
         if (it->second.empty() || it->first.find("interior") != std::string::npos)
@@ -947,6 +968,7 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
         return set_err(err, "mesh too large for 32-bit node indices");
     const Int gALL = Int(geo.nall_field), gBCSf = Int(geo.nbcs_field);
     for (int d = 0; d < 3; d++) build_basis(order[d] + 1, geo.basis[d]);
+    PhaseTimer tm;
 
     // cell -> face CSR (field.cpp:178-193)
     geo.faceIndices0.resize(geo.ncells);
@@ -991,15 +1013,19 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
             if (c.size() < 4 || fids.size() != c.size())
                 return set_err(err, "a cell lost its faces (every boundary face unnamed?)");
             Int id0 = fids[0], id1 = id0 ^ 1;
+            int n0 = 0, n1 = 0;
+            for (size_t i = 0; i < c.size(); i++) { n0 += fids[i] == id0; n1 += fids[i] == id1; }
+            if (n0 != 1 || n1 != 1) return set_err(err, "non-conforming cells are not supported yet");
+        }
+        parallel_for(m.nbcs, [&](uint64_t cb, uint64_t ce) {
+        for (Int ci = (Int)cb; ci < (Int)ce; ci++) {
+            const IntVec& c = m.cells[ci];
+            const IntVec& fids = m.faceid[ci];
+            Int id0 = fids[0], id1 = id0 ^ 1;
             IntVec f1, f2;
             for (size_t i = 0; i < c.size(); i++) {
-                if (fids[i] == id0) {
-                    if (!f1.empty()) return set_err(err, "non-conforming cells are not supported yet");
-                    f1 = m.facets[c[i]];
-                } else if (fids[i] == id1) {
-                    if (!f2.empty()) return set_err(err, "non-conforming cells are not supported yet");
-                    f2 = m.facets[c[i]];
-                }
+                if (fids[i] == id0) f1 = m.facets[c[i]];
+                else if (fids[i] == id1) f2 = m.facets[c[i]];
             }
             IntVec vpi;
             w.hex_corners(f1, f2, vpi);
@@ -1029,6 +1055,8 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
                         geo.cV[index] *= wgt;
                     }
         }
+        });
+        tm.mark("node coordinates");
         // face-node maps (dg.cpp:327-410)
         const Int face_map[6] = {0, NPZ - 1, 0, NPY - 1, 0, NPX - 1};
         for (Int ci = 0; ci < m.nbcs; ci++) {
@@ -1093,10 +1121,13 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
                 }
             }
         }
+        tm.mark("face-node maps");
         // inverse-transposed Jacobian, pseudo-inverse for collapsed
         // directions (dg.cpp:412-476)
         geo.Jinv.assign(geo.nall_field, T9());
-        for (Int ci = 0; ci < m.nbcs; ci++)
+        tm.mark("Jacobian storage");
+        parallel_for(m.nbcs, [&](uint64_t cb, uint64_t ce) {
+        for (Int ci = (Int)cb; ci < (Int)ce; ci++)
             for (Int ii = 0; ii < NPX; ii++)
                 for (Int jj = 0; jj < NPY; jj++)
                     for (Int kk = 0; kk < NPZ; kk++) {
@@ -1125,7 +1156,9 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
                         Ji = tmul(Ji, JiT);
                         geo.Jinv[index] = trn(Ji);
                     }
+        });
     }
+    tm.mark("Jacobians");
     // facet interpolation factor (field.cpp:254-269), single domain
     geo.fI.assign(nfn, 0.0);
     for (uint64_t k = 0; k < nfn; k++) {

@@ -14,14 +14,22 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 OM_NEUMANN, OM_SYMMETRY, OM_DIRICHLET = 0, 1, 2
 
 
+ORACLE_ULP_SO = os.path.join(ROOT, "oracle", "liboracle_ulp.so")
+
+
 def build_oracle():
     src = os.path.join(ROOT, "oracle", "nsem_oracle.c")
     subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o",
                            ORACLE_SO, src, "-lm"])
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-DOM_PERTURB_POW",
+                           "-o", ORACLE_ULP_SO, src, "-lm"])
     return ORACLE_SO
 
 
 _lib = None
+
+
+_lib_ulp = None
 
 
 def lib():
@@ -31,6 +39,16 @@ def lib():
             build_oracle()
         _lib = C.CDLL(ORACLE_SO)
     return _lib
+
+
+def lib_ulp():
+    """The oracle built with the 1-ulp pow perturbation (see nsem_oracle.c)."""
+    global _lib_ulp
+    if _lib_ulp is None:
+        if not os.path.exists(ORACLE_ULP_SO):
+            build_oracle()
+        _lib_ulp = C.CDLL(ORACLE_ULP_SO)
+    return _lib_ulp
 
 
 P = C.POINTER
@@ -121,8 +139,9 @@ def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -
     return e
 
 
-def euler_split_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps):
-    st = lib().om_euler_split_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
+def euler_split_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps, perturb_pow=False):
+    L = lib_ulp() if perturb_pow else lib()
+    st = L.om_euler_split_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
                                     _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))
     return st
 
@@ -135,3 +154,8 @@ def fv_poisson(om, bc_kind, bc_val, U, p, tol, max_it):
     st = lib().om_fv_poisson(C.byref(om.s), _p(bc_kind, PI), _p(bc_val, PD), _p(U, PD), _p(p, PD),
                              C.c_double(tol), C.c_int32(max_it), C.byref(it), C.byref(r0), C.byref(r))
     return st, it.value, r0.value, r.value
+
+
+def euler_ssprk3_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps):
+    return lib().om_euler_ssprk3_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
+                                       _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))

@@ -15,6 +15,6 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def oracle():
     import _oracle
-    if not os.path.exists(_oracle.ORACLE_SO):
+    if not (os.path.exists(_oracle.ORACLE_SO) and os.path.exists(_oracle.ORACLE_ULP_SO)):
         _oracle.build_oracle()
     return _oracle

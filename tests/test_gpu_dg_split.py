@@ -1,6 +1,14 @@
-"""HIP split-step (reference-mode) Euler path vs the oracle: every field
-within 1e-12 of the oracle (norm-relative L2 and max-norm), on the same decks
-whose oracle results are bit-identical to the reference build."""
+"""HIP split-step (reference-mode) Euler path vs the oracle, on the same
+decks whose oracle results are bit-identical to the reference build.
+
+Tolerance.  rho and theta: 1e-12 norm-relative (L2 and max-norm).  U and p'
+are small differences of large terms (p' = P0 (rho theta R/P0)^gamma - p_ref,
+the momentum tendency a near-balance of pressure gradient and buoyancy); the
+device's pow() is not glibc's and can differ by an ulp, which moves them by
+far more than 1e-12.  Their bound is therefore measured, not assumed: the
+oracle is run a second time with its equation-of-state pow() moved by a
+pseudo-random +-1 ulp, and the device must stay within 10x that 1-ulp
+spread (and never above 1e-8)."""
 import sys
 
 import numpy as np
@@ -43,17 +51,24 @@ def test_split_steps_match_oracle(oracle, tmp_path, case):
     print(f"[gpu] split steps {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field
+    pert = [a.copy() for a in (rho, U, T, p)]
     done = 0
     for s in case["check_steps"][:2]:
         oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, s - done)
+        oracle.euler_split_steps(om, prm, p_ref, rho_ref, *pert, s - done, perturb_pow=True)
         dg.split_steps(s - done)
         done = s
         g_rho, g_U, g_T, g_p = dg.get_state()
-        for name, a, b in (("rho", g_rho, rho[:nb]), ("U", g_U, U[:3 * nb]), ("T", g_T, T[:nb]),
-                           ("p", g_p, p[:nb])):
-            l2, linf = relerr(a, b)
-            print(f"  step {s} {name}: rel L2 {l2:.2e} rel max {linf:.2e}", flush=True)
-            assert l2 <= TOL and linf <= TOL, (s, name, l2, linf)
+        for name, a, b, c, n in (("rho", g_rho, rho, pert[0], nb), ("U", g_U, U, pert[1], 3 * nb),
+                                 ("T", g_T, T, pert[2], nb), ("p", g_p, p, pert[3], nb)):
+            l2, linf = relerr(a, b[:n])
+            b2, bi = relerr(c[:n], b[:n])
+            tol2, toli = TOL, TOL
+            if name in ("U", "p"):
+                tol2, toli = min(max(TOL, 10 * b2), 1e-8), min(max(TOL, 10 * bi), 1e-8)
+            print(f"  step {s} {name}: rel L2 {l2:.2e} rel max {linf:.2e} "
+                  f"(1-ulp pow spread {b2:.1e} / {bi:.1e})", flush=True)
+            assert l2 <= tol2 and linf <= toli, (s, name, l2, linf, tol2, toli)
     dg.close()
 
 
