@@ -31,9 +31,9 @@ def test_rk3_fp64_matches_oracle(oracle, tmp_path, case):
     for name, a, b in (("rho", g_rho, rho[:nb]), ("U", g_U, U[:3 * nb]), ("T", g_T, T[:nb])):
         l2, linf = relerr(a, b)
         assert l2 <= tol[name], (name, l2, linf)
-    # p' recomputed by the exact-difference formula vs P0 pow(...) - p_ref
-    l2, _ = relerr(g_p, p[:nb])
-    assert l2 <= 1e-9
+    # p' by the exact-difference formula vs P0 pow(...) - p_ref: a difference
+    # of ~1e5 Pa numbers, compared relative to the full pressure
+    assert np.linalg.norm(g_p - p[:nb]) / np.linalg.norm(p_ref[:nb]) <= 1e-13
     dg.close()
 
 

@@ -1,14 +1,15 @@
 """HIP split-step (reference-mode) Euler path vs the oracle, on the same
 decks whose oracle results are bit-identical to the reference build.
 
-Tolerance.  rho and theta: 1e-12 norm-relative (L2 and max-norm).  U and p'
-are small differences of large terms (p' = P0 (rho theta R/P0)^gamma - p_ref,
-the momentum tendency a near-balance of pressure gradient and buoyancy); the
-device's pow() is not glibc's and can differ by an ulp, which moves them by
-far more than 1e-12.  Their bound is therefore measured, not assumed: the
-oracle is run a second time with its equation-of-state pow() moved by a
-pseudo-random +-1 ulp, and the device must stay within 10x that 1-ulp
-spread (and never above 1e-8)."""
+Tolerance.  rho and theta: 1e-12 norm-relative (L2 and max-norm).
+p' = P0 (rho theta R/P0)^gamma - p_ref is a difference of two ~1e5 Pa
+numbers, so it is compared relative to the full pressure: 1e-13 of p_ref.
+U's tendency is a near-balance of pressure gradient and buoyancy, and the
+device's pow() is not glibc's (it can differ by an ulp), which moves U by far
+more than 1e-12.  Its bound is therefore measured, not assumed: the oracle is
+run a second time with its equation-of-state pow() moved by a pseudo-random
++-1 ulp, and the device must stay within 10x that 1-ulp spread (and never
+above 1e-8)."""
 import sys
 
 import numpy as np
@@ -64,8 +65,13 @@ def test_split_steps_match_oracle(oracle, tmp_path, case):
             l2, linf = relerr(a, b[:n])
             b2, bi = relerr(c[:n], b[:n])
             tol2, toli = TOL, TOL
-            if name in ("U", "p"):
+            if name == "U":
                 tol2, toli = min(max(TOL, 10 * b2), 1e-8), min(max(TOL, 10 * bi), 1e-8)
+            if name == "p":
+                pr = p_ref[:nb]
+                l2 = np.linalg.norm(a - b[:n]) / np.linalg.norm(pr)
+                linf = np.abs(a - b[:n]).max() / np.abs(pr).max()
+                tol2, toli = 1e-13, 1e-13
             print(f"  step {s} {name}: rel L2 {l2:.2e} rel max {linf:.2e} "
                   f"(1-ulp pow spread {b2:.1e} / {bi:.1e})", flush=True)
             assert l2 <= tol2 and linf <= toli, (s, name, l2, linf, tol2, toli)
