@@ -180,10 +180,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fv", action="store_true")
     ap.add_argument("--fv-n", type=int, default=128)
+    ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
+                    help="only time the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    from nebulasem_amd import stream_copy
+    if args.copy_only:
+        nbytes = args.copy_only << 20
+        t = stream_copy(nbytes, max(args.steps, 1))
+        print(json.dumps({"kernel": "stream_copy_kernel", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "seconds_per_copy": t, "GBps": 2 * nbytes / t / 1e9}))
+        return
+    t_copy = stream_copy(1 << 30, 10)
+    stream_bw = 2 * (1 << 30) / t_copy
+    log(f"stream copy 1 GiB: {stream_bw / 1e9:.0f} GB/s (read + write)")
     ndiv = tuple(int(x) for x in args.ndiv.split(","))
     N = 7
     dt = 0.005
@@ -221,7 +233,9 @@ def main():
                 "kernel": "rk_stage_kernel<float,float,8,8,8>",
                 "bytes_per_node": b32, "nodes_per_launch": nb,
                 "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
-                "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK}
+                "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK,
+                "stream_copy_GBps": stream_bw / 1e9,
+                "frac_of_stream_copy": achieved / stream_bw}
     fv = None
     if not args.no_fv:
         fv = fv_bench(args.fv_n)

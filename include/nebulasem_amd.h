@@ -182,6 +182,14 @@ int nsem_fv_apply(nsem_fv* fv, const double* x, double* y);
 int nsem_fv_solve(nsem_fv* fv, double* x, int32_t mode, double tol, int32_t max_it,
                   nsem_fv_result* res);
 
+/* ----------------------------------------------------------- measurement --
+ * Device-to-device copy of `bytes` (a multiple of 16) through a plain
+ * one-float4-per-thread kernel, `reps` timed launches after one warm-up:
+ * the achievable HBM bandwidth the kernels are compared with, and the
+ * known-size kernel the HBM byte counters are calibrated on.  No reference
+ * counterpart. */
+int nsem_stream_copy(uint64_t bytes, int32_t reps, double* seconds_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,13 @@ class Mesh:
             pass
 
 
+def stream_copy(nbytes, reps=10):
+    """Seconds per launch of a plain device copy of nbytes (nsem_stream_copy)."""
+    sec = C.c_double()
+    check(lib().nsem_stream_copy(C.c_uint64(nbytes), reps, C.byref(sec)))
+    return sec.value
+
+
 def grid_generate(spec_path, grid_path):
     check(lib().nsem_grid_generate(spec_path.encode(), grid_path.encode()))
 
