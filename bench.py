@@ -230,7 +230,7 @@ def main():
     achieved = b32 * nb / res[32]["stage_s"]
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "rk_stage_kernel<float,float,8,8,8>",
+                "kernel": "rk_stage_kernel_n7<float,float,6>",
                 "bytes_per_node": b32, "nodes_per_launch": nb,
                 "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
                 "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK,

@@ -87,8 +87,12 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     if (g.np < 2) return nsem_fail(NSEM_EINVAL, "DG path needs polynomial order >= 1");
     if (g.np > 1024) return nsem_fail(NSEM_EUNSUPPORTED, "at most 1024 nodes per element");
     if (names.size() > (size_t)kMaxPatches) return nsem_fail(NSEM_EUNSUPPORTED, "too many boundary patches");
-    if (15 * (size_t)g.np * sizeof(double) > 65536)
-        return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the LDS staging (order <= 7 in 3D)");
+    {
+        const int epb = g.np >= 256 ? 1 : 256 / (int)g.np;
+        if (15 * (size_t)g.np * sizeof(double) > 65536 || g.np > 512 ||
+            rk_lds_bytes(g.npx, g.npy, g.npz, epb, sizeof(double)) > 65536)
+            return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the LDS staging (order <= 7 in 3D)");
+    }
     if (!(prm->dt > 0)) return nsem_fail(NSEM_EINVAL, "dt must be positive");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)

@@ -35,6 +35,18 @@ struct RkArgs {
     int check;
     int* flag;
 };
+// LDS of the RK stage kernel, in values of the residual type: per element
+// max(15 flux rows, 10 primitive rows + 5 face-flux rows) x NP, plus the three
+// derivative matrices once per work-group.
+__host__ __device__ inline int rk_lds_per_element(int nx, int ny, int nz) {
+    const int np = nx * ny * nz, nfn = 2 * (nx * ny + nx * nz + ny * nz);
+    const int a = 15 * np, b = 10 * np + 5 * nfn;
+    return a > b ? a : b;
+}
+inline size_t rk_lds_bytes(int nx, int ny, int nz, int epb, size_t value_size) {
+    return ((size_t)epb * rk_lds_per_element(nx, ny, nz) + nx * nx + ny * ny + nz * nz) * value_size;
+}
+
 template <typename R, typename S>
 hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st);
 

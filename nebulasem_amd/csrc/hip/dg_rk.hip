@@ -25,6 +25,8 @@
 // derivative along the three GLL lines, Rusanov surface terms from the
 // neighbour traces (global, L2/MALL resident), update fused.  Work-groups are
 // mapped to elements XCD-contiguously so face neighbours share an L2.
+#include <cstdlib>
+
 #include "dg_kernels.hpp"
 
 namespace nsem {
@@ -89,10 +91,85 @@ __device__ __forceinline__ double ld_delta(const S* d, uint64_t i) { return d ? 
 
 }  // namespace
 
+// Face-node task t (0 .. NFN-1) of an element: face group g, face-local
+// (a, b) and the element-local node.  Groups 0,1 (k = 0, NZ-1) have NX*NY
+// nodes, 2,3 (j) NX*NZ, 4,5 (i) NY*NZ, enumerated group by group in the
+// (a, b) order of face_ab.
+__device__ __forceinline__ void face_task(int t, int NX, int NY, int NZ, int& g, int& fa, int& fb,
+                                          int& node) {
+    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
+    int i, j, k, l;
+    if (t < 2 * FK) {
+        g = t >= FK; l = t - g * FK;
+        fa = l / NY; fb = l - fa * NY;
+        i = fa; j = fb; k = g ? NZ - 1 : 0;
+    } else if (t < 2 * (FK + FJ)) {
+        l = t - 2 * FK;
+        const int h = l >= FJ;
+        g = 2 + h; l -= h * FJ;
+        fa = l / NZ; fb = l - fa * NZ;
+        i = fa; j = h ? NY - 1 : 0; k = fb;
+    } else {
+        l = t - 2 * (FK + FJ);
+        const int h = l >= FI;
+        g = 4 + h; l -= h * FI;
+        fa = l / NZ; fb = l - fa * NZ;
+        i = h ? NX - 1 : 0; j = fa; k = fb;
+    }
+    node = i * NY * NZ + j * NZ + k;
+}
+
+// Inverse of face_task for a node on face group g.
+__device__ __forceinline__ int face_task_index(int g, int fa, int fb, int NX, int NY, int NZ) {
+    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
+    switch (g) {
+        case 0: return fa * NY + fb;
+        case 1: return FK + fa * NY + fb;
+        case 2: return 2 * FK + fa * NZ + fb;
+        case 3: return 2 * FK + FJ + fa * NZ + fb;
+        case 4: return 2 * (FK + FJ) + fa * NZ + fb;
+        default: return 2 * (FK + FJ) + FI + fa * NZ + fb;
+    }
+}
+
+// Neighbour state of one face-node task, loaded before the volume term so
+// that its latency overlaps the element's own loads.
+template <typename R>
+struct NbrLoad {
+    R rp, m[3], tp, rref, pref;
+};
+
+template <typename R, typename S>
+__device__ __forceinline__ void load_nbr(const DgDev& d, const RkArgs<S>& a, uint64_t m, NbrLoad<R>& L) {
+    L.rp = (R)(a.q[0][m] + ld_delta(a.din[0], m));
+    L.m[0] = (R)(a.q[1][m] + ld_delta(a.din[1], m));
+    L.m[1] = (R)(a.q[2][m] + ld_delta(a.din[2], m));
+    L.m[2] = (R)(a.q[3][m] + ld_delta(a.din[3], m));
+    L.tp = (R)(a.q[4][m] + ld_delta(a.din[4], m));
+    L.rref = (R)d.rho_ref[m];
+    L.pref = (R)d.p_ref[m];
+}
+
+// Primitive rows kept in LDS for the face tasks (the owner side of each face
+// node): rho', m (3), Theta', U (3), theta - T0, p'.  c and |U| are
+// recomputed from them with the same operations as make_prim.
+constexpr int kPrimRows = 10;
+
+// Work-group of one element (NP threads, one per node; several elements per
+// group when NP < 256).  Phases, each closed by a barrier:
+//   1. own state -> primitives -> contravariant fluxes (15 rows) in LDS;
+//      face tasks issue their neighbour loads
+//   2. volume term along the three GLL lines -> registers
+//   3. own primitives -> LDS (over the flux rows)
+//   4. one thread per face node (6 faces x (N+1)^2 = 384 tasks at N = 7,
+//      one face per wavefront): Rusanov flux -> LDS
+//   5. each node subtracts/adds the fluxes of its faces in face-group order,
+//      then the stage update.
 template <typename R, typename S, int NXc, int NYc, int NZc>
-__global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
+__device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
+    const int NFN = 2 * (NX * NY + NX * NZ + NY * NZ);
     const int epb = blockDim.x / NP;
     const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
     // XCD-contiguous element mapping: work-group b runs on XCD b % 8
@@ -101,11 +178,29 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
     const uint32_t e = bmap * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ unsigned char lds_raw[];
-    R* G = reinterpret_cast<R*>(lds_raw) + (size_t)le * 15 * NP;
+    const int per_elem = rk_lds_per_element(NX, NY, NZ);
+    R* G = reinterpret_cast<R*>(lds_raw) + (size_t)le * per_elem;
+    R* FL = G + kPrimRows * NP;                      // face fluxes [5][NFN]
+    R* Dl = reinterpret_cast<R*>(lds_raw) + (size_t)epb * per_elem;  // D0, D1, D2
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
     const uint64_t nb = d.nb;
     const R T0 = (R)d.T0, gam = (R)d.gamma, gR = (R)(d.gamma * d.R);
+
+    for (int t = threadIdx.x; t < NX * NX + NY * NY + NZ * NZ; t += blockDim.x) {
+        const int s1 = NX * NX, s2 = s1 + NY * NY;
+        Dl[t] = (R)(t < s1 ? d.D[0][t] : t < s2 ? d.D[1][t - s1] : d.D[2][t - s2]);
+    }
+
+    // neighbour loads of this thread's first face task
+    NbrLoad<R> nl{};
+    if (active && q < NFN) {
+        int g, fa, fb, fnode;
+        face_task(q, NX, NY, NZ, g, fa, fb, fnode);
+        const EFace ef = d.eface[(size_t)e * 6 + g];
+        if (((ef.meta >> 4) & 1) && !((ef.meta >> 5) & 1))
+            load_nbr<R, S>(d, a, face_node(ef.other, ef.meta & 7, fa, fb, NX, NY, NZ), nl);
+    }
 
     Prim<R> P{};
     double base[5] = {0, 0, 0, 0, 0};
@@ -147,93 +242,112 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
         }
     }
     __syncthreads();
-    if (!active) return;
 
-    R res[5];  // volume - surface (= -r of the reference)
-    {
-        const double* D0 = d.D[0];
-        const double* D1 = d.D[1];
-        const double* D2 = d.D[2];
-        R d0[NXc ? NXc : kMaxN], d1[NYc ? NYc : kMaxN], d2[NZc ? NZc : kMaxN];
-        for (int t = 0; t < NX; t++) d0[t] = (R)D0[t * NX + i];
-        for (int t = 0; t < NY; t++) d1[t] = (R)D1[t * NY + j];
-        for (int t = 0; t < NZ; t++) d2[t] = (R)D2[t * NZ + k];
+    R res[5] = {0, 0, 0, 0, 0};  // volume - surface (= -r of the reference)
+    if (active) {
+        const R* D0 = Dl;
+        const R* D1 = Dl + NX * NX;
+        const R* D2 = D1 + NY * NY;
         for (int row = 0; row < 5; row++) {
             const R* G0 = G + (row * 3 + 0) * NP;
             const R* G1 = G + (row * 3 + 1) * NP;
             const R* G2 = G + (row * 3 + 2) * NP;
             R v = 0;
-            for (int t = 0; t < NX; t++) v += d0[t] * G0[t * NY * NZ + j * NZ + k];
-            for (int t = 0; t < NY; t++) v += d1[t] * G1[i * NY * NZ + t * NZ + k];
-            for (int t = 0; t < NZ; t++) v += d2[t] * G2[i * NY * NZ + j * NZ + t];
+            for (int t = 0; t < NX; t++) v += D0[t * NX + i] * G0[t * NY * NZ + j * NZ + k];
+            for (int t = 0; t < NY; t++) v += D1[t * NY + j] * G1[i * NY * NZ + t * NZ + k];
+            for (int t = 0; t < NZ; t++) v += D2[t * NZ + k] * G2[i * NY * NZ + j * NZ + t];
             res[row] = v;
         }
     }
+    __syncthreads();
+    if (active) {
+        const R pr[kPrimRows] = {P.rp, P.m[0], P.m[1], P.m[2], P.tp, P.u[0], P.u[1], P.u[2], P.thp, P.pp};
+        for (int r = 0; r < kPrimRows; r++) G[r * NP + q] = pr[r];
+    }
+    __syncthreads();
+
+    if (active) {
+        for (int t = q; t < NFN; t += NP) {
+            int g, fa, fb, fnode;
+            face_task(t, NX, NY, NZ, g, fa, fb, fnode);
+            const EFace ef = d.eface[(size_t)e * 6 + g];
+            if (!((ef.meta >> 4) & 1)) continue;
+            const bool own = (ef.meta >> 3) & 1;
+            const bool ghost = (ef.meta >> 5) & 1;
+            const int og = ef.meta & 7;
+            const double wgt = face_weight(d, own ? g : og, fa, fb);
+            const R nv[3] = {(R)(d.gFN[3 * (size_t)ef.facet] * wgt), (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
+                             (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
+            Prim<R> A{};
+            A.rp = G[0 * NP + fnode];
+            A.m[0] = G[1 * NP + fnode]; A.m[1] = G[2 * NP + fnode]; A.m[2] = G[3 * NP + fnode];
+            A.tp = G[4 * NP + fnode];
+            A.u[0] = G[5 * NP + fnode]; A.u[1] = G[6 * NP + fnode]; A.u[2] = G[7 * NP + fnode];
+            A.thp = G[8 * NP + fnode];
+            A.pp = G[9 * NP + fnode];
+            A.c = r_sqrt<R>(gR * (T0 + A.thp));
+            A.umag = r_sqrt<R>(A.u[0] * A.u[0] + A.u[1] * A.u[1] + A.u[2] * A.u[2]);
+            Prim<R> Q{};
+            if (!ghost) {
+                NbrLoad<R> L = nl;
+                if (t != q) load_nbr<R, S>(d, a, face_node(ef.other, og, fa, fb, NX, NY, NZ), L);
+                Q.rp = L.rp; Q.m[0] = L.m[0]; Q.m[1] = L.m[1]; Q.m[2] = L.m[2]; Q.tp = L.tp;
+                make_prim(Q, L.rref, L.pref, T0, gam, gR);
+            } else {
+                Q = A;
+                const BcTable& bt = *d.bc;
+                const int pi = ef.meta >> 8;
+                const int kind = bt.kind[1][pi];
+                if (kind == BC_SYMMETRY) {
+                    sym_r(A.u, nv, Q.u);
+                    sym_r(A.m, nv, Q.m);
+                } else if (kind == BC_DIRICHLET) {
+                    Q.u[0] = (R)bt.value[1][pi]; Q.u[1] = (R)bt.value[2][pi]; Q.u[2] = (R)bt.value[3][pi];
+                    const R rho = (R)d.rho_ref[(uint64_t)e * NP + fnode] + A.rp;
+                    Q.m[0] = rho * Q.u[0]; Q.m[1] = rho * Q.u[1]; Q.m[2] = rho * Q.u[2];
+                }
+                Q.umag = r_sqrt<R>(Q.u[0] * Q.u[0] + Q.u[1] * Q.u[1] + Q.u[2] * Q.u[2]);
+            }
+            const Prim<R>& O = own ? A : Q;
+            const Prim<R>& N = own ? Q : A;
+            const R fI = ghost ? R(0) : R(0.5);
+            const R fJ = R(1) - fI;
+            const R lam = ((O.umag * fI + N.umag * fJ) + (O.c * fI + N.c * fJ)) * R(0.5);
+            const R mn = r_sqrt<R>(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            const R en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+            R flux[5];
+            // rho: avg(m).n - lambda (rho'_N - rho'_O) |n|
+            flux[0] = (O.m[0] * fI + N.m[0] * fJ) * nv[0] + (O.m[1] * fI + N.m[1] * fJ) * nv[1] +
+                      (O.m[2] * fI + N.m[2] * fJ) * nv[2] - lam * (N.rp - O.rp) * mn;
+            // momentum: avg(m (x) U + p'I).n - e_r lambda (dm . n)
+            const R dmn = lam * ((N.m[0] - O.m[0]) * nv[0] + (N.m[1] - O.m[1]) * nv[1] + (N.m[2] - O.m[2]) * nv[2]);
+            const R un_O = O.u[0] * nv[0] + O.u[1] * nv[1] + O.u[2] * nv[2];
+            const R un_N = N.u[0] * nv[0] + N.u[1] * nv[1] + N.u[2] * nv[2];
+            for (int r = 0; r < 3; r++)
+                flux[1 + r] = (O.m[r] * un_O + O.pp * nv[r]) * fI + (N.m[r] * un_N + N.pp * nv[r]) * fJ - en[r] * dmn;
+            // Theta: avg(m theta).n - lambda (Theta'_N - Theta'_O) |n|
+            {
+                const R mnO = O.m[0] * nv[0] + O.m[1] * nv[1] + O.m[2] * nv[2];
+                const R mnN = N.m[0] * nv[0] + N.m[1] * nv[1] + N.m[2] * nv[2];
+                flux[4] = (mnO * T0 + mnO * O.thp) * fI + (mnN * T0 + mnN * N.thp) * fJ - lam * (N.tp - O.tp) * mn;
+            }
+            for (int v = 0; v < 5; v++) FL[v * NFN + t] = flux[v];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
 
     for (int g = 0; g < 6; g++) {
         if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
-        const EFace ef = d.eface[(size_t)e * 6 + g];
-        if (!((ef.meta >> 4) & 1)) continue;
-        const bool own = (ef.meta >> 3) & 1;
-        const bool ghost = (ef.meta >> 5) & 1;
-        const int og = ef.meta & 7;
+        const uint32_t meta = d.eface[(size_t)e * 6 + g].meta;
+        if (!((meta >> 4) & 1)) continue;
         int fa, fb;
         face_ab(g, i, j, k, fa, fb);
-        const double wgt = face_weight(d, own ? g : og, fa, fb);
-        const R nv[3] = {(R)(d.gFN[3 * (size_t)ef.facet] * wgt), (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
-                         (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
-        Prim<R> Q{};
-        if (!ghost) {
-            const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
-            Q.rp = (R)(a.q[0][m] + ld_delta(a.din[0], m));
-            Q.m[0] = (R)(a.q[1][m] + ld_delta(a.din[1], m));
-            Q.m[1] = (R)(a.q[2][m] + ld_delta(a.din[2], m));
-            Q.m[2] = (R)(a.q[3][m] + ld_delta(a.din[3], m));
-            Q.tp = (R)(a.q[4][m] + ld_delta(a.din[4], m));
-            make_prim(Q, (R)d.rho_ref[m], (R)d.p_ref[m], T0, gam, gR);
-        } else {
-            Q = P;
-            const BcTable& bt = *d.bc;
-            const int pi = ef.meta >> 8;
-            const int kind = bt.kind[1][pi];
-            if (kind == BC_SYMMETRY) {
-                sym_r(P.u, nv, Q.u);
-                sym_r(P.m, nv, Q.m);
-            } else if (kind == BC_DIRICHLET) {
-                Q.u[0] = (R)bt.value[1][pi]; Q.u[1] = (R)bt.value[2][pi]; Q.u[2] = (R)bt.value[3][pi];
-                const R rho = rref + P.rp;
-                Q.m[0] = rho * Q.u[0]; Q.m[1] = rho * Q.u[1]; Q.m[2] = rho * Q.u[2];
-            }
-            Q.umag = r_sqrt<R>(Q.u[0] * Q.u[0] + Q.u[1] * Q.u[1] + Q.u[2] * Q.u[2]);
-        }
-        const Prim<R>& O = own ? P : Q;
-        const Prim<R>& N = own ? Q : P;
-        const R fI = ghost ? R(0) : R(0.5);
-        const R fJ = R(1) - fI;
-        const R lam = ((O.umag * fI + N.umag * fJ) + (O.c * fI + N.c * fJ)) * R(0.5);
-        const R mn = r_sqrt<R>(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-        const R en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
-        R flux[5];
-        // rho: avg(m).n - lambda (rho'_N - rho'_O) |n|
-        flux[0] = (O.m[0] * fI + N.m[0] * fJ) * nv[0] + (O.m[1] * fI + N.m[1] * fJ) * nv[1] +
-                  (O.m[2] * fI + N.m[2] * fJ) * nv[2] - lam * (N.rp - O.rp) * mn;
-        // momentum: avg(m (x) U + p'I).n - e_r lambda (dm . n)
-        const R dmn = lam * ((N.m[0] - O.m[0]) * nv[0] + (N.m[1] - O.m[1]) * nv[1] + (N.m[2] - O.m[2]) * nv[2]);
-        for (int r = 0; r < 3; r++) {
-            const R un_O = O.u[0] * nv[0] + O.u[1] * nv[1] + O.u[2] * nv[2];
-            const R un_N = N.u[0] * nv[0] + N.u[1] * nv[1] + N.u[2] * nv[2];
-            flux[1 + r] = (O.m[r] * un_O + O.pp * nv[r]) * fI + (N.m[r] * un_N + N.pp * nv[r]) * fJ - en[r] * dmn;
-        }
-        // Theta: avg(m theta).n - lambda (Theta'_N - Theta'_O) |n|
-        {
-            const R mnO = O.m[0] * nv[0] + O.m[1] * nv[1] + O.m[2] * nv[2];
-            const R mnN = N.m[0] * nv[0] + N.m[1] * nv[1] + N.m[2] * nv[2];
-            flux[4] = (mnO * T0 + mnO * O.thp) * fI + (mnN * T0 + mnN * N.thp) * fJ - lam * (N.tp - O.tp) * mn;
-        }
-        if (own)
-            for (int v = 0; v < 5; v++) res[v] -= flux[v];
+        const int t = face_task_index(g, fa, fb, NX, NY, NZ);
+        if ((meta >> 3) & 1)
+            for (int v = 0; v < 5; v++) res[v] -= FL[v * NFN + t];
         else
-            for (int v = 0; v < 5; v++) res[v] += flux[v];
+            for (int v = 0; v < 5; v++) res[v] += FL[v * NFN + t];
     }
     // L(q) = (volume - surface)/cV + buoyancy; increments and base
     R cvn;
@@ -261,6 +375,19 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
 }
 
 template <typename R, typename S, int X, int Y, int Z>
+__global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
+    rk_stage_body<R, S, X, Y, Z>(d, a);
+}
+
+// N = 7 (512 nodes, one element per work-group): registers capped so that two
+// work-groups (16 waves) share a CU in both precisions.
+template <typename R, typename S, int W>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
+rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
+    rk_stage_body<R, S, 8, 8, 8>(d, a);
+}
+
+template <typename R, typename S, int X, int Y, int Z>
 static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
     const int NP = d.np;
     if (NP > 512) return hipErrorInvalidValue;
@@ -268,8 +395,23 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st
     int threads = (epb * NP + 63) / 64 * 64;
     unsigned blocks = (d.nelem + epb - 1) / epb;
     blocks = (blocks + 7) / 8 * 8;  // whole XCD chunks
-    const size_t lds = (size_t)epb * 15 * NP * sizeof(R);
-    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
+    const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
+    if (lds > 65536) return hipErrorInvalidValue;
+    if constexpr (X == 8 && Y == 8 && Z == 8) {
+        static const int w = [] {
+            const char* s = getenv("NSEM_RK_WAVES");
+            return s ? atoi(s) : 0;
+        }();
+        // measured on the 64x64x32 N=7 workload: FP32 fastest at 6 waves/SIMD
+        // (three work-groups per CU, 4 VGPRs spilled), FP64 at 4 (no spills)
+        if (w == 6 || (w == 0 && sizeof(R) == 4))
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 6>), dim3(blocks), dim3(threads), lds, st, d, a);
+        else if (w == 5)
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 5>), dim3(blocks), dim3(threads), lds, st, d, a);
+        else
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 4>), dim3(blocks), dim3(threads), lds, st, d, a);
+    } else
+        hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
     return hipGetLastError();
 }
 
