@@ -122,8 +122,7 @@ __device__ __forceinline__ void face_ab(int g, int i, int j, int k, int& a, int&
 // Node of element `cell` on face group g at face-local (a,b).
 __device__ __forceinline__ uint64_t face_node(uint32_t cell, int g, int a, int b, int NX, int NY,
                                               int NZ) {
-    const int fm[6] = {0, NZ - 1, 0, NY - 1, 0, NX - 1};
-    int v = fm[g];
+    const int v = (g & 1) ? (g < 2 ? NZ - 1 : g < 4 ? NY - 1 : NX - 1) : 0;
     int i, j, k;
     if (g < 2) { i = a; j = b; k = v; }
     else if (g < 4) { i = a; j = v; k = b; }

@@ -29,6 +29,11 @@
 
 #include "dg_kernels.hpp"
 
+// The method-of-lines path has no bit-level counterpart in the reference, so
+// products and sums are fused into FMAs here (the split reference-order
+// kernels keep the build-wide -ffp-contract=off).
+#pragma clang fp contract(fast)
+
 namespace nsem {
 
 namespace {
@@ -132,22 +137,44 @@ __device__ __forceinline__ int face_task_index(int g, int fa, int fb, int NX, in
     }
 }
 
-// Neighbour state of one face-node task, loaded before the volume term so
-// that its latency overlaps the element's own loads.
+// Everything one face-node task reads from global memory, gathered before
+// the volume term so that its latency overlaps the element's own loads:
+// the face entry, the weighted facet normal and the neighbour's state (or,
+// on a boundary face, the owner node's background density).
 template <typename R>
-struct NbrLoad {
+struct FaceTask {
+    uint32_t meta;
+    int g, fa, fb, fnode;
+    R nv[3];
     R rp, m[3], tp, rref, pref;
 };
 
 template <typename R, typename S>
-__device__ __forceinline__ void load_nbr(const DgDev& d, const RkArgs<S>& a, uint64_t m, NbrLoad<R>& L) {
-    L.rp = (R)(a.q[0][m] + ld_delta(a.din[0], m));
-    L.m[0] = (R)(a.q[1][m] + ld_delta(a.din[1], m));
-    L.m[1] = (R)(a.q[2][m] + ld_delta(a.din[2], m));
-    L.m[2] = (R)(a.q[3][m] + ld_delta(a.din[3], m));
-    L.tp = (R)(a.q[4][m] + ld_delta(a.din[4], m));
-    L.rref = (R)d.rho_ref[m];
-    L.pref = (R)d.p_ref[m];
+__device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S>& a, uint32_t e, int t,
+                                                  int NX, int NY, int NZ, FaceTask<R>& F) {
+    face_task(t, NX, NY, NZ, F.g, F.fa, F.fb, F.fnode);
+    const EFace ef = d.eface[(size_t)e * 6 + F.g];
+    F.meta = ef.meta;
+    if (!((ef.meta >> 4) & 1)) return;
+    const bool own = (ef.meta >> 3) & 1;
+    const int og = ef.meta & 7;
+    const double wgt = face_weight(d, own ? F.g : og, F.fa, F.fb);
+    F.nv[0] = (R)(d.gFN[3 * (size_t)ef.facet] * wgt);
+    F.nv[1] = (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt);
+    F.nv[2] = (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt);
+    if (!((ef.meta >> 5) & 1)) {
+        const uint64_t m = face_node(ef.other, og, F.fa, F.fb, NX, NY, NZ);
+        F.rp = (R)(a.q[0][m] + ld_delta(a.din[0], m));
+        F.m[0] = (R)(a.q[1][m] + ld_delta(a.din[1], m));
+        F.m[1] = (R)(a.q[2][m] + ld_delta(a.din[2], m));
+        F.m[2] = (R)(a.q[3][m] + ld_delta(a.din[3], m));
+        F.tp = (R)(a.q[4][m] + ld_delta(a.din[4], m));
+        F.rref = (R)d.rho_ref[m];
+        F.pref = (R)d.p_ref[m];
+    } else {
+        F.rref = (R)d.rho_ref[(uint64_t)e * (NX * NY * NZ) + F.fnode];
+        F.pref = 0;
+    }
 }
 
 // Primitive rows kept in LDS for the face tasks (the owner side of each face
@@ -192,27 +219,26 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
         Dl[t] = (R)(t < s1 ? d.D[0][t] : t < s2 ? d.D[1][t - s1] : d.D[2][t - s2]);
     }
 
-    // neighbour loads of this thread's first face task
-    NbrLoad<R> nl{};
-    if (active && q < NFN) {
-        int g, fa, fb, fnode;
-        face_task(q, NX, NY, NZ, g, fa, fb, fnode);
-        const EFace ef = d.eface[(size_t)e * 6 + g];
-        if (((ef.meta >> 4) & 1) && !((ef.meta >> 5) & 1))
-            load_nbr<R, S>(d, a, face_node(ef.other, ef.meta & 7, fa, fb, NX, NY, NZ), nl);
-    }
+    // global reads of this thread's first face task
+    FaceTask<R> ft{};
+    if (active && q < NFN) prepare_face_task<R, S>(d, a, e, q, NX, NY, NZ, ft);
 
     Prim<R> P{};
-    double base[5] = {0, 0, 0, 0, 0};
-    double din[5] = {0, 0, 0, 0, 0};
-    R rref = 0, pref = 0;
+    // the part of the stage output known before the residual:
+    // b din (+ q on the last stage); the output is pre + b dt L(q + din)
+    double pre[5] = {0, 0, 0, 0, 0};
+    R rref = 0, pref = 0, cvn = 1;
     if (active) {
+        double base[5], din[5];
         for (int v = 0; v < 5; v++) {
             base[v] = a.q[v][n];
-            din[v] = ld_delta(a.din[v], n);
+            din[v] = a.din[v] ? (double)a.din[v][n] : 0.0;
+            pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
         }
         rref = (R)d.rho_ref[n];
         pref = (R)d.p_ref[n];
+        if constexpr (sizeof(R) == 4) cvn = d.cV32[n];
+        else cvn = (R)d.cV[n];
         P.rp = (R)(base[0] + din[0]);
         P.m[0] = (R)(base[1] + din[1]); P.m[1] = (R)(base[2] + din[2]); P.m[2] = (R)(base[3] + din[3]);
         P.tp = (R)(base[4] + din[4]);
@@ -268,16 +294,13 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
 
     if (active) {
         for (int t = q; t < NFN; t += NP) {
-            int g, fa, fb, fnode;
-            face_task(t, NX, NY, NZ, g, fa, fb, fnode);
-            const EFace ef = d.eface[(size_t)e * 6 + g];
-            if (!((ef.meta >> 4) & 1)) continue;
-            const bool own = (ef.meta >> 3) & 1;
-            const bool ghost = (ef.meta >> 5) & 1;
-            const int og = ef.meta & 7;
-            const double wgt = face_weight(d, own ? g : og, fa, fb);
-            const R nv[3] = {(R)(d.gFN[3 * (size_t)ef.facet] * wgt), (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
-                             (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
+            if (t != q) prepare_face_task<R, S>(d, a, e, t, NX, NY, NZ, ft);
+            const FaceTask<R>& F = ft;
+            if (!((F.meta >> 4) & 1)) continue;
+            const bool own = (F.meta >> 3) & 1;
+            const bool ghost = (F.meta >> 5) & 1;
+            const int fnode = F.fnode;
+            const R* nv = F.nv;
             Prim<R> A{};
             A.rp = G[0 * NP + fnode];
             A.m[0] = G[1 * NP + fnode]; A.m[1] = G[2 * NP + fnode]; A.m[2] = G[3 * NP + fnode];
@@ -289,21 +312,19 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             A.umag = r_sqrt<R>(A.u[0] * A.u[0] + A.u[1] * A.u[1] + A.u[2] * A.u[2]);
             Prim<R> Q{};
             if (!ghost) {
-                NbrLoad<R> L = nl;
-                if (t != q) load_nbr<R, S>(d, a, face_node(ef.other, og, fa, fb, NX, NY, NZ), L);
-                Q.rp = L.rp; Q.m[0] = L.m[0]; Q.m[1] = L.m[1]; Q.m[2] = L.m[2]; Q.tp = L.tp;
-                make_prim(Q, L.rref, L.pref, T0, gam, gR);
+                Q.rp = F.rp; Q.m[0] = F.m[0]; Q.m[1] = F.m[1]; Q.m[2] = F.m[2]; Q.tp = F.tp;
+                make_prim(Q, F.rref, F.pref, T0, gam, gR);
             } else {
                 Q = A;
                 const BcTable& bt = *d.bc;
-                const int pi = ef.meta >> 8;
+                const int pi = F.meta >> 8;
                 const int kind = bt.kind[1][pi];
                 if (kind == BC_SYMMETRY) {
                     sym_r(A.u, nv, Q.u);
                     sym_r(A.m, nv, Q.m);
                 } else if (kind == BC_DIRICHLET) {
                     Q.u[0] = (R)bt.value[1][pi]; Q.u[1] = (R)bt.value[2][pi]; Q.u[2] = (R)bt.value[3][pi];
-                    const R rho = (R)d.rho_ref[(uint64_t)e * NP + fnode] + A.rp;
+                    const R rho = F.rref + A.rp;
                     Q.m[0] = rho * Q.u[0]; Q.m[1] = rho * Q.u[1]; Q.m[2] = rho * Q.u[2];
                 }
                 Q.umag = r_sqrt<R>(Q.u[0] * Q.u[0] + Q.u[1] * Q.u[1] + Q.u[2] * Q.u[2]);
@@ -350,9 +371,6 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             for (int v = 0; v < 5; v++) res[v] += FL[v * NFN + t];
     }
     // L(q) = (volume - surface)/cV + buoyancy; increments and base
-    R cvn;
-    if constexpr (sizeof(R) == 4) cvn = d.cV32[n];
-    else cvn = (R)d.cV[n];
     const R icv = R(1) / cvn;
     R Lq[5];
     for (int v = 0; v < 5; v++) Lq[v] = res[v] * icv;
@@ -363,9 +381,9 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     }
     const double dt = d.dt;
     for (int v = 0; v < 5; v++) {
-        const double incr = a.b * (din[v] + dt * (double)Lq[v]);
-        if (a.last) a.qout[v][n] = base[v] + incr;
-        else a.dout[v][n] = (S)incr;
+        const double out = pre[v] + a.b * dt * (double)Lq[v];
+        if (a.last) a.qout[v][n] = out;
+        else a.dout[v][n] = (S)out;
     }
     if (a.check) {
         bool bad = false;
@@ -403,8 +421,10 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st
             return s ? atoi(s) : 0;
         }();
         // measured on the 64x64x32 N=7 workload: FP32 fastest at 6 waves/SIMD
-        // (three work-groups per CU, 4 VGPRs spilled), FP64 at 4 (no spills)
-        if (w == 6 || (w == 0 && sizeof(R) == 4))
+        // (three work-groups per CU), FP64 at 4 (two work-groups per CU)
+        if (w == 8)
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 8>), dim3(blocks), dim3(threads), lds, st, d, a);
+        else if (w == 6 || (w == 0 && sizeof(R) == 4))
             hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 6>), dim3(blocks), dim3(threads), lds, st, d, a);
         else if (w == 5)
             hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 5>), dim3(blocks), dim3(threads), lds, st, d, a);
