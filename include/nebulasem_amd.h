@@ -121,6 +121,12 @@ void nsem_dg_destroy(nsem_dg* d);
  * method-of-lines state; getting it returns whichever path ran last. */
 int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p);
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
+/* Per-step diagnostics of the current state (split or method-of-lines),
+ * reduced on the device over the interior nodes: out[6] = Courant number
+ * max, min, avg (Mesh::calc_courant, field.cpp:440-448), total mass
+ * sum rho cV, energy sum rho cV (g.x + |U|^2/2 + (p/P0)^(R/cp) theta cv) and
+ * volume sum cV (euler.cpp:261-283, reduce_* field.h:953-1006). */
+int nsem_dg_diagnostics(nsem_dg* d, double* out);
 /* Hydrostatic reference p_ref, rho_ref at the interior nodes (euler.cpp:121-123). */
 int nsem_dg_get_background(const nsem_dg* d, double* p_ref, double* rho_ref);
 /* Initial density from theta' and p' as euler.cpp:136-149. */

@@ -239,6 +239,12 @@ class DGEuler:
         check(lib().nsem_dg_get_conserved(self._h, _pd(rp), _pd(m), _pd(tp)))
         return rp, m, tp
 
+    def diagnostics(self):
+        """(Courant max, min, avg, mass, energy, volume) of the current state."""
+        out = np.empty(6)
+        check(lib().nsem_dg_diagnostics(self.handle, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def synchronize(self):
         check(lib().nsem_dg_synchronize(self._h))
 

@@ -51,6 +51,8 @@ struct nsem_dg {
     // method-of-lines state: FP64 conserved perturbations (rho', m, Theta'),
     // double-buffered, and stage increments in FP64 and FP32
     DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
+    // diagnostics: g.x per node, block partials, result
+    DevBuf gh, diag_slots, diag_out;
     int mcur = 0;
     bool mol_current = false;
     // profiling of the dominant kernels
@@ -171,7 +173,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         }
     }
     // metric, volumes, background (euler.cpp:104-131)
-    std::vector<double> JcV(9 * nb), cV(nb), pref(nb), rref(nb);
+    std::vector<double> JcV(9 * nb), cV(nb), pref(nb), rref(nb), ghv(nb, 0.0);
     std::vector<float> JcV32(9 * nb), cV32(nb);
     const double R = prm->cp - prm->cv, gam = prm->cp / prm->cv;
     parallel_for(nb, [&](uint64_t nbeg, uint64_t nend) {
@@ -187,6 +189,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             double gh = prm->gravity[2] * x[2] + 0.0;
             gh = prm->gravity[1] * x[1] + gh;
             gh = prm->gravity[0] * x[0] + gh;
+            ghv[n] = gh;
             pref[n] = prm->P0 * std::pow(1.0 + gh / (prm->cp * prm->T0), prm->cp / R);
             rref[n] = (prm->P0 / (R * prm->T0)) * std::pow(pref[n] / prm->P0, 1 / gam);
         } else {
@@ -202,7 +205,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     if ((e = upload(d->JcV, JcV, s)) != hipSuccess || (e = upload(d->cV, cV, s)) != hipSuccess ||
         (e = upload(d->JcV32, JcV32, s)) != hipSuccess || (e = upload(d->cV32, cV32, s)) != hipSuccess ||
         (e = upload(d->p_ref, pref, s)) != hipSuccess || (e = upload(d->rho_ref, rref, s)) != hipSuccess ||
-        (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess)
+        (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess ||
+        (e = upload(d->gh, ghv, s)) != hipSuccess)
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     for (int dir = 0; dir < 3; dir++) {
         if ((e = upload(d->D[dir], g.basis[dir].dpsi, s)) != hipSuccess ||
@@ -222,6 +226,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             HIP_TRY(d->d64[a][v].alloc(nb * sizeof(double)));
             HIP_TRY(d->d32[a][v].alloc(nb * sizeof(float)));
         }
+    HIP_TRY(d->diag_slots.alloc(kDiagBlocks * 6 * sizeof(double)));
+    HIP_TRY(d->diag_out.alloc(6 * sizeof(double)));
     HIP_TRY(d->flag.alloc(sizeof(int)));
     HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -310,6 +316,24 @@ int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p) 
         HIP_TRY(hipMemcpyAsync(dst[v], d->st[d->cur][v].p, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     for (uint64_t n = 0; n < nb; n++) { U[3 * n] = ux[n]; U[3 * n + 1] = uy[n]; U[3 * n + 2] = uz[n]; }
+    return NSEM_OK;
+}
+
+int nsem_dg_diagnostics(nsem_dg* d, double* out) {
+    if (!d || !out) return nsem_fail(NSEM_EINVAL, "null argument");
+    DiagState ds{};
+    ds.mol = d->mol_current ? 1 : 0;
+    ds.rho = d->st[d->cur][0].as<double>();
+    ds.ux = d->st[d->cur][1].as<double>();
+    ds.uy = d->st[d->cur][2].as<double>();
+    ds.uz = d->st[d->cur][3].as<double>();
+    ds.T = d->st[d->cur][4].as<double>();
+    ds.p = d->st[d->cur][5].as<double>();
+    for (int v = 0; v < 5; v++) ds.q[v] = d->mq[d->mcur][v].as<double>();
+    HIP_TRY(launch_diagnostics(d->dev, ds, d->gh.as<double>(), d->diag_slots.as<double>(),
+                               d->diag_out.as<double>(), d->stream));
+    HIP_TRY(hipMemcpyAsync(out, d->diag_out.p, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
     return NSEM_OK;
 }
 

@@ -50,6 +50,24 @@ inline size_t rk_lds_bytes(int nx, int ny, int nz, int epb, size_t value_size) {
 template <typename R, typename S>
 hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st);
 
+// Diagnostics input: the split (primitive) state or the method-of-lines
+// conserved state, whichever is current.
+struct DiagState {
+    int mol;
+    const double* rho;
+    const double* ux;
+    const double* uy;
+    const double* uz;
+    const double* T;
+    const double* p;
+    const double* q[5];
+};
+constexpr int kDiagBlocks = 1024;
+// out (device, 6 doubles): Courant max, min, avg, mass, energy, volume;
+// slots: kDiagBlocks * 6 doubles.
+hipError_t launch_diagnostics(const DgDev& d, const DiagState& s, const double* gh, double* slots,
+                              double* out, hipStream_t st);
+
 // One split forward-Euler step (two fused element kernels).
 hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
                              hipStream_t st);

@@ -24,6 +24,8 @@
  *       applyImplicitBCs field.h:2541-2582, SolveT PCG solve.cpp:259-280,403-417
  *       with the reference's stopping rule sqrt(|sum z^2| / |sum x^2|) (solve.cpp:11-35).
  *   om_euler_hydrostatic   euler.cpp:104-149 background and initial density.
+ *   om_euler_diagnostics   euler.cpp:261-283, calc_courant field.cpp:440-448,
+ *       reduce_* field.h:953-1006: Courant max/min/avg, mass, energy, volume.
  *   om_euler_ssprk3_steps  method-of-lines SSP-RK3 (no reference counterpart),
  *       composed from the pinned operators above; "parity unpinned" as a
  *       whole: pinned only through its operators and the conservation and
@@ -278,6 +280,35 @@ void om_euler_hydrostatic(const om_mesh* m, const om_euler* e, const double* T, 
         double pt = p[i] + p_ref[i];
         rho[i] = (e->P0 / (R * (T[i] + e->T0))) * pow(pt / e->P0, 1 / gam);
     }
+}
+
+/* Per-step diagnostics of apps/euler/euler.cpp:261-283 over the interior
+ * nodes, summed in node order as the reference's reduce_* do in a serial
+ * build (field.h:953-1006): out = {Courant max, min, avg (calc_courant,
+ * field.cpp:440-448: |U| dt / cV^(1/3)), mass sum rho cV, energy
+ * sum rho cV (gh + |U|^2/2 + ((p'+p_ref)/P0)^(R/cp) (T'+T0) cv), volume}.
+ * T is theta' (the reference adds T0 before the step and removes it after). */
+void om_euler_diagnostics(const om_mesh* m, const om_euler* e, const double* p_ref, const double* rho,
+                          const double* U, const double* T, const double* p, double* out) {
+    const double R = e->cp - e->cv;
+    double cmax = -1e30, cmin = 1e30, csum = 0, mass = 0, energy = 0, vol = 0;
+    for (uint64_t i = 0; i < m->nbf; i++) {
+        const double* u = &U[3 * i];
+        const double um = sqrt(dot3(u, u));
+        const double c = um * e->dt / pow(m->cV[i], 1.0 / 3);
+        if (fabs(c) > cmax) cmax = fabs(c);
+        if (fabs(c) < cmin) cmin = fabs(c);
+        csum += c;
+        mass += rho[i] * m->cV[i];
+        const double gh = e->buoyancy ? dot3(e->g, &m->cC[3 * i]) : 0.0;
+        const double th = T[i] + e->T0;
+        double sf = gh + 0.5 * dot3(u, u) + pow((p[i] + p_ref[i]) / e->P0, R / e->cp) * th * e->cv;
+        sf = rho[i] * m->cV[i] * sf;
+        energy += sf;
+        vol += m->cV[i];
+    }
+    out[0] = cmax; out[1] = cmin; out[2] = csum / (double)m->nbf;
+    out[3] = mass; out[4] = energy; out[5] = vol;
 }
 
 /* The reference's split forward-Euler step, nsteps times.  All arrays are

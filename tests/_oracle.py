@@ -120,6 +120,14 @@ def euler_hydrostatic(om, params, T, p):
     return p_ref, rho_ref, rho
 
 
+def euler_diagnostics(om, params, p_ref, rho, U, T, p):
+    """(Courant max, min, avg, mass, energy, volume) as euler.cpp:261-283."""
+    out = np.empty(6)
+    lib().om_euler_diagnostics(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho, PD), _p(U, PD),
+                               _p(T, PD), _p(p, PD), _p(out, PD))
+    return out
+
+
 def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -9.80606),
                 buoyancy=1):
     """bcs: dict field -> (kinds int32 array, values float64 array)."""

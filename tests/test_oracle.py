@@ -109,3 +109,31 @@ def test_poisson_against_dense_solve(oracle, tmp_path):
             A[c1, c1] += fD[k]  # Dirichlet p = 0 on xmin
     x = np.linalg.solve(A, b)
     assert np.allclose(p[:n], x, rtol=1e-9, atol=1e-11 * np.abs(x).max())
+
+
+def _log_numbers(lines):
+    """Courant max/min/avg and mass/energy/volume losses from the reference's
+    last two per-step log lines (euler.cpp:273-279)."""
+    c = [ln for ln in lines if "Courant number" in ln][-1].split()
+    m = [ln for ln in lines if "Mass loss" in ln][-1].split()
+    cour = [c[c.index("Max:") + 1], c[c.index("Min:") + 1], c[c.index("Avg:") + 1]]
+    loss = [m[m.index("Mass") + 2], m[m.index("Energy") + 2], m[m.index("Volume") + 2]]
+    return cour, loss
+
+
+@pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
+def test_oracle_diagnostics_match_reference_log(oracle, tmp_path, case):
+    """Courant numbers (%g) and mass/energy/volume losses (%.12g) after the
+    run's last step, printed by the reference build, reproduced digit for
+    digit from the oracle's state and its diagnostics restatement."""
+    mesh, om, prm, p_ref, rho_ref, rho, U, T, p = run_oracle_euler(oracle, tmp_path, case)
+    d0 = oracle.euler_diagnostics(om, prm, p_ref, rho, U, T, p)
+    steps = max(case["check_steps"])
+    assert oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, steps) == 0
+    d1 = oracle.euler_diagnostics(om, prm, p_ref, rho, U, T, p)
+    cour, loss = _log_numbers(FX["euler"][case["name"]]["log_tail"])
+    assert ["%g" % v for v in d1[:3]] == cour
+    mine = ["%.12g" % ((d0[k] - d1[k]) / d0[k]) for k in (3, 4, 5)]
+    assert mine[1:] == loss[1:], (mine, loss)
+    # the mass loss is round-off (~1e-16): equal to the printed digits too
+    assert mine[0] == loss[0], (mine, loss)
