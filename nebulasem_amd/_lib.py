@@ -242,7 +242,7 @@ class DGEuler:
     def diagnostics(self):
         """(Courant max, min, avg, mass, energy, volume) of the current state."""
         out = np.empty(6)
-        check(lib().nsem_dg_diagnostics(self.handle, out.ctypes.data_as(C.c_void_p)))
+        check(lib().nsem_dg_diagnostics(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
     def synchronize(self):

@@ -137,4 +137,45 @@ __device__ __forceinline__ double face_weight(const DgDev& d, int owner_g, int a
     return d.w[1][a] * d.w[2][b] / 4;
 }
 
+// Face-node task t (0 .. NFN-1) of an element: face group g, face-local
+// (a, b) and the element-local node.  Groups 0,1 (k = 0, NZ-1) have NX*NY
+// nodes, 2,3 (j) NX*NZ, 4,5 (i) NY*NZ, enumerated group by group in the
+// (a, b) order of face_ab.
+__device__ __forceinline__ void face_task(int t, int NX, int NY, int NZ, int& g, int& fa, int& fb,
+                                          int& node) {
+    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
+    int i, j, k, l;
+    if (t < 2 * FK) {
+        g = t >= FK; l = t - g * FK;
+        fa = l / NY; fb = l - fa * NY;
+        i = fa; j = fb; k = g ? NZ - 1 : 0;
+    } else if (t < 2 * (FK + FJ)) {
+        l = t - 2 * FK;
+        const int h = l >= FJ;
+        g = 2 + h; l -= h * FJ;
+        fa = l / NZ; fb = l - fa * NZ;
+        i = fa; j = h ? NY - 1 : 0; k = fb;
+    } else {
+        l = t - 2 * (FK + FJ);
+        const int h = l >= FI;
+        g = 4 + h; l -= h * FI;
+        fa = l / NZ; fb = l - fa * NZ;
+        i = h ? NX - 1 : 0; j = fa; k = fb;
+    }
+    node = i * NY * NZ + j * NZ + k;
+}
+
+// Inverse of face_task for a node on face group g.
+__device__ __forceinline__ int face_task_index(int g, int fa, int fb, int NX, int NY, int NZ) {
+    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
+    switch (g) {
+        case 0: return fa * NY + fb;
+        case 1: return FK + fa * NY + fb;
+        case 2: return 2 * FK + fa * NZ + fb;
+        case 3: return 2 * FK + FJ + fa * NZ + fb;
+        case 4: return 2 * (FK + FJ) + fa * NZ + fb;
+        default: return 2 * (FK + FJ) + FI + fa * NZ + fb;
+    }
+}
+
 }  // namespace nsem

@@ -24,6 +24,8 @@ struct SplitOut {
 // One fused residual + SSP-RK3 stage.  q: FP64 base state (5 SoA arrays:
 // rho', m_x, m_y, m_z, Theta'); din: stage increment in (null at stage 1);
 // out: b*(din + dt L(q + din)) to dout, or q + that to qout when last.
+// The tangent-linear instantiation (residual type Dual<1>, S = double) also
+// reads tq (tangent of q), tdin and writes tdout or tqout.
 template <typename S>
 struct RkArgs {
     const double* q[5];
@@ -34,6 +36,10 @@ struct RkArgs {
     int last;
     int check;
     int* flag;
+    const double* tq[5];
+    const double* tdin[5];
+    double* tdout[5];
+    double* tqout[5];
 };
 // LDS of the RK stage kernel, in values of the residual type: per element
 // max(15 flux rows, 10 primitive rows + 5 face-flux rows) x NP, plus the three

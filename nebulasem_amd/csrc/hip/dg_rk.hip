@@ -25,117 +25,58 @@
 // derivative along the three GLL lines, Rusanov surface terms from the
 // neighbour traces (global, L2/MALL resident), update fused.  Work-groups are
 // mapped to elements XCD-contiguously so face neighbours share an L2.
+// The method-of-lines path has no bit-level counterpart in the reference, so
+// products and sums are fused into FMAs here, in the headers' inline physics
+// too (the split reference-order kernels keep the build-wide
+// -ffp-contract=off).
+#pragma clang fp contract(fast)
+
 #include <cstdlib>
 
 #include "dg_kernels.hpp"
-
-// The method-of-lines path has no bit-level counterpart in the reference, so
-// products and sums are fused into FMAs here (the split reference-order
-// kernels keep the build-wide -ffp-contract=off).
-#pragma clang fp contract(fast)
+#include "dg_physics.hpp"
 
 namespace nsem {
 
 namespace {
 
-template <typename R>
-struct Prim {
-    R rp, m[3], tp;   // perturbation conserved: rho', m, Theta'
-    R u[3];           // velocity
-    R thp;            // theta - T0
-    R pp;             // p'
-    R c;              // sqrt(gamma R theta)
-    R umag;
-};
-
-template <typename R>
-__device__ __forceinline__ R r_expm1(R x);
-template <>
-__device__ __forceinline__ double r_expm1<double>(double x) { return expm1(x); }
-template <>
-__device__ __forceinline__ float r_expm1<float>(float x) { return expm1f(x); }
-template <typename R>
-__device__ __forceinline__ R r_log1p(R x);
-template <>
-__device__ __forceinline__ double r_log1p<double>(double x) { return log1p(x); }
-template <>
-__device__ __forceinline__ float r_log1p<float>(float x) { return log1pf(x); }
-template <typename R>
-__device__ __forceinline__ R r_sqrt(R x);
-template <>
-__device__ __forceinline__ double r_sqrt<double>(double x) { return sqrt(x); }
-template <>
-__device__ __forceinline__ float r_sqrt<float>(float x) { return sqrtf(x); }
-
-template <typename R>
-__device__ __forceinline__ void make_prim(Prim<R>& P, R rho_ref, R p_ref, R T0, R gam, R gR) {
-    const R rho = rho_ref + P.rp;
-    const R ir = R(1) / rho;
-    P.u[0] = P.m[0] * ir; P.u[1] = P.m[1] * ir; P.u[2] = P.m[2] * ir;
-    P.thp = (P.tp - T0 * P.rp) * ir;
-    P.pp = p_ref * r_expm1<R>(gam * r_log1p<R>(P.tp / (rho_ref * T0)));
-    P.c = r_sqrt<R>(gR * (T0 + P.thp));
-    P.umag = r_sqrt<R>(P.u[0] * P.u[0] + P.u[1] * P.u[1] + P.u[2] * P.u[2]);
+// Node-array access.  With O32 the byte offset is formed in 32 bits (node
+// arrays below 4 GiB each), so the load takes the array base from scalar
+// registers and one shared 32-bit offset instead of a 64-bit address per
+// array.
+template <bool O32, typename T>
+__device__ __forceinline__ T ldn(const T* base, uint64_t i) {
+    if constexpr (O32)
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
+    else
+        return base[i];
+}
+template <bool O32, typename T>
+__device__ __forceinline__ void stn(T* base, uint64_t i, T v) {
+    if constexpr (O32)
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T))) = v;
+    else
+        base[i] = v;
 }
 
-// SYMMETRY ghost of a vector in arithmetic type R.
-template <typename R>
-__device__ __forceinline__ void sym_r(const R* p, const R* n, R* r) {
-    const R mn = r_sqrt<R>(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    const R e0 = n[0] / mn, e1 = n[1] / mn, e2 = n[2] / mn;
-    const R dn = e0 * p[0] + e1 * p[1] + e2 * p[2];
-    r[0] = p[0] - e0 * dn; r[1] = p[1] - e1 * dn; r[2] = p[2] - e2 * dn;
-    const R mr = r_sqrt<R>(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-    if (mr <= R(1e-7)) return;
-    const R s = r_sqrt<R>(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) / mr;
-    r[0] *= s; r[1] *= s; r[2] *= s;
-}
+template <bool O32, typename S>
+__device__ __forceinline__ double ld_delta(const S* d, uint64_t i) { return d ? (double)ldn<O32>(d, i) : 0.0; }
 
-template <typename S>
-__device__ __forceinline__ double ld_delta(const S* d, uint64_t i) { return d ? (double)d[i] : 0.0; }
+// State q + d_in of variable v at node n in the residual type (with its
+// tangent for the tangent-linear instantiation).
+template <typename R, typename S, bool O32>
+__device__ __forceinline__ R load_state(const RkArgs<S>& a, int v, uint64_t n) {
+    const double x = ldn<O32>(a.q[v], n) + ld_delta<O32>(a.din[v], n);
+    if constexpr (is_dual<R>::value) {
+        R r(x);
+        r.d[0] = ldn<O32>(a.tq[v], n) + ld_delta<O32>(a.tdin[v], n);
+        return r;
+    } else {
+        return (R)x;
+    }
+}
 
 }  // namespace
-
-// Face-node task t (0 .. NFN-1) of an element: face group g, face-local
-// (a, b) and the element-local node.  Groups 0,1 (k = 0, NZ-1) have NX*NY
-// nodes, 2,3 (j) NX*NZ, 4,5 (i) NY*NZ, enumerated group by group in the
-// (a, b) order of face_ab.
-__device__ __forceinline__ void face_task(int t, int NX, int NY, int NZ, int& g, int& fa, int& fb,
-                                          int& node) {
-    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
-    int i, j, k, l;
-    if (t < 2 * FK) {
-        g = t >= FK; l = t - g * FK;
-        fa = l / NY; fb = l - fa * NY;
-        i = fa; j = fb; k = g ? NZ - 1 : 0;
-    } else if (t < 2 * (FK + FJ)) {
-        l = t - 2 * FK;
-        const int h = l >= FJ;
-        g = 2 + h; l -= h * FJ;
-        fa = l / NZ; fb = l - fa * NZ;
-        i = fa; j = h ? NY - 1 : 0; k = fb;
-    } else {
-        l = t - 2 * (FK + FJ);
-        const int h = l >= FI;
-        g = 4 + h; l -= h * FI;
-        fa = l / NZ; fb = l - fa * NZ;
-        i = h ? NX - 1 : 0; j = fa; k = fb;
-    }
-    node = i * NY * NZ + j * NZ + k;
-}
-
-// Inverse of face_task for a node on face group g.
-__device__ __forceinline__ int face_task_index(int g, int fa, int fb, int NX, int NY, int NZ) {
-    const int FK = NX * NY, FJ = NX * NZ, FI = NY * NZ;
-    switch (g) {
-        case 0: return fa * NY + fb;
-        case 1: return FK + fa * NY + fb;
-        case 2: return 2 * FK + fa * NZ + fb;
-        case 3: return 2 * FK + FJ + fa * NZ + fb;
-        case 4: return 2 * (FK + FJ) + fa * NZ + fb;
-        default: return 2 * (FK + FJ) + FI + fa * NZ + fb;
-    }
-}
 
 // Everything one face-node task reads from global memory, gathered before
 // the volume term so that its latency overlaps the element's own loads:
@@ -149,7 +90,7 @@ struct FaceTask {
     R rp, m[3], tp, rref, pref;
 };
 
-template <typename R, typename S>
+template <typename R, typename S, bool O32>
 __device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S>& a, uint32_t e, int t,
                                                   int NX, int NY, int NZ, FaceTask<R>& F) {
     face_task(t, NX, NY, NZ, F.g, F.fa, F.fb, F.fnode);
@@ -164,15 +105,15 @@ __device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S
     F.nv[2] = (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt);
     if (!((ef.meta >> 5) & 1)) {
         const uint64_t m = face_node(ef.other, og, F.fa, F.fb, NX, NY, NZ);
-        F.rp = (R)(a.q[0][m] + ld_delta(a.din[0], m));
-        F.m[0] = (R)(a.q[1][m] + ld_delta(a.din[1], m));
-        F.m[1] = (R)(a.q[2][m] + ld_delta(a.din[2], m));
-        F.m[2] = (R)(a.q[3][m] + ld_delta(a.din[3], m));
-        F.tp = (R)(a.q[4][m] + ld_delta(a.din[4], m));
-        F.rref = (R)d.rho_ref[m];
-        F.pref = (R)d.p_ref[m];
+        F.rp = load_state<R, S, O32>(a, 0, m);
+        F.m[0] = load_state<R, S, O32>(a, 1, m);
+        F.m[1] = load_state<R, S, O32>(a, 2, m);
+        F.m[2] = load_state<R, S, O32>(a, 3, m);
+        F.tp = load_state<R, S, O32>(a, 4, m);
+        F.rref = (R)ldn<O32>(d.rho_ref, m);
+        F.pref = (R)ldn<O32>(d.p_ref, m);
     } else {
-        F.rref = (R)d.rho_ref[(uint64_t)e * (NX * NY * NZ) + F.fnode];
+        F.rref = (R)ldn<O32>(d.rho_ref, (uint64_t)e * (NX * NY * NZ) + F.fnode);
         F.pref = 0;
     }
 }
@@ -192,7 +133,7 @@ constexpr int kPrimRows = 10;
 //      one face per wavefront): Rusanov flux -> LDS
 //   5. each node subtracts/adds the fluxes of its faces in face-group order,
 //      then the stage update.
-template <typename R, typename S, int NXc, int NYc, int NZc>
+template <typename R, typename S, int NXc, int NYc, int NZc, bool O32>
 __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -221,50 +162,48 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
 
     // global reads of this thread's first face task
     FaceTask<R> ft{};
-    if (active && q < NFN) prepare_face_task<R, S>(d, a, e, q, NX, NY, NZ, ft);
+    if (active && q < NFN) prepare_face_task<R, S, O32>(d, a, e, q, NX, NY, NZ, ft);
 
     Prim<R> P{};
     // the part of the stage output known before the residual:
     // b din (+ q on the last stage); the output is pre + b dt L(q + din)
     double pre[5] = {0, 0, 0, 0, 0};
+    double tpre[5] = {0, 0, 0, 0, 0};  // its tangent (tangent-linear instantiation)
     R rref = 0, pref = 0, cvn = 1;
     if (active) {
         double base[5], din[5];
         for (int v = 0; v < 5; v++) {
-            base[v] = a.q[v][n];
-            din[v] = a.din[v] ? (double)a.din[v][n] : 0.0;
+            base[v] = ldn<O32>(a.q[v], n);
+            din[v] = ld_delta<O32>(a.din[v], n);
             pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
+            if constexpr (is_dual<R>::value) {
+                const double tb = ldn<O32>(a.tq[v], n), td = ld_delta<O32>(a.tdin[v], n);
+                tpre[v] = a.last ? tb + a.b * td : a.b * td;
+            }
         }
-        rref = (R)d.rho_ref[n];
-        pref = (R)d.p_ref[n];
-        if constexpr (sizeof(R) == 4) cvn = d.cV32[n];
-        else cvn = (R)d.cV[n];
-        P.rp = (R)(base[0] + din[0]);
-        P.m[0] = (R)(base[1] + din[1]); P.m[1] = (R)(base[2] + din[2]); P.m[2] = (R)(base[3] + din[3]);
-        P.tp = (R)(base[4] + din[4]);
+        rref = (R)ldn<O32>(d.rho_ref, n);
+        pref = (R)ldn<O32>(d.p_ref, n);
+        if constexpr (sizeof(R) == 4) cvn = ldn<O32>(d.cV32, n);
+        else cvn = (R)ldn<O32>(d.cV, n);
+        P.rp = load_state<R, S, O32>(a, 0, n);
+        P.m[0] = load_state<R, S, O32>(a, 1, n);
+        P.m[1] = load_state<R, S, O32>(a, 2, n);
+        P.m[2] = load_state<R, S, O32>(a, 3, n);
+        P.tp = load_state<R, S, O32>(a, 4, n);
         make_prim(P, rref, pref, T0, gam, gR);
         R J[9];
         if constexpr (sizeof(R) == 4)
-            for (int c = 0; c < 9; c++) J[c] = d.JcV32[(size_t)c * nb + n];
+            for (int c = 0; c < 9; c++) J[c] = ldn<O32>(d.JcV32 + (size_t)c * nb, n);
         else
-            for (int c = 0; c < 9; c++) J[c] = (R)d.JcV[(size_t)c * nb + n];
+            for (int c = 0; c < 9; c++) J[c] = (R)ldn<O32>(d.JcV + (size_t)c * nb, n);
         // rows: rho, m_x, m_y, m_z, Theta
         for (int row = 0; row < 5; row++) {
-            R f0, f1, f2;
-            if (row == 0) { f0 = P.m[0]; f1 = P.m[1]; f2 = P.m[2]; }
-            else if (row < 4) {
-                const int r = row - 1;
-                f0 = P.m[r] * P.u[0] + (r == 0 ? P.pp : R(0));
-                f1 = P.m[r] * P.u[1] + (r == 1 ? P.pp : R(0));
-                f2 = P.m[r] * P.u[2] + (r == 2 ? P.pp : R(0));
-            } else {
-                f0 = P.m[0] * T0 + P.m[0] * P.thp;
-                f1 = P.m[1] * T0 + P.m[1] * P.thp;
-                f2 = P.m[2] * T0 + P.m[2] * P.thp;
-            }
-            G[(row * 3 + 0) * NP + q] = f0 * J[0] + f1 * J[6] + f2 * J[8];
-            G[(row * 3 + 1) * NP + q] = f0 * J[3] + f1 * J[1] + f2 * J[7];
-            G[(row * 3 + 2) * NP + q] = f0 * J[5] + f1 * J[4] + f2 * J[2];
+            R f0, f1, f2, g0, g1, g2;
+            flux_row(P, row, T0, f0, f1, f2);
+            contravariant(J, f0, f1, f2, g0, g1, g2);
+            G[(row * 3 + 0) * NP + q] = g0;
+            G[(row * 3 + 1) * NP + q] = g1;
+            G[(row * 3 + 2) * NP + q] = g2;
         }
     }
     __syncthreads();
@@ -294,7 +233,7 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
 
     if (active) {
         for (int t = q; t < NFN; t += NP) {
-            if (t != q) prepare_face_task<R, S>(d, a, e, t, NX, NY, NZ, ft);
+            if (t != q) prepare_face_task<R, S, O32>(d, a, e, t, NX, NY, NZ, ft);
             const FaceTask<R>& F = ft;
             if (!((F.meta >> 4) & 1)) continue;
             const bool own = (F.meta >> 3) & 1;
@@ -308,50 +247,21 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             A.u[0] = G[5 * NP + fnode]; A.u[1] = G[6 * NP + fnode]; A.u[2] = G[7 * NP + fnode];
             A.thp = G[8 * NP + fnode];
             A.pp = G[9 * NP + fnode];
-            A.c = r_sqrt<R>(gR * (T0 + A.thp));
-            A.umag = r_sqrt<R>(A.u[0] * A.u[0] + A.u[1] * A.u[1] + A.u[2] * A.u[2]);
+            A.c = r_sqrt(gR * (T0 + A.thp));
+            A.umag = r_sqrt(A.u[0] * A.u[0] + A.u[1] * A.u[1] + A.u[2] * A.u[2]);
             Prim<R> Q{};
             if (!ghost) {
                 Q.rp = F.rp; Q.m[0] = F.m[0]; Q.m[1] = F.m[1]; Q.m[2] = F.m[2]; Q.tp = F.tp;
                 make_prim(Q, F.rref, F.pref, T0, gam, gR);
             } else {
-                Q = A;
                 const BcTable& bt = *d.bc;
                 const int pi = F.meta >> 8;
-                const int kind = bt.kind[1][pi];
-                if (kind == BC_SYMMETRY) {
-                    sym_r(A.u, nv, Q.u);
-                    sym_r(A.m, nv, Q.m);
-                } else if (kind == BC_DIRICHLET) {
-                    Q.u[0] = (R)bt.value[1][pi]; Q.u[1] = (R)bt.value[2][pi]; Q.u[2] = (R)bt.value[3][pi];
-                    const R rho = F.rref + A.rp;
-                    Q.m[0] = rho * Q.u[0]; Q.m[1] = rho * Q.u[1]; Q.m[2] = rho * Q.u[2];
-                }
-                Q.umag = r_sqrt<R>(Q.u[0] * Q.u[0] + Q.u[1] * Q.u[1] + Q.u[2] * Q.u[2]);
+                const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
+                ghost_state(A, nv, bt.kind[1][pi], uv, F.rref, Q);
             }
-            const Prim<R>& O = own ? A : Q;
-            const Prim<R>& N = own ? Q : A;
-            const R fI = ghost ? R(0) : R(0.5);
-            const R fJ = R(1) - fI;
-            const R lam = ((O.umag * fI + N.umag * fJ) + (O.c * fI + N.c * fJ)) * R(0.5);
-            const R mn = r_sqrt<R>(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-            const R en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
             R flux[5];
-            // rho: avg(m).n - lambda (rho'_N - rho'_O) |n|
-            flux[0] = (O.m[0] * fI + N.m[0] * fJ) * nv[0] + (O.m[1] * fI + N.m[1] * fJ) * nv[1] +
-                      (O.m[2] * fI + N.m[2] * fJ) * nv[2] - lam * (N.rp - O.rp) * mn;
-            // momentum: avg(m (x) U + p'I).n - e_r lambda (dm . n)
-            const R dmn = lam * ((N.m[0] - O.m[0]) * nv[0] + (N.m[1] - O.m[1]) * nv[1] + (N.m[2] - O.m[2]) * nv[2]);
-            const R un_O = O.u[0] * nv[0] + O.u[1] * nv[1] + O.u[2] * nv[2];
-            const R un_N = N.u[0] * nv[0] + N.u[1] * nv[1] + N.u[2] * nv[2];
-            for (int r = 0; r < 3; r++)
-                flux[1 + r] = (O.m[r] * un_O + O.pp * nv[r]) * fI + (N.m[r] * un_N + N.pp * nv[r]) * fJ - en[r] * dmn;
-            // Theta: avg(m theta).n - lambda (Theta'_N - Theta'_O) |n|
-            {
-                const R mnO = O.m[0] * nv[0] + O.m[1] * nv[1] + O.m[2] * nv[2];
-                const R mnN = N.m[0] * nv[0] + N.m[1] * nv[1] + N.m[2] * nv[2];
-                flux[4] = (mnO * T0 + mnO * O.thp) * fI + (mnN * T0 + mnN * N.thp) * fJ - lam * (N.tp - O.tp) * mn;
-            }
+            if (own) rusanov(A, Q, nv, ghost ? R(0) : R(0.5), T0, flux);
+            else rusanov(Q, A, nv, ghost ? R(0) : R(0.5), T0, flux);
             for (int v = 0; v < 5; v++) FL[v * NFN + t] = flux[v];
         }
     }
@@ -371,7 +281,7 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             for (int v = 0; v < 5; v++) res[v] += FL[v * NFN + t];
     }
     // L(q) = (volume - surface)/cV + buoyancy; increments and base
-    const R icv = R(1) / cvn;
+    const R icv = r_div(R(1), cvn);
     R Lq[5];
     for (int v = 0; v < 5; v++) Lq[v] = res[v] * icv;
     if (d.buoyancy) {
@@ -381,20 +291,25 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     }
     const double dt = d.dt;
     for (int v = 0; v < 5; v++) {
-        const double out = pre[v] + a.b * dt * (double)Lq[v];
-        if (a.last) a.qout[v][n] = out;
-        else a.dout[v][n] = (S)out;
+        const double out = pre[v] + a.b * dt * value_of(Lq[v]);
+        if (a.last) stn<O32>(a.qout[v], n, out);
+        else stn<O32>(a.dout[v], n, (S)out);
+        if constexpr (is_dual<R>::value) {
+            const double tout = tpre[v] + a.b * dt * Lq[v].d[0];
+            if (a.last) stn<O32>(a.tqout[v], n, tout);
+            else stn<O32>(a.tdout[v], n, tout);
+        }
     }
     if (a.check) {
         bool bad = false;
-        for (int v = 0; v < 5; v++) bad |= !isfinite((double)Lq[v]);
+        for (int v = 0; v < 5; v++) bad |= !isfinite(value_of(Lq[v]));
         if (bad) *a.flag = 1;
     }
 }
 
 template <typename R, typename S, int X, int Y, int Z>
 __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z>(d, a);
+    rk_stage_body<R, S, X, Y, Z, false>(d, a);
 }
 
 // N = 7 (512 nodes, one element per work-group): registers capped so that two
@@ -402,7 +317,7 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
 template <typename R, typename S, int W>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
 rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, 8, 8, 8>(d, a);
+    rk_stage_body<R, S, 8, 8, 8, true>(d, a);
 }
 
 template <typename R, typename S, int X, int Y, int Z>
@@ -415,7 +330,10 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st
     blocks = (blocks + 7) / 8 * 8;  // whole XCD chunks
     const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
     if (lds > 65536) return hipErrorInvalidValue;
-    if constexpr (X == 8 && Y == 8 && Z == 8) {
+    // the N = 7 kernels form node offsets in 32 bits
+    const bool off32 = d.nb * sizeof(double) < (1ull << 32);
+    if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
+      if (off32) {
         static const int w = [] {
             const char* s = getenv("NSEM_RK_WAVES");
             return s ? atoi(s) : 0;
@@ -430,8 +348,10 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st
             hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 5>), dim3(blocks), dim3(threads), lds, st, d, a);
         else
             hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 4>), dim3(blocks), dim3(threads), lds, st, d, a);
-    } else
-        hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
     return hipGetLastError();
 }
 
@@ -446,5 +366,6 @@ hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
 
 template hipError_t launch_rk_stage<double, double>(const DgDev&, const RkArgs<double>&, hipStream_t);
 template hipError_t launch_rk_stage<float, float>(const DgDev&, const RkArgs<float>&, hipStream_t);
+template hipError_t launch_rk_stage<Dual<1>, double>(const DgDev&, const RkArgs<double>&, hipStream_t);
 
 }  // namespace nsem

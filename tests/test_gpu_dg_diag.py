@@ -8,7 +8,8 @@ Energy: 1e-12 relative.  Courant max and avg: 1e-8 relative (they follow |U|,
 which inherits the 1-ulp pow() band of the split step); Courant min: within
 1e-8 of the max (the minimum speed is round-off sized).  The energy loss
 after the run is checked against the reference build's printed value to
-1e-4 relative (a difference of two sums, each good to ~1e-16)."""
+1e-4 relative plus 2e-14 absolute (a difference of two sums, each good to
+~1e-15 in a different summation order)."""
 import numpy as np
 import pytest
 
@@ -45,7 +46,9 @@ def test_split_diagnostics_match_oracle(oracle, tmp_path, case):
     check(d1, r1)
     _, loss = _log_numbers(FX["euler"][case["name"]]["log_tail"])
     e_loss = (d0[4] - d1[4]) / d0[4]
-    assert abs(e_loss - float(loss[1])) <= 1e-4 * abs(float(loss[1])), (e_loss, loss[1])
+    # each energy sum is good to ~1e-15 relative (summation order), so a loss
+    # of round-off size is held to that absolute level
+    assert abs(e_loss - float(loss[1])) <= 1e-4 * abs(float(loss[1])) + 2e-14, (e_loss, loss[1])
     dg.close()
 
 
