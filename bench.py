@@ -98,6 +98,19 @@ def setup_dg(ndiv, size, order, dt, seed, tmp):
     return mesh, dg, (rho, U, T, p)
 
 
+def profiled_traffic(key):
+    """HBM bytes per launch of the dominant kernel, from the committed counter
+    profile (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", "rk_stage_64x64x32_n7_hbm_bytes.md")
+    try:
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -164,8 +177,10 @@ def fv_bench(n, tol=1e-10):
     # per-iteration bytes: SpMV+dot (p 8, 3 couplings 24, diag 8, Ap 8) + update (x p r Ap
     # read, x r z write, idiag read: 64) + p update (z p read, p write: 24)
     b_iter = 48 + 64 + 24
-    t_iter = out["fp64"]["seconds"] / max(out["fp64"]["iterations"], 1)
-    out["fp64"]["GBps_per_iter"] = b_iter * n ** 3 / t_iter / 1e9
+    for name, scale in (("fp64", 1.0), ("fp32_refined", 0.5)):
+        t_iter = out[name]["seconds"] / max(out[name]["iterations"], 1)
+        out[name]["GBps_per_iter"] = scale * b_iter * n ** 3 / t_iter / 1e9
+        out[name]["frac_hbm_peak"] = out[name]["GBps_per_iter"] * 1e9 / HBM_PEAK
     out["cells"] = n ** 3
     fv.close()
     return out
@@ -179,7 +194,7 @@ def main():
     ap.add_argument("--ndiv", type=str, default="64,64,32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fv", action="store_true")
-    ap.add_argument("--fv-n", type=int, default=128)
+    ap.add_argument("--fv-n", type=int, default=256)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only time the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -229,7 +244,9 @@ def main():
     b64 = algorithmic_bytes_per_node(8, 8, 8, 8, N)
     achieved = b32 * nb / res[32]["stage_s"]
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
+                "frac": achieved / HBM_PEAK,
+                "traffic": profiled_traffic("traffic_bytes_per_launch_fp32") if ndiv == (64, 64, 32) else None,
+                "traffic_source": "profiles/rk_stage_64x64x32_n7_hbm_bytes.md (mean of the 3 stages)",
                 "kernel": "rk_stage_kernel_n7<float,float,6>",
                 "bytes_per_node": b32, "nodes_per_launch": nb,
                 "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
