@@ -121,6 +121,22 @@ void nsem_dg_destroy(nsem_dg* d);
  * method-of-lines state; getting it returns whichever path ran last. */
 int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p);
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
+/* ---- derivatives of the FP64 method-of-lines step (no reference
+ * counterpart; for gradient checks).  Conserved-variable arrays are 5*nb
+ * doubles: blocks rho', m_x, m_y, m_z, Theta' of nb nodes each.  They act on
+ * the method-of-lines state (set by nsem_dg_set_state, advanced by
+ * nsem_dg_rk3_steps).  Element order: N <= 7 in 3D for the adjoint, N <= 5
+ * for the tangent-linear kernels (their LDS staging).                     */
+/* L(q) and its directional derivative dL = (dL/dq) dq at the current state. */
+int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL);
+/* t = (dL/dq)^T w at the current state. */
+int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t);
+/* Advance nsteps FP64 SSP-RK3 steps and carry the tangent dq (in place) along. */
+int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq);
+/* Advance nsteps FP64 SSP-RK3 steps from q0 and return, in place of
+ * lam = dJ/dq_N, the gradient dJ/dq0 = (dq_N/dq0)^T lam.  States are
+ * checkpointed every `checkpoint` steps and recomputed segment by segment. */
+int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam);
 /* Per-step diagnostics of the current state (split or method-of-lines),
  * reduced on the device over the interior nodes: out[6] = Courant number
  * max, min, avg (Mesh::calc_courant, field.cpp:440-448), total mass
@@ -142,6 +158,9 @@ int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps);
  * FP64 base state and accumulation.  No reference counterpart (the
  * reference's explicit schemes all reduce to forward Euler). */
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision);
+/* Set the method-of-lines state directly (m: xyz per node); it becomes the
+ * current state. */
+int nsem_dg_set_conserved(nsem_dg* d, const double* rho_p, const double* m, const double* theta_p);
 /* Conserved perturbations of the method-of-lines state (interior nodes). */
 int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p);
 /* Same as the step functions, returning the device time of the steps. */

@@ -239,6 +239,36 @@ class DGEuler:
         check(lib().nsem_dg_get_conserved(self._h, _pd(rp), _pd(m), _pd(tp)))
         return rp, m, tp
 
+    # ---- derivatives of the FP64 method-of-lines step (conserved arrays of
+    # shape (5, nb): rho', m_x, m_y, m_z, Theta')
+    def residual_tlm(self, dq):
+        dq = np.ascontiguousarray(dq, dtype=np.float64).reshape(5, -1)
+        L, dL = np.empty_like(dq), np.empty_like(dq)
+        check(lib().nsem_dg_residual_tlm(self._h, _pd(dq), _pd(L), _pd(dL)))
+        return L, dL
+
+    def residual_vjp(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(5, -1)
+        t = np.empty_like(w)
+        check(lib().nsem_dg_residual_vjp(self._h, _pd(w), _pd(t)))
+        return t
+
+    def rk3_tlm(self, n, dq):
+        dq = np.array(dq, dtype=np.float64).reshape(5, -1)
+        check(lib().nsem_dg_rk3_tlm(self._h, int(n), _pd(dq)))
+        return dq
+
+    def rk3_adjoint(self, n, lam, checkpoint=10):
+        lam = np.array(lam, dtype=np.float64).reshape(5, -1)
+        check(lib().nsem_dg_rk3_adjoint(self._h, int(n), int(checkpoint), _pd(lam)))
+        return lam
+
+    def set_conserved(self, rho_p, m, theta_p):
+        """Set the method-of-lines state from conserved perturbations."""
+        check(lib().nsem_dg_set_conserved(self._h, _pd(np.ascontiguousarray(rho_p, dtype=np.float64)),
+                                          _pd(np.ascontiguousarray(m, dtype=np.float64)),
+                                          _pd(np.ascontiguousarray(theta_p, dtype=np.float64))))
+
     def diagnostics(self):
         """(Courant max, min, avg, mass, energy, volume) of the current state."""
         out = np.empty(6)

@@ -18,54 +18,7 @@
 
 using namespace nsem;
 
-#define HIP_TRY(x)                                                                  \
-    do {                                                                            \
-        hipError_t e_ = (x);                                                        \
-        if (e_ != hipSuccess)                                                       \
-            return nsem_fail(NSEM_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t n) { bytes = n; return hipMalloc(&p, n ? n : 8); }
-    template <typename T> T* as() const { return (T*)p; }
-};
-
-}  // namespace
-
-struct nsem_dg {
-    DgDev dev{};
-    nsem_euler_params prm{};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t nb = 0;
-    // geometry
-    DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
-    // double-buffered state: rho ux uy uz T p
-    DevBuf st[2][6];
-    int cur = 0;
-    // method-of-lines state: FP64 conserved perturbations (rho', m, Theta'),
-    // double-buffered, and stage increments in FP64 and FP32
-    DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
-    // diagnostics: g.x per node, block partials, result
-    DevBuf gh, diag_slots, diag_out;
-    int mcur = 0;
-    bool mol_current = false;
-    // profiling of the dominant kernels
-    bool profile = false;
-    double prof_ms[4] = {0, 0, 0, 0};
-    uint64_t prof_n[4] = {0, 0, 0, 0};
-    std::vector<double> host_p_ref, host_rho_ref;
-    ~nsem_dg() {
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+#include "dg_context.hpp"
 
 namespace {
 
@@ -281,6 +234,19 @@ int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const doub
     return NSEM_OK;
 }
 
+int nsem_dg_set_conserved(nsem_dg* d, const double* rho_p, const double* m, const double* theta_p) {
+    if (!d || !rho_p || !m || !theta_p) return nsem_fail(NSEM_EINVAL, "null argument");
+    const uint64_t nb = d->nb;
+    std::vector<double> mx(nb), my(nb), mz(nb);
+    for (uint64_t n = 0; n < nb; n++) { mx[n] = m[3 * n]; my[n] = m[3 * n + 1]; mz[n] = m[3 * n + 2]; }
+    const double* src[5] = {rho_p, mx.data(), my.data(), mz.data(), theta_p};
+    for (int v = 0; v < 5; v++)
+        HIP_TRY(hipMemcpyAsync(d->mq[d->mcur][v].p, src[v], nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->mol_current = true;
+    return NSEM_OK;
+}
+
 int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p) {
     if (!d || !rho_p || !m || !theta_p) return nsem_fail(NSEM_EINVAL, "null argument");
     const uint64_t nb = d->nb;
@@ -397,6 +363,7 @@ int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
                 a.qout[v] = qb[v].as<double>();
             }
             a.b = kB[stage];
+            a.bdt = kB[stage] * d->dev.dt;
             a.last = stage == 2;
             a.check = (stage == 2);
             a.flag = d->flag.as<int>();

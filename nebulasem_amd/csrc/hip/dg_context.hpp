@@ -1,0 +1,64 @@
+// The device-side state of a DG Euler problem (the nsem_dg handle), shared
+// by the files that implement its entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../status.hpp"
+#include "dg_common.hpp"
+#include "dg_kernels.hpp"
+#include "nebulasem_amd.h"
+
+#define HIP_TRY(x)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            return nsem_fail(NSEM_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace nsem {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t n) { bytes = n; return hipMalloc(&p, n ? n : 8); }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+}  // namespace nsem
+
+struct nsem_dg {
+    using DevBuf = nsem::DevBuf;
+    using DgDev = nsem::DgDev;
+    DgDev dev{};
+    nsem_euler_params prm{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t nb = 0;
+    // geometry
+    DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
+    // double-buffered state: rho ux uy uz T p
+    DevBuf st[2][6];
+    int cur = 0;
+    // method-of-lines state: FP64 conserved perturbations (rho', m, Theta'),
+    // double-buffered, and stage increments in FP64 and FP32
+    DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
+    // diagnostics: g.x per node, block partials, result
+    DevBuf gh, diag_slots, diag_out;
+    int mcur = 0;
+    bool mol_current = false;
+    // profiling of the dominant kernels
+    bool profile = false;
+    double prof_ms[4] = {0, 0, 0, 0};
+    uint64_t prof_n[4] = {0, 0, 0, 0};
+    std::vector<double> host_p_ref, host_rho_ref;
+    ~nsem_dg() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+

@@ -33,6 +33,8 @@ struct RkArgs {
     S* dout[5];
     double* qout[5];
     double b;
+    double bdt;   // b * dt (the factor of L in the output)
+    int epb;      // elements per work-group (set by the launcher)
     int last;
     int check;
     int* flag;
@@ -41,6 +43,31 @@ struct RkArgs {
     double* tdout[5];
     double* tqout[5];
 };
+// Vector-Jacobian product of the method-of-lines residual at Q = q + din:
+// t = (dL/dQ)^T w, all FP64 SoA node arrays (din may be null).
+struct VjpArgs {
+    int epb;      // elements per work-group (set by the launcher)
+    const double* q[5];
+    const double* din[5];
+    const double* w[5];
+    double* t[5];
+};
+hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st);
+size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb);
+
+// The discrete adjoint's stage algebra on node arrays:
+//   ad_out = c1 (ad_in + dt t)   (skipped when ad_out is null)
+//   acc_out = acc_in + c2 dt t
+struct AdjCombineArgs {
+    const double* t[5];
+    const double* ad_in[5];
+    double* ad_out[5];
+    const double* acc_in[5];
+    double* acc_out[5];
+    double dt, c1, c2;
+};
+hipError_t launch_adj_combine(uint64_t nb, const AdjCombineArgs& a, hipStream_t st);
+
 // LDS of the RK stage kernel, in values of the residual type: per element
 // max(15 flux rows, 10 primitive rows + 5 face-flux rows) x NP, plus the three
 // derivative matrices once per work-group.

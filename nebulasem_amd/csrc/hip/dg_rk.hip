@@ -138,7 +138,7 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int NFN = 2 * (NX * NY + NX * NZ + NY * NZ);
-    const int epb = blockDim.x / NP;
+    const int epb = a.epb;
     const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
     // XCD-contiguous element mapping: work-group b runs on XCD b % 8
     const uint32_t per_xcd = (gridDim.x + 7) / 8;
@@ -289,13 +289,12 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
         Lq[2] += P.rp * (R)d.g[1];
         Lq[3] += P.rp * (R)d.g[2];
     }
-    const double dt = d.dt;
     for (int v = 0; v < 5; v++) {
-        const double out = pre[v] + a.b * dt * value_of(Lq[v]);
+        const double out = pre[v] + a.bdt * value_of(Lq[v]);
         if (a.last) stn<O32>(a.qout[v], n, out);
         else stn<O32>(a.dout[v], n, (S)out);
         if constexpr (is_dual<R>::value) {
-            const double tout = tpre[v] + a.b * dt * Lq[v].d[0];
+            const double tout = tpre[v] + a.bdt * Lq[v].d[0];
             if (a.last) stn<O32>(a.tqout[v], n, tout);
             else stn<O32>(a.tdout[v], n, tout);
         }
@@ -321,15 +320,19 @@ rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
 }
 
 template <typename R, typename S, int X, int Y, int Z>
-static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
+static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t st) {
+    RkArgs<S> a = a_in;
     const int NP = d.np;
     if (NP > 512) return hipErrorInvalidValue;
+    // elements per work-group: fill 256 threads, as far as the LDS allows
     int epb = NP >= 256 ? 1 : 256 / NP;
+    while (epb > 1 && rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R)) > 65536) epb--;
     int threads = (epb * NP + 63) / 64 * 64;
     unsigned blocks = (d.nelem + epb - 1) / epb;
     blocks = (blocks + 7) / 8 * 8;  // whole XCD chunks
     const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
     if (lds > 65536) return hipErrorInvalidValue;
+    a.epb = epb;
     // the N = 7 kernels form node offsets in 32 bits
     const bool off32 = d.nb * sizeof(double) < (1ull << 32);
     if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
