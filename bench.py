@@ -41,11 +41,22 @@ def algorithmic_bytes_per_node(s_in, s_base, s_out, s_geo, N):
     return NQ * (s_in + s_base + s_out) + 10 * s_geo + 2 * 8 + 3.0 / (N + 1) * (3 * s_geo + 8)
 
 
+# Box corner points and the block's hex vertex order over them (x, then y,
+# then z fastest for ndiv = (nx, ny, nz)), as a block description file writes
+# them: points list, then `8{0 1 5 4 3 2 6 7}`.
+BLOCK_ORDER = (0, 1, 5, 4, 3, 2, 6, 7)
+
+
+def box_points(lx, ly, lz):
+    return [(0, 0, 0), (lx, 0, 0), (lx, 0, lz), (0, 0, lz), (0, ly, 0), (lx, ly, 0), (lx, ly, lz), (0, ly, lz)]
+
+
 def box_patches(lx, ly, lz):
-    c = [(0, 0, 0), (lx, 0, 0), (lx, 0, lz), (0, 0, lz), (0, ly, 0), (lx, ly, 0), (lx, ly, lz), (0, ly, lz)]
+    """Block corners in hex vertex order and the named wall patches."""
+    c = box_points(lx, ly, lz)
     planes = [("top", (3, 2, 6, 7)), ("bottom", (0, 1, 5, 4)), ("front", (0, 1, 2, 3)),
               ("back", (4, 5, 6, 7)), ("sides", (1, 5, 6, 2)), ("sides", (0, 4, 7, 3))]
-    return c, [(n, [c[i] for i in g]) for n, g in planes]
+    return [c[i] for i in BLOCK_ORDER], [(n, [c[i] for i in g]) for n, g in planes]
 
 
 def bubbles(seed, lx, ly, lz, k=4):
@@ -96,6 +107,113 @@ def setup_dg(ndiv, size, order, dt, seed, tmp):
     del f
     log(f"fields + device set-up {time.time() - t:.1f}s")
     return mesh, dg, (rho, U, T, p)
+
+
+def adjoint_bench(nsteps=50, checkpoint=10, dt=0.001):
+    """Discrete adjoint of the 2D bubble: 64x64 elements (1 thick in y) over
+    1000 x 1000 m, N = 5, Gaussian dtheta = 0.5 K (radius 250 m, centre
+    x = 500, z = 350) plus seeded 1e-3 K noise; J = 1/2 |theta'(T) - theta_obs|^2
+    after nsteps FP64 SSP-RK3 steps, theta_obs from a forward run with other
+    noise; gradient with respect to the initial theta' (at constant pressure)
+    with checkpoints every `checkpoint` steps.  Reports the adjoint's cost
+    relative to the forward run, the tangent-linear / adjoint dot-product
+    identity and a central-difference check of the gradient."""
+    from nebulasem_amd import DGEuler, Mesh, euler_params, read_field
+    lx, ly, lz = 1000.0, 100.0, 1000.0
+    c = box_points(lx, ly, lz)
+    patches = [("top", [c[i] for i in (3, 2, 6, 7)]), ("bottom", [c[i] for i in (0, 1, 5, 4)]),
+               ("delete", [c[i] for i in (0, 1, 2, 3)]), ("delete", [c[i] for i in (4, 5, 6, 7)]),
+               ("sides", [c[i] for i in (1, 5, 6, 2)]), ("sides", [c[i] for i in (0, 4, 7, 3)])]
+    mesh = Mesh.box([c[i] for i in BLOCK_ORDER], (64, 1, 64), patches=patches, order=(5, 0, 5))
+    names = [n for n, _ in patches if n != "delete"]
+    g = (0.0, 0.0, -9.80606)
+    with tempfile.TemporaryDirectory() as tmp:
+        field_file(os.path.join(tmp, "T0.txt"), 1, ["gaussian 0 0.5 500 50 350 250 1000 250"], names, "NEUMANN")
+        field_file(os.path.join(tmp, "U0.txt"), 3, ["uniform 0 0 0"], names, "SYMMETRY")
+        field_file(os.path.join(tmp, "p0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        field_file(os.path.join(tmp, "rho0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        f = {k: read_field(mesh, os.path.join(tmp, k + "0.txt"), nc, g)
+             for k, nc in (("T", 1), ("U", 3), ("p", 1), ("rho", 1))}
+    nb = mesh.info.nbcs_field
+    dg = DGEuler.from_fields(mesh, euler_params(dt), f)
+    pr, rr = dg.background()
+    P0, T0, cp, cv = 101325.0, 300.0, 1004.67, 715.5
+    R, gam = cp - cv, cp / cv
+    rng = np.random.default_rng(20261203)
+    bubble = np.ascontiguousarray(f["T"][0][:nb, 0])
+    th0 = bubble + 1e-3 * rng.standard_normal(nb)
+    th_obs0 = bubble + 1e-3 * rng.standard_normal(nb)
+
+    def q_of(th):                       # constant-pressure initial state
+        r = (P0 / (R * (th + T0))) * (pr / P0) ** (1 / gam)
+        z = np.zeros(nb)
+        return np.stack([r - rr, z, z, z, r * (th + T0) - rr * T0])
+
+    def dq_dth(th):                     # derivative of q_of, per node
+        r = (P0 / (R * (th + T0))) * (pr / P0) ** (1 / gam)
+        dr = -r / (th + T0)
+        return dr, dr * (th + T0) + r
+
+    def set_q(q):
+        dg.set_conserved(q[0], np.ascontiguousarray(q[1:4].T), q[4])
+
+    def run(q):
+        set_q(q)
+        dg.rk3_steps(nsteps, 64)
+        rp, m, tp = dg.get_conserved()
+        m = m.reshape(-1, 3)
+        return np.stack([rp, m[:, 0], m[:, 1], m[:, 2], tp])
+
+    def theta(q):
+        return (q[4] + rr * T0) / (rr + q[0]) - T0
+
+    obs = theta(run(q_of(th_obs0)))
+
+    def J(th):
+        return 0.5 * float(np.sum((theta(run(q_of(th))) - obs) ** 2))
+
+    q0 = q_of(th0)
+    set_q(q0)
+    dg.synchronize()
+    t = time.time()
+    qn = run(q0)
+    t_fwd = time.time() - t
+    r = theta(qn) - obs
+    rho = rr + qn[0]
+    lam = np.zeros_like(qn)
+    lam[4] = r / rho
+    lam[0] = -r * (qn[4] + rr * T0) / rho ** 2
+    set_q(q0)
+    dg.synchronize()
+    t = time.time()
+    gq = dg.rk3_adjoint(nsteps, lam, checkpoint)
+    t_adj = time.time() - t
+    dr, dtp = dq_dth(th0)
+    grad = gq[0] * dr + gq[4] * dtp
+    v = rng.standard_normal(nb)
+    h = 1e-4
+    fd = (J(th0 + h * v) - J(th0 - h * v)) / (2 * h)
+    ad = float(np.dot(grad, v))
+    # dot-product identity over the whole trajectory
+    scale = np.array([max(np.abs(q0[k]).max(), 1e-3) for k in range(5)])[:, None]
+    vq = rng.standard_normal(q0.shape) * scale
+    wq = rng.standard_normal(q0.shape)
+    set_q(q0)
+    Mv = dg.rk3_tlm(nsteps, vq)
+    set_q(q0)
+    MTw = dg.rk3_adjoint(nsteps, wq, checkpoint)
+    lhs, rhs = float(np.vdot(Mv, wq)), float(np.vdot(vq, MTw))
+    dg.close()
+    out = {"workload": f"2D bubble 64x1x64 N=5 ({nb} nodes), {nsteps} FP64 SSP-RK3 steps, dt={dt}, "
+                       f"checkpoint every {checkpoint}",
+           "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+           "J": 0.5 * float(np.sum(r ** 2)), "grad_fd": fd, "grad_adjoint": ad,
+           "grad_rel_err": abs(fd - ad) / abs(ad),
+           "dot_tlm": lhs, "dot_adjoint": rhs, "dot_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs))}
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms "
+        f"({out['adjoint_over_forward']:.2f}x), FD rel err {out['grad_rel_err']:.2e}, "
+        f"dot-product rel err {out['dot_rel_err']:.2e}")
+    return out
 
 
 def profiled_traffic(key):
@@ -156,11 +274,12 @@ def cpu_baseline(seconds_budget=20.0):
 
 def fv_bench(n, tol=1e-10):
     from nebulasem_amd import BC_DIRICHLET, BC_NEUMANN, FVPoisson, Mesh
-    corners = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]
+    corners = box_points(1.0, 1.0, 1.0)
     planes = [("xmin", (0, 4, 7, 3)), ("xmax", (1, 5, 6, 2)), ("ymin", (0, 1, 2, 3)),
               ("ymax", (4, 5, 6, 7)), ("zmin", (0, 1, 5, 4)), ("zmax", (3, 2, 6, 7))]
     t = time.time()
-    mesh = Mesh.box(corners, (n, n, n), patches=[(nm, [corners[i] for i in g]) for nm, g in planes])
+    mesh = Mesh.box([corners[i] for i in BLOCK_ORDER], (n, n, n),
+                    patches=[(nm, [corners[i] for i in g]) for nm, g in planes])
     kinds = np.array([BC_DIRICHLET if b == "xmin" else BC_NEUMANN for b in mesh.boundaries], dtype=np.int32)
     fv = FVPoisson(mesh, kinds, np.zeros(len(kinds)))
     rng = np.random.default_rng(20261202)
@@ -194,6 +313,7 @@ def main():
     ap.add_argument("--ndiv", type=str, default="64,64,32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fv", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fv-n", type=int, default=256)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only time the plain copy kernel over MIB MiB (counter calibration)")
@@ -253,6 +373,7 @@ def main():
                 "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK,
                 "stream_copy_GBps": stream_bw / 1e9,
                 "frac_of_stream_copy": achieved / stream_bw}
+    adj = None if args.no_adjoint else adjoint_bench()
     fv = None
     if not args.no_fv:
         fv = fv_bench(args.fv_n)
@@ -278,6 +399,7 @@ def main():
         "split_reference_step_fp64": {"value": split_rate, "ms_per_step": ts / ks * 1e3},
         "roofline": roofline,
         "pcg": fv,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
