@@ -69,11 +69,13 @@ struct AdjCombineArgs {
 hipError_t launch_adj_combine(uint64_t nb, const AdjCombineArgs& a, hipStream_t st);
 
 // LDS of the RK stage kernel, in values of the residual type: per element
-// max(15 flux rows, 10 primitive rows + 5 face-flux rows) x NP, plus the three
-// derivative matrices once per work-group.
+// max(15 flux rows x NP, 10 padded primitive rows + 5 face-flux rows x NFN),
+// plus the three derivative matrices once per work-group.
+__host__ __device__ inline int rk_prim_stride(int np) { return np + (np + 7) / 8; }
+__host__ __device__ inline int rk_prim_slot(int node) { return node + (node >> 3); }
 __host__ __device__ inline int rk_lds_per_element(int nx, int ny, int nz) {
     const int np = nx * ny * nz, nfn = 2 * (nx * ny + nx * nz + ny * nz);
-    const int a = 15 * np, b = 10 * np + 5 * nfn;
+    const int a = 15 * np, b = 10 * rk_prim_stride(np) + 5 * nfn;
     return a > b ? a : b;
 }
 inline size_t rk_lds_bytes(int nx, int ny, int nz, int epb, size_t value_size) {

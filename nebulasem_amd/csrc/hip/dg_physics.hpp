@@ -80,7 +80,9 @@ __device__ __forceinline__ float r_expm1(float x) { return expm1f(x); }
 __device__ __forceinline__ double r_log1p(double x) { return log1p(x); }
 __device__ __forceinline__ float r_log1p(float x) { return log1pf(x); }
 __device__ __forceinline__ double r_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float r_sqrt(float x) { return sqrtf(x); }
+// FP32: the hardware square root (1 ulp) rather than the correctly rounded
+// sequence; the FP32 form's error budget is set by its FP32 state, not by it.
+__device__ __forceinline__ float r_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <int K>
 __device__ __forceinline__ Dual<K> r_expm1(const Dual<K>& x) {

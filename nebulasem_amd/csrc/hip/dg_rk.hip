@@ -31,8 +31,6 @@
 // -ffp-contract=off).
 #pragma clang fp contract(fast)
 
-#include <cstdlib>
-
 #include "dg_kernels.hpp"
 #include "dg_physics.hpp"
 
@@ -44,32 +42,44 @@ namespace {
 // arrays below 4 GiB each), so the load takes the array base from scalar
 // registers and one shared 32-bit offset instead of a 64-bit address per
 // array.
+// (The empty asm keeps the 32-bit offset opaque, so it is not re-widened to
+// a 64-bit address computation.)
+__device__ __forceinline__ uint32_t byte_off(uint64_t i, uint32_t size) {
+    uint32_t off = (uint32_t)i * size;
+    __asm__("" : "+v"(off));
+    return off;
+}
 template <bool O32, typename T>
 __device__ __forceinline__ T ldn(const T* base, uint64_t i) {
     if constexpr (O32)
-        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T)));
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off(i, sizeof(T)));
     else
         return base[i];
 }
 template <bool O32, typename T>
 __device__ __forceinline__ void stn(T* base, uint64_t i, T v) {
     if constexpr (O32)
-        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint32_t)((uint32_t)i * (uint32_t)sizeof(T))) = v;
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off(i, sizeof(T))) = v;
     else
         base[i] = v;
 }
 
-template <bool O32, typename S>
-__device__ __forceinline__ double ld_delta(const S* d, uint64_t i) { return d ? (double)ldn<O32>(d, i) : 0.0; }
+// Increment of variable v at node n (DIN: the stage has one; a compile-time
+// choice, so the loads carry no null-pointer branches between them).
+template <bool O32, bool DIN, typename S>
+__device__ __forceinline__ double ld_delta(const S* d, uint64_t i) {
+    if constexpr (DIN) return (double)ldn<O32>(d, i);
+    else return 0.0;
+}
 
 // State q + d_in of variable v at node n in the residual type (with its
 // tangent for the tangent-linear instantiation).
-template <typename R, typename S, bool O32>
+template <typename R, typename S, bool O32, bool DIN>
 __device__ __forceinline__ R load_state(const RkArgs<S>& a, int v, uint64_t n) {
-    const double x = ldn<O32>(a.q[v], n) + ld_delta<O32>(a.din[v], n);
+    const double x = ldn<O32>(a.q[v], n) + ld_delta<O32, DIN>(a.din[v], n);
     if constexpr (is_dual<R>::value) {
         R r(x);
-        r.d[0] = ldn<O32>(a.tq[v], n) + ld_delta<O32>(a.tdin[v], n);
+        r.d[0] = ldn<O32>(a.tq[v], n) + ld_delta<O32, DIN>(a.tdin[v], n);
         return r;
     } else {
         return (R)x;
@@ -80,8 +90,9 @@ __device__ __forceinline__ R load_state(const RkArgs<S>& a, int v, uint64_t n) {
 
 // Everything one face-node task reads from global memory, gathered before
 // the volume term so that its latency overlaps the element's own loads:
-// the face entry, the weighted facet normal and the neighbour's state (or,
-// on a boundary face, the owner node's background density).
+// the face entry, the weighted facet normal and the neighbour's state (on a
+// boundary or absent face the owner's own face node is read instead, so the
+// loads need no branch; only its background density is used there).
 template <typename R>
 struct FaceTask {
     uint32_t meta;
@@ -90,37 +101,43 @@ struct FaceTask {
     R rp, m[3], tp, rref, pref;
 };
 
-template <typename R, typename S, bool O32>
+template <typename R, typename S, bool O32, bool DIN>
 __device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S>& a, uint32_t e, int t,
                                                   int NX, int NY, int NZ, FaceTask<R>& F) {
     face_task(t, NX, NY, NZ, F.g, F.fa, F.fb, F.fnode);
-    const EFace ef = d.eface[(size_t)e * 6 + F.g];
+    // one 12-byte load of the face entry (facet, other, meta)
+    const uint3 efv = *reinterpret_cast<const uint3*>(d.eface + (size_t)e * 6 + F.g);
+    const EFace ef{efv.x, efv.y, efv.z};
     F.meta = ef.meta;
-    if (!((ef.meta >> 4) & 1)) return;
+    const bool present = (ef.meta >> 4) & 1;
+    const bool interior = present && !((ef.meta >> 5) & 1);
     const bool own = (ef.meta >> 3) & 1;
     const int og = ef.meta & 7;
-    const double wgt = face_weight(d, own ? F.g : og, F.fa, F.fb);
-    F.nv[0] = (R)(d.gFN[3 * (size_t)ef.facet] * wgt);
-    F.nv[1] = (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt);
-    F.nv[2] = (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt);
-    if (!((ef.meta >> 5) & 1)) {
-        const uint64_t m = face_node(ef.other, og, F.fa, F.fb, NX, NY, NZ);
-        F.rp = load_state<R, S, O32>(a, 0, m);
-        F.m[0] = load_state<R, S, O32>(a, 1, m);
-        F.m[1] = load_state<R, S, O32>(a, 2, m);
-        F.m[2] = load_state<R, S, O32>(a, 3, m);
-        F.tp = load_state<R, S, O32>(a, 4, m);
-        F.rref = (R)ldn<O32>(d.rho_ref, m);
-        F.pref = (R)ldn<O32>(d.p_ref, m);
-    } else {
-        F.rref = (R)ldn<O32>(d.rho_ref, (uint64_t)e * (NX * NY * NZ) + F.fnode);
-        F.pref = 0;
-    }
+    // facet-node weight w_a w_b / 4 in the owner's face layout (face_weight)
+    const int wg = own ? F.g : og;
+    const double* wa = wg < 4 ? d.w[0] : d.w[1];
+    const double* wb = wg < 2 ? d.w[1] : d.w[2];
+    const double wgt = wa[F.fa] * wb[F.fb] / 4;
+    const size_t f3 = 3 * (size_t)(present ? ef.facet : 0);
+    F.nv[0] = (R)(d.gFN[f3] * wgt);
+    F.nv[1] = (R)(d.gFN[f3 + 1] * wgt);
+    F.nv[2] = (R)(d.gFN[f3 + 2] * wgt);
+    const uint64_t m = interior ? face_node(ef.other, og, F.fa, F.fb, NX, NY, NZ)
+                                : (uint64_t)e * (NX * NY * NZ) + F.fnode;
+    F.rp = load_state<R, S, O32, DIN>(a, 0, m);
+    F.m[0] = load_state<R, S, O32, DIN>(a, 1, m);
+    F.m[1] = load_state<R, S, O32, DIN>(a, 2, m);
+    F.m[2] = load_state<R, S, O32, DIN>(a, 3, m);
+    F.tp = load_state<R, S, O32, DIN>(a, 4, m);
+    F.rref = (R)ldn<O32>(d.rho_ref, m);
+    F.pref = (R)ldn<O32>(d.p_ref, m);
 }
 
 // Primitive rows kept in LDS for the face tasks (the owner side of each face
 // node): rho', m (3), Theta', U (3), theta - T0, p'.  c and |U| are
-// recomputed from them with the same operations as make_prim.
+// recomputed from them with the same operations as make_prim.  The rows are
+// padded by one slot per 8 nodes: a face of constant k reads every 8th node,
+// which would otherwise hit the same few LDS banks.
 constexpr int kPrimRows = 10;
 
 // Work-group of one element (NP threads, one per node; several elements per
@@ -133,7 +150,7 @@ constexpr int kPrimRows = 10;
 //      one face per wavefront): Rusanov flux -> LDS
 //   5. each node subtracts/adds the fluxes of its faces in face-group order,
 //      then the stage update.
-template <typename R, typename S, int NXc, int NYc, int NZc, bool O32>
+template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN>
 __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -148,7 +165,8 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     extern __shared__ unsigned char lds_raw[];
     const int per_elem = rk_lds_per_element(NX, NY, NZ);
     R* G = reinterpret_cast<R*>(lds_raw) + (size_t)le * per_elem;
-    R* FL = G + kPrimRows * NP;                      // face fluxes [5][NFN]
+    const int NPP = rk_prim_stride(NP);
+    R* FL = G + kPrimRows * NPP;                     // face fluxes [5][NFN]
     R* Dl = reinterpret_cast<R*>(lds_raw) + (size_t)epb * per_elem;  // D0, D1, D2
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
@@ -160,9 +178,6 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
         Dl[t] = (R)(t < s1 ? d.D[0][t] : t < s2 ? d.D[1][t - s1] : d.D[2][t - s2]);
     }
 
-    // global reads of this thread's first face task
-    FaceTask<R> ft{};
-    if (active && q < NFN) prepare_face_task<R, S, O32>(d, a, e, q, NX, NY, NZ, ft);
 
     Prim<R> P{};
     // the part of the stage output known before the residual:
@@ -170,32 +185,44 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     double pre[5] = {0, 0, 0, 0, 0};
     double tpre[5] = {0, 0, 0, 0, 0};  // its tangent (tangent-linear instantiation)
     R rref = 0, pref = 0, cvn = 1;
+    FaceTask<R> ft{};   // global reads of this thread's first face task
     if (active) {
+        // all of this phase's global reads first (own node, then the face
+        // task), straight-line, so they are in flight together
         double base[5], din[5];
         for (int v = 0; v < 5; v++) {
             base[v] = ldn<O32>(a.q[v], n);
-            din[v] = ld_delta<O32>(a.din[v], n);
-            pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
-            if constexpr (is_dual<R>::value) {
-                const double tb = ldn<O32>(a.tq[v], n), td = ld_delta<O32>(a.tdin[v], n);
-                tpre[v] = a.last ? tb + a.b * td : a.b * td;
-            }
+            din[v] = ld_delta<O32, DIN>(a.din[v], n);
         }
         rref = (R)ldn<O32>(d.rho_ref, n);
         pref = (R)ldn<O32>(d.p_ref, n);
         if constexpr (sizeof(R) == 4) cvn = ldn<O32>(d.cV32, n);
         else cvn = (R)ldn<O32>(d.cV, n);
-        P.rp = load_state<R, S, O32>(a, 0, n);
-        P.m[0] = load_state<R, S, O32>(a, 1, n);
-        P.m[1] = load_state<R, S, O32>(a, 2, n);
-        P.m[2] = load_state<R, S, O32>(a, 3, n);
-        P.tp = load_state<R, S, O32>(a, 4, n);
-        make_prim(P, rref, pref, T0, gam, gR);
         R J[9];
         if constexpr (sizeof(R) == 4)
             for (int c = 0; c < 9; c++) J[c] = ldn<O32>(d.JcV32 + (size_t)c * nb, n);
         else
             for (int c = 0; c < 9; c++) J[c] = (R)ldn<O32>(d.JcV + (size_t)c * nb, n);
+        if (q < NFN) prepare_face_task<R, S, O32, DIN>(d, a, e, q, NX, NY, NZ, ft);
+        for (int v = 0; v < 5; v++) {
+            pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
+            if constexpr (is_dual<R>::value) {
+                const double tb = ldn<O32>(a.tq[v], n), td = ld_delta<O32, DIN>(a.tdin[v], n);
+                tpre[v] = a.last ? tb + a.b * td : a.b * td;
+            }
+        }
+        if constexpr (is_dual<R>::value) {
+            P.rp = load_state<R, S, O32, DIN>(a, 0, n);
+            P.m[0] = load_state<R, S, O32, DIN>(a, 1, n);
+            P.m[1] = load_state<R, S, O32, DIN>(a, 2, n);
+            P.m[2] = load_state<R, S, O32, DIN>(a, 3, n);
+            P.tp = load_state<R, S, O32, DIN>(a, 4, n);
+        } else {
+            P.rp = (R)(base[0] + din[0]);
+            P.m[0] = (R)(base[1] + din[1]); P.m[1] = (R)(base[2] + din[2]); P.m[2] = (R)(base[3] + din[3]);
+            P.tp = (R)(base[4] + din[4]);
+        }
+        make_prim(P, rref, pref, T0, gam, gR);
         // rows: rho, m_x, m_y, m_z, Theta
         for (int row = 0; row < 5; row++) {
             R f0, f1, f2, g0, g1, g2;
@@ -227,13 +254,13 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     __syncthreads();
     if (active) {
         const R pr[kPrimRows] = {P.rp, P.m[0], P.m[1], P.m[2], P.tp, P.u[0], P.u[1], P.u[2], P.thp, P.pp};
-        for (int r = 0; r < kPrimRows; r++) G[r * NP + q] = pr[r];
+        for (int r = 0; r < kPrimRows; r++) G[r * NPP + rk_prim_slot(q)] = pr[r];
     }
     __syncthreads();
 
     if (active) {
         for (int t = q; t < NFN; t += NP) {
-            if (t != q) prepare_face_task<R, S, O32>(d, a, e, t, NX, NY, NZ, ft);
+            if (t != q) prepare_face_task<R, S, O32, DIN>(d, a, e, t, NX, NY, NZ, ft);
             const FaceTask<R>& F = ft;
             if (!((F.meta >> 4) & 1)) continue;
             const bool own = (F.meta >> 3) & 1;
@@ -241,12 +268,13 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             const int fnode = F.fnode;
             const R* nv = F.nv;
             Prim<R> A{};
-            A.rp = G[0 * NP + fnode];
-            A.m[0] = G[1 * NP + fnode]; A.m[1] = G[2 * NP + fnode]; A.m[2] = G[3 * NP + fnode];
-            A.tp = G[4 * NP + fnode];
-            A.u[0] = G[5 * NP + fnode]; A.u[1] = G[6 * NP + fnode]; A.u[2] = G[7 * NP + fnode];
-            A.thp = G[8 * NP + fnode];
-            A.pp = G[9 * NP + fnode];
+            const R* Pf = G + rk_prim_slot(fnode);
+            A.rp = Pf[0 * NPP];
+            A.m[0] = Pf[1 * NPP]; A.m[1] = Pf[2 * NPP]; A.m[2] = Pf[3 * NPP];
+            A.tp = Pf[4 * NPP];
+            A.u[0] = Pf[5 * NPP]; A.u[1] = Pf[6 * NPP]; A.u[2] = Pf[7 * NPP];
+            A.thp = Pf[8 * NPP];
+            A.pp = Pf[9 * NPP];
             A.c = r_sqrt(gR * (T0 + A.thp));
             A.umag = r_sqrt(A.u[0] * A.u[0] + A.u[1] * A.u[1] + A.u[2] * A.u[2]);
             Prim<R> Q{};
@@ -306,17 +334,32 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     }
 }
 
-template <typename R, typename S, int X, int Y, int Z>
+template <typename R, typename S, int X, int Y, int Z, bool DIN>
 __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z, false>(d, a);
+    rk_stage_body<R, S, X, Y, Z, false, DIN>(d, a);
 }
 
-// N = 7 (512 nodes, one element per work-group): registers capped so that two
-// work-groups (16 waves) share a CU in both precisions.
-template <typename R, typename S, int W>
+// N = 7 (512 nodes, one element per work-group), node offsets in 32 bits,
+// registers capped at W waves per SIMD: measured on the 64x64x32 N=7
+// workload, FP32 runs fastest at 6 (three work-groups per CU), FP64 at 4
+// (two work-groups per CU).
+template <typename R, typename S, int W, bool DIN>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
 rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, 8, 8, 8, true>(d, a);
+    rk_stage_body<R, S, 8, 8, 8, true, DIN>(d, a);
+}
+
+template <typename R, typename S, int X, int Y, int Z, bool DIN>
+static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
+                             hipStream_t st) {
+    constexpr int W = sizeof(R) == 4 ? 6 : 4;
+    if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
+        if (d.nb * sizeof(double) < (1ull << 32)) {
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN>), dim3(blocks), dim3(threads), lds, st, d, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN>), dim3(blocks), dim3(threads), lds, st, d, a);
 }
 
 template <typename R, typename S, int X, int Y, int Z>
@@ -333,28 +376,8 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
     const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
     if (lds > 65536) return hipErrorInvalidValue;
     a.epb = epb;
-    // the N = 7 kernels form node offsets in 32 bits
-    const bool off32 = d.nb * sizeof(double) < (1ull << 32);
-    if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
-      if (off32) {
-        static const int w = [] {
-            const char* s = getenv("NSEM_RK_WAVES");
-            return s ? atoi(s) : 0;
-        }();
-        // measured on the 64x64x32 N=7 workload: FP32 fastest at 6 waves/SIMD
-        // (three work-groups per CU), FP64 at 4 (two work-groups per CU)
-        if (w == 8)
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 8>), dim3(blocks), dim3(threads), lds, st, d, a);
-        else if (w == 6 || (w == 0 && sizeof(R) == 4))
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 6>), dim3(blocks), dim3(threads), lds, st, d, a);
-        else if (w == 5)
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 5>), dim3(blocks), dim3(threads), lds, st, d, a);
-        else
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, 4>), dim3(blocks), dim3(threads), lds, st, d, a);
-        return hipGetLastError();
-      }
-    }
-    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
+    if (a.din[0]) launch_rk_kernel<R, S, X, Y, Z, true>(d, a, blocks, threads, lds, st);
+    else launch_rk_kernel<R, S, X, Y, Z, false>(d, a, blocks, threads, lds, st);
     return hipGetLastError();
 }
 

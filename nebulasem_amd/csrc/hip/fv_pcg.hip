@@ -23,6 +23,8 @@
 // visible across XCDs), so results are run-to-run identical.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fv_kernels.hpp"
 
 namespace nsem {
@@ -110,6 +112,69 @@ __global__ void __launch_bounds__(kThreads) spmv_dot_kernel(FvOp<R> A, const R* 
         R y = apply_row(A, p, n);
         Ap[n] = y;
         v[0] += (double)p[n] * (double)y;
+    }
+    block_partials<1>(v, slots);
+}
+
+// Structured grids: Ap = A p marching along x.  Each thread owns one (j, k)
+// column of a chunk of x-planes and keeps p and the x-coupling of the
+// previous, current and next plane in registers, so every array is read from
+// HBM once; the y and z neighbours are the adjacent threads' loads (L2).
+// Work-groups are mapped XCD-contiguously (work-group b on XCD b % 8 takes
+// logical group (b % 8) G/8 + b/8), so the column blocks of one x-chunk, and
+// their y neighbours, share an L2.  Same row arithmetic as apply_row.
+struct MarchGrid {
+    uint32_t ncolblk, nchunk, chunk, groups;   // groups: multiple of 8
+};
+
+inline MarchGrid march_grid(int nx, int ny, int nz) {
+    MarchGrid m{};
+    const uint64_t ncol = (uint64_t)ny * nz;
+    m.ncolblk = (uint32_t)((ncol + kThreads - 1) / kThreads);
+    uint32_t nchunk = kCgSlots / std::max<uint32_t>(m.ncolblk, 1);
+    nchunk = std::max<uint32_t>(1, std::min<uint32_t>(nchunk, (uint32_t)nx));
+    m.chunk = (nx + nchunk - 1) / nchunk;
+    m.nchunk = (nx + m.chunk - 1) / m.chunk;
+    m.groups = (m.nchunk * m.ncolblk + 7) / 8 * 8;
+    return m;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kThreads) spmv_dot_march_kernel(FvOp<R> A, MarchGrid mg, const R* __restrict__ p,
+                                                                 R* __restrict__ Ap, const CgScalars* sc,
+                                                                 double* slots) {
+    if (sc->done) return;
+    double v[1] = {0};
+    const uint32_t lb = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+    if (lb < mg.nchunk * mg.ncolblk) {
+        const uint32_t c = lb / mg.ncolblk, cb = lb - c * mg.ncolblk;
+        const uint64_t sx = (uint64_t)A.ny * A.nz;
+        const uint64_t col = (uint64_t)cb * kThreads + threadIdx.x;
+        if (col < sx) {
+            const int j = (int)(col / A.nz), k = (int)(col - (uint64_t)j * A.nz);
+            const int i0 = c * mg.chunk, i1 = min(A.nx, (int)(i0 + mg.chunk));
+            const R* c0 = A.cpl;
+            const R* c1 = A.cpl + A.n;
+            const R* c2 = A.cpl + 2 * A.n;
+            uint64_t n = (uint64_t)i0 * sx + col;
+            R pm = i0 > 0 ? p[n - sx] : R(0), c0m = i0 > 0 ? c0[n - sx] : R(0);
+            R pc = p[n];
+            for (int i = i0; i < i1; i++, n += sx) {
+                const bool hx = i + 1 < A.nx;
+                const R pn = hx ? p[n + sx] : R(0);
+                const R c0c = c0[n];
+                R acc = pc * A.diag[n];
+                if (hx) acc -= pn * c0c;
+                if (i > 0) acc -= pm * c0m;
+                if (j + 1 < A.ny) acc -= p[n + A.nz] * c1[n];
+                if (j > 0) acc -= p[n - A.nz] * c1[n - A.nz];
+                if (k + 1 < A.nz) acc -= p[n + 1] * c2[n];
+                if (k > 0) acc -= p[n - 1] * c2[n - 1];
+                Ap[n] = acc;
+                v[0] += (double)pc * (double)acc;
+                pm = pc; pc = pn; c0m = c0c;
+            }
+        }
     }
     block_partials<1>(v, slots);
 }
@@ -353,9 +418,17 @@ template <typename R>
 hipError_t fv_cg_iterations(const FvOp<R>& A, int count, R* x, R* r, R* z, R* p, R* Ap, CgScalars* sc,
                             double* slots, hipStream_t s) {
     const unsigned g = grid_for(A.n);
+    const MarchGrid mg = A.structured ? march_grid(A.nx, A.ny, A.nz) : MarchGrid{};
+    const bool march = A.structured && mg.groups <= (unsigned)kCgSlots;
     for (int it = 0; it < count; it++) {
-        hipLaunchKernelGGL(spmv_dot_kernel<R>, dim3(g), dim3(kThreads), 0, s, A, p, Ap, sc, slots);
-        hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(kThreads), 0, s, sc, slots, g);
+        if (march) {
+            hipLaunchKernelGGL(spmv_dot_march_kernel<R>, dim3(mg.groups), dim3(kThreads), 0, s, A, mg, p, Ap, sc,
+                               slots);
+            hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(kThreads), 0, s, sc, slots, mg.groups);
+        } else {
+            hipLaunchKernelGGL(spmv_dot_kernel<R>, dim3(g), dim3(kThreads), 0, s, A, p, Ap, sc, slots);
+            hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(kThreads), 0, s, sc, slots, g);
+        }
         hipLaunchKernelGGL(update_kernel<R>, dim3(g), dim3(kThreads), 0, s, A.n, x, r, z, p, Ap, A.idiag, sc, slots);
         hipLaunchKernelGGL(beta_kernel, dim3(1), dim3(kThreads), 0, s, sc, slots, g);
         hipLaunchKernelGGL(pupdate_kernel<R>, dim3(g), dim3(kThreads), 0, s, A.n, p, z, sc);
