@@ -233,6 +233,53 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def cpu_reference(steps=6, threads=16):
+    """The reference's own CPU path (apps/euler, its OpenMP build compiled
+    from the reference sources into oracle/_ref by oracle/ref.mk) on a
+    bounded sample of the workload: 16x16x8 elements at N = 7 (1,048,576
+    nodes, the workload's element shape and order), split forward-Euler
+    steps, `threads` OpenMP threads.  Step time = spacing of the per-step
+    log lines (setup excluded), median over the steps."""
+    import subprocess
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _cases import PATCHES_3D, _spec, euler_controls, field_file as ref_field_file
+    exe = os.path.join(ROOT, "oracle", "_ref", "euler_omp")
+    mesher = os.path.join(ROOT, "oracle", "_ref", "mesh")
+    if not (os.path.exists(exe) and os.path.exists(mesher)):
+        return None
+    case = dict(ndiv=(16, 16, 8), size=(2500.0, 2500.0, 1250.0), order=(7, 7, 7), patches=PATCHES_3D,
+                dt=0.005, check_steps=[steps], T=["gaussian 0 0.5 1250 1250 600 400 400 400"])
+    names = sorted(n for n, _ in PATCHES_3D)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "spec"), "w") as f:
+            f.write(_spec(case))
+        with open(os.path.join(d, "controls"), "w") as f:
+            f.write(euler_controls(case).replace("write_interval               1\n",
+                                                 "write_interval               100000\n"))
+        for fn, nc, gens, kind in (("T0.txt", 1, case["T"], "NEUMANN"), ("U0.txt", 3, ["uniform 0 0 0"], "SYMMETRY"),
+                                   ("p0.txt", 1, ["uniform 0"], "NEUMANN"), ("rho0.txt", 1, ["uniform 0"], "NEUMANN")):
+            with open(os.path.join(d, fn), "w") as f:
+                f.write(ref_field_file(nc, gens, [(n, kind) for n in names]))
+        subprocess.run([mesher, "spec", "-o", "grid_0.bin"], cwd=d, check=True, capture_output=True)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        t = time.time()
+        r = subprocess.run([exe, "controls"], cwd=d, env=env, capture_output=True, text=True, timeout=300)
+        wall = time.time() - t
+        if r.returncode != 0:
+            log(f"reference run failed: {r.stderr[-300:]}")
+            return None
+        stamps = [int(ln.split()[0]) for ln in r.stdout.splitlines() if " Time " in ln]
+    dts = np.diff(stamps) * 1e-3
+    if len(dts) < 1:
+        return None
+    step = float(np.median(dts))
+    nodes = 16 * 16 * 8 * 512
+    return {"value": NQ * nodes / step, "unit": "DOF*stage/s", "cores": threads, "kind": "reference",
+            "sample": f"reference apps/euler (OpenMP build, {threads} threads), split forward-Euler step "
+                      f"(1 stage/step), 16x16x8 elements N=7 = {nodes} nodes, median of {len(dts)} steps "
+                      f"{step * 1e3:.0f} ms/step ({wall:.1f}s run incl. setup)"}
+
+
 def cpu_baseline(seconds_budget=20.0):
     """The oracle (single-threaded C restatement of the reference's split
     step) on a bounded sample: 16x16x8 elements at N = 7 (1,048,576 nodes,
@@ -377,7 +424,12 @@ def main():
     fv = None
     if not args.no_fv:
         fv = fv_bench(args.fv_n)
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None
+    if not args.no_cpu:
+        port = cpu_baseline(seconds_budget=10.0)
+        cpu = cpu_reference() or port
+        if cpu is not port:
+            cpu["port_single_thread"] = port
     line = {
         "metric": "dGSEM DOF*stage/s (FP32-perturbation SSP-RK3, N=7)",
         "value": res[32]["rate"],

@@ -67,4 +67,27 @@ $(OBJ)/geomdump.o: oracle/ref_geomdump.cpp
 $(OUT)/geomdump: $(OBJ)/geomdump.o $(OUT)/libnebulasem_ref.a
 	$(CXX) -o $@ $^ $(LDFLAGS)
 
+# The reference's OpenMP build of the Euler and Poisson apps (its USE_OMP
+# option, -O3 as its release build), for timing the CPU baseline on the host
+# cores; results are not compared bit-for-bit with anything.
+OBJOMP   ?= oracle/_obj_omp
+OMPFLAGS := $(filter-out -O2 -ffp-contract=off,$(CXXFLAGS)) -O3 -fopenmp
+LIBOBJOMP := $(patsubst %.cpp,$(OBJOMP)/%.o,$(LIBSRC))
+
+$(OBJOMP)/%.o: $(REF)/%.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(OMPFLAGS) -c $< -o $@
+
+$(OUT)/libnebulasem_ref_omp.a: $(LIBOBJOMP)
+	@mkdir -p $(OUT)
+	ar rcs $@ $^
+
+$(OUT)/euler_omp: $(OBJOMP)/apps/euler/euler.o $(OUT)/libnebulasem_ref_omp.a
+	$(CXX) -fopenmp -o $@ $^ $(LDFLAGS)
+
+$(OUT)/potential_omp: $(OBJOMP)/apps/potential/potential.o $(OUT)/libnebulasem_ref_omp.a
+	$(CXX) -fopenmp -o $@ $^ $(LDFLAGS)
+
+all: $(OUT)/euler_omp $(OUT)/potential_omp
+
 .PHONY: all
