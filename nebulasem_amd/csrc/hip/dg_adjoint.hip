@@ -21,12 +21,14 @@
 // Dual<1>.
 //
 // Step level (SSP-RK3 in increment form, d1 = dt L(q), d2 = (d1 + dt L(q+d1))/4,
-// q+ = q + 2(d2 + dt L(q+d2))/3), backwards:
+// q+ = q + 2(d2 + dt L(q+d2))/3), backwards, each line one VJP launch with
+// the algebra fused into its epilogue:
 //   t3 = J(q+d2)^T lam;  ad2 = 2/3 (lam + dt t3);  acc = lam + 2/3 dt t3
 //   t2 = J(q+d1)^T ad2;  ad1 = 1/4 (ad2 + dt t2);  acc += 1/4 dt t2
 //   t1 = J(q)^T ad1;                               lam = acc + dt t1
 // with the forward states recomputed from checkpoints stored every
-// `checkpoint` steps (one segment of states held at a time).
+// `checkpoint` steps, one segment at a time (its states and stage increments
+// held while the segment is swept backwards).
 #pragma clang fp contract(fast)
 
 #include <hip/hip_runtime.h>
@@ -48,6 +50,100 @@ __device__ __forceinline__ D1 seeded(double v, bool seed) {
     D1 r(v);
     r.d[0] = seed ? 1.0 : 0.0;
     return r;
+}
+
+// d p' / d Theta' of p' = p_ref expm1(gamma log1p(Theta' / (rho_ref T0))).
+__device__ __forceinline__ double dpp_dtp(const Prim<double>& P, double rref, double pref, double T0, double gam) {
+    return gam * (pref + P.pp) / (rref * T0 + P.tp);
+}
+
+// Cotangent of the conserved state (rho', m, Theta') from cotangents of the
+// primitives u, theta and p' (u = m/rho, theta = (Theta' + rho_ref T0)/rho).
+__device__ __forceinline__ void prim_to_conserved(const Prim<double>& P, double rref, double pref, double T0,
+                                                  double gam, const double ub[3], double thb, double ppb,
+                                                  double qb[5]) {
+    const double rho = rref + P.rp, ir = 1.0 / rho, th = T0 + P.thp;
+    for (int k = 0; k < 3; k++) qb[1 + k] += ub[k] * ir;
+    qb[0] -= (ub[0] * P.u[0] + ub[1] * P.u[1] + ub[2] * P.u[2]) * ir + thb * th * ir;
+    qb[4] += thb * ir + ppb * dpp_dtp(P, rref, pref, T0, gam);
+}
+
+// Vector-Jacobian product of the flux rows F(Q) at one node: Q-cotangent
+// (added to qb) of the row cotangents fbar[5][3].
+__device__ __forceinline__ void flux_rows_vjp(const Prim<double>& P, double rref, double pref, double T0, double gam,
+                                              const double fbar[5][3], double qb[5]) {
+    double ub[3] = {0, 0, 0}, thb = 0, ppb = 0;
+    for (int c = 0; c < 3; c++) {
+        qb[1 + c] += fbar[0][c];                       // F_rho = m
+        qb[1 + c] += fbar[4][c] * (T0 + P.thp);        // F_Theta = m theta
+        thb += fbar[4][c] * P.m[c];
+    }
+    for (int r = 0; r < 3; r++) {                      // F_m_r = m_r u + p' e_r
+        ppb += fbar[1 + r][r];
+        for (int c = 0; c < 3; c++) {
+            qb[1 + r] += fbar[1 + r][c] * P.u[c];
+            ub[c] += fbar[1 + r][c] * P.m[r];
+        }
+    }
+    prim_to_conserved(P, rref, pref, T0, gam, ub, thb, ppb, qb);
+}
+
+// Vector-Jacobian product of the interior Rusanov flux (rusanov(), owner O,
+// neighbour N) with respect to the conserved state of one side S, for the
+// flux cotangent phi[5]; added to qb.
+__device__ __forceinline__ void rusanov_vjp_side(const Prim<double>& O, const Prim<double>& N, const double nv[3],
+                                                 double fI, double T0, double gam, double gR, const double phi[5],
+                                                 bool side_o, double rrS, double prS, double qb[5]) {
+    const double fJ = 1.0 - fI;
+    const Prim<double>& S = side_o ? O : N;
+    const double fS = side_o ? fI : fJ, sD = side_o ? -1.0 : 1.0;
+    const double lam = ((O.umag * fI + N.umag * fJ) + (O.c * fI + N.c * fJ)) * 0.5;
+    const double nmag = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const double en[3] = {nv[0] / nmag, nv[1] / nmag, nv[2] / nmag};
+    const double Dm = (N.m[0] - O.m[0]) * nv[0] + (N.m[1] - O.m[1]) * nv[1] + (N.m[2] - O.m[2]) * nv[2];
+    const double phe = phi[1] * en[0] + phi[2] * en[1] + phi[3] * en[2];
+    const double lbar = -phi[0] * (N.rp - O.rp) * nmag - phe * Dm - phi[4] * (N.tp - O.tp) * nmag;
+    const double rho = rrS + S.rp;
+    const double mnS = S.m[0] * nv[0] + S.m[1] * nv[1] + S.m[2] * nv[2];
+    const double unS = S.u[0] * nv[0] + S.u[1] * nv[1] + S.u[2] * nv[2];
+    const double thS = T0 + S.thp;
+    const double phm = phi[1] * S.m[0] + phi[2] * S.m[1] + phi[3] * S.m[2];
+    const double phn = phi[1] * nv[0] + phi[2] * nv[1] + phi[3] * nv[2];
+    qb[0] += -lam * nmag * phi[0] * sD - fS * phm * unS / rho;
+    for (int k = 0; k < 3; k++)
+        qb[1 + k] += fS * nv[k] * phi[0] + fS * (phi[1 + k] * unS + phm * nv[k] / rho) - phe * lam * sD * nv[k] +
+                     fS * phi[4] * thS * nv[k];
+    qb[4] += -lam * nmag * phi[4] * sD;
+    double thb = fS * phi[4] * mnS;
+    const double sb = 0.5 * fS * lbar;               // cotangent of |U_S| and of c_S
+    double ub[3] = {0, 0, 0};
+    if (S.umag > 0)
+        for (int k = 0; k < 3; k++) ub[k] = sb * S.u[k] / S.umag;
+    if (S.c > 0) thb += sb * gR / (2.0 * S.c);
+    prim_to_conserved(S, rrS, prS, T0, gam, ub, thb, fS * phn, qb);
+}
+
+// Boundary face: the flux through the ghost state built from the owner
+// side, differentiated by forward-mode sweeps (kept out of line: rare).
+__device__ __noinline__ void ghost_face_vjp(const double QA[5], const double nvd[3], int kind, const double uv[3],
+                                            double rrA, double prA, double T0, double gam, double gR,
+                                            const double coef[5], double contrib[5]) {
+    const D1 nv[3] = {D1(nvd[0]), D1(nvd[1]), D1(nvd[2])};
+#pragma nounroll
+    for (int kk = 0; kk < 5; kk++) {
+        Prim<D1> A;
+        A.rp = seeded(QA[0], kk == 0);
+        A.m[0] = seeded(QA[1], kk == 1); A.m[1] = seeded(QA[2], kk == 2); A.m[2] = seeded(QA[3], kk == 3);
+        A.tp = seeded(QA[4], kk == 4);
+        make_prim(A, D1(rrA), D1(prA), D1(T0), D1(gam), D1(gR));
+        Prim<D1> N;
+        ghost_state(A, nv, kind, uv, D1(rrA), N);
+        D1 flux[5];
+        rusanov(A, N, nv, D1(0.0), D1(T0), flux);
+        double acc = 0;
+        for (int v = 0; v < 5; v++) acc += coef[v] * flux[v].d[0];
+        contrib[kk] = acc;
+    }
 }
 
 template <int NXc, int NYc, int NZc>
@@ -106,20 +202,11 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
             fbar[row][1] = h0 * J[6] + h1 * J[1] + h2 * J[4];
             fbar[row][2] = h0 * J[8] + h1 * J[7] + h2 * J[2];
         }
-#pragma nounroll
-        for (int kk = 0; kk < 5; kk++) {
-            Prim<D1> P;
-            P.rp = seeded(Qn[0], kk == 0);
-            P.m[0] = seeded(Qn[1], kk == 1); P.m[1] = seeded(Qn[2], kk == 2); P.m[2] = seeded(Qn[3], kk == 3);
-            P.tp = seeded(Qn[4], kk == 4);
-            make_prim(P, D1(rref), D1(pref), D1(T0), D1(gam), D1(gR));
-            double acc = 0;
-            for (int row = 0; row < 5; row++) {
-                D1 f0, f1, f2;
-                flux_row(P, row, D1(T0), f0, f1, f2);
-                acc += fbar[row][0] * f0.d[0] + fbar[row][1] * f1.d[0] + fbar[row][2] * f2.d[0];
-            }
-            qbar[kk] += acc;
+        {
+            Prim<double> P;
+            P.rp = Qn[0]; P.m[0] = Qn[1]; P.m[1] = Qn[2]; P.m[2] = Qn[3]; P.tp = Qn[4];
+            make_prim(P, rref, pref, T0, gam, gR);
+            flux_rows_vjp(P, rref, pref, T0, gam, fbar, qbar);
         }
         if (d.buoyancy) qbar[0] += wn[1] * d.g[0] + wn[2] * d.g[1] + wn[3] * d.g[2];
 
@@ -133,50 +220,35 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                 const bool ghost = (ef.meta >> 5) & 1;
                 const int og = ef.meta & 7;
                 const double wgt = face_weight(d, own ? g : og, fa, fb);
-                const D1 nv[3] = {D1(d.gFN[3 * (size_t)ef.facet] * wgt), D1(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
-                                  D1(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
                 const uint64_t na = (uint64_t)e * NP + fnode;
                 const double rrA = d.rho_ref[na], prA = d.p_ref[na];
                 double QA[5], coef[5];
                 for (int v = 0; v < 5; v++) QA[v] = Ql[v * NP + fnode];
-                Prim<D1> B;
-                int kind = 0;
-                double uv[3] = {0, 0, 0};
+                const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                                       d.gFN[3 * (size_t)ef.facet + 2] * wgt};
                 if (!ghost) {
+                    // interior face: the hand-written Rusanov VJP
                     const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
                     const double cvm = d.cV[m];
-                    B.rp = D1(a.q[0][m] + (a.din[0] ? a.din[0][m] : 0.0));
-                    B.m[0] = D1(a.q[1][m] + (a.din[1] ? a.din[1][m] : 0.0));
-                    B.m[1] = D1(a.q[2][m] + (a.din[2] ? a.din[2][m] : 0.0));
-                    B.m[2] = D1(a.q[3][m] + (a.din[3] ? a.din[3][m] : 0.0));
-                    B.tp = D1(a.q[4][m] + (a.din[4] ? a.din[4][m] : 0.0));
-                    make_prim(B, D1(d.rho_ref[m]), D1(d.p_ref[m]), D1(T0), D1(gam), D1(gR));
+                    Prim<double> Ap, Bp;
+                    Ap.rp = QA[0]; Ap.m[0] = QA[1]; Ap.m[1] = QA[2]; Ap.m[2] = QA[3]; Ap.tp = QA[4];
+                    make_prim(Ap, rrA, prA, T0, gam, gR);
+                    Bp.rp = a.q[0][m] + (a.din[0] ? a.din[0][m] : 0.0);
+                    Bp.m[0] = a.q[1][m] + (a.din[1] ? a.din[1][m] : 0.0);
+                    Bp.m[1] = a.q[2][m] + (a.din[2] ? a.din[2][m] : 0.0);
+                    Bp.m[2] = a.q[3][m] + (a.din[3] ? a.din[3][m] : 0.0);
+                    Bp.tp = a.q[4][m] + (a.din[4] ? a.din[4][m] : 0.0);
+                    make_prim(Bp, d.rho_ref[m], d.p_ref[m], T0, gam, gR);
                     const double s = own ? -1.0 : 1.0;
                     for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - a.w[v][m] / cvm);
+                    if (own) rusanov_vjp_side(Ap, Bp, nvd, 0.5, T0, gam, gR, coef, true, rrA, prA, contrib);
+                    else rusanov_vjp_side(Bp, Ap, nvd, 0.5, T0, gam, gR, coef, false, rrA, prA, contrib);
                 } else {
                     const BcTable& bt = *d.bc;
                     const int pi = ef.meta >> 8;
-                    kind = bt.kind[1][pi];
-                    uv[0] = bt.value[1][pi]; uv[1] = bt.value[2][pi]; uv[2] = bt.value[3][pi];
+                    const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
                     for (int v = 0; v < 5; v++) coef[v] = -Wl[v * NP + fnode];
-                }
-#pragma nounroll
-                for (int kk = 0; kk < 5; kk++) {
-                    Prim<D1> A;
-                    A.rp = seeded(QA[0], kk == 0);
-                    A.m[0] = seeded(QA[1], kk == 1); A.m[1] = seeded(QA[2], kk == 2); A.m[2] = seeded(QA[3], kk == 3);
-                    A.tp = seeded(QA[4], kk == 4);
-                    make_prim(A, D1(rrA), D1(prA), D1(T0), D1(gam), D1(gR));
-                    Prim<D1> N;
-                    if (!ghost) N = B;
-                    else ghost_state(A, nv, kind, uv, D1(rrA), N);
-                    D1 flux[5];
-                    const D1 fI = ghost ? D1(0.0) : D1(0.5);
-                    if (own) rusanov(A, N, nv, fI, D1(T0), flux);
-                    else rusanov(N, A, nv, fI, D1(T0), flux);
-                    double acc = 0;
-                    for (int v = 0; v < 5; v++) acc += coef[v] * flux[v].d[0];
-                    contrib[kk] = acc;
+                    ghost_face_vjp(QA, nvd, bt.kind[1][pi], uv, rrA, prA, T0, gam, gR, coef, contrib);
                 }
             }
             for (int v = 0; v < 5; v++) FL[v * NFN + t] = contrib[v];
@@ -192,16 +264,10 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         const int t = face_task_index(g, fa, fb, NX, NY, NZ);
         for (int v = 0; v < 5; v++) qbar[v] += FL[v * NFN + t];
     }
-    for (int v = 0; v < 5; v++) a.t[v][n] = qbar[v];
-}
-
-__global__ void __launch_bounds__(256) adj_combine_kernel(uint64_t nb, AdjCombineArgs a) {
-    const uint64_t n = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (n >= nb) return;
     for (int v = 0; v < 5; v++) {
-        const double t = a.t[v][n];
-        if (a.ad_out[v]) a.ad_out[v][n] = a.c1 * (a.ad_in[v][n] + a.dt * t);
-        a.acc_out[v][n] = a.acc_in[v][n] + a.c2 * a.dt * t;
+        if (a.t[0]) a.t[v][n] = qbar[v];
+        if (a.ad_out[0]) a.ad_out[v][n] = a.c1 * (wn[v] + a.dt * qbar[v]);
+        if (a.acc_out[0]) a.acc_out[v][n] = a.acc_in[v][n] + a.c2 * a.dt * qbar[v];
     }
 }
 
@@ -236,29 +302,39 @@ hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
     return launch_vjp_t<0, 0, 0>(d, a, st);
 }
 
-hipError_t launch_adj_combine(uint64_t nb, const AdjCombineArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(adj_combine_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, nb, a);
-    return hipGetLastError();
-}
-
 }  // namespace nsem
 
 using namespace nsem;
 
 namespace {
 
-// Five FP64 node arrays.
+// Five FP64 node arrays: a view into the handle's workspace.
 struct Field5 {
-    DevBuf b[5];
-    hipError_t alloc(uint64_t nb) {
-        for (int v = 0; v < 5; v++) {
-            hipError_t e = b[v].alloc(nb * sizeof(double));
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
-    double* p(int v) const { return b[v].as<double>(); }
+    double* b[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* p(int v) const { return b[v]; }
 };
+
+// n views of 5 node arrays each, in the handle's workspace (allocated once
+// and grown when a call needs more, so repeated calls allocate nothing).
+hipError_t workspace(nsem_dg* d, size_t n, std::vector<Field5>& out) {
+    const size_t need = n * 5 * d->nb * sizeof(double);
+    if (d->adj_ws.bytes < need) {
+        if (d->adj_ws.p) {
+            hipError_t e = hipStreamSynchronize(d->stream);
+            if (e != hipSuccess) return e;
+            (void)hipFree(d->adj_ws.p);
+            d->adj_ws.p = nullptr;
+            d->adj_ws.bytes = 0;
+        }
+        hipError_t e = d->adj_ws.alloc(need);
+        if (e != hipSuccess) return e;
+    }
+    out.resize(n);
+    double* base = d->adj_ws.as<double>();
+    for (size_t k = 0; k < n; k++)
+        for (int v = 0; v < 5; v++) out[k].b[v] = base + (k * 5 + v) * d->nb;
+    return hipSuccess;
+}
 
 hipError_t upload5(const Field5& f, const double* host, uint64_t nb, hipStream_t s) {
     for (int v = 0; v < 5; v++) {
@@ -302,54 +378,36 @@ hipError_t fwd_stage(nsem_dg* d, int stage, const double* const q[5], const doub
 }
 
 struct AdjointWork {
-    Field5 d1, d2, t, ad1, ad2, acc;
-    hipError_t alloc(uint64_t nb) {
-        hipError_t e;
-        if ((e = d1.alloc(nb)) != hipSuccess || (e = d2.alloc(nb)) != hipSuccess || (e = t.alloc(nb)) != hipSuccess ||
-            (e = ad1.alloc(nb)) != hipSuccess || (e = ad2.alloc(nb)) != hipSuccess || (e = acc.alloc(nb)) != hipSuccess)
-            return e;
-        return hipSuccess;
-    }
+    Field5 ad1, ad2, acc;
 };
 
 void ptrs(const Field5& f, double* out[5]) { for (int v = 0; v < 5; v++) out[v] = f.p(v); }
 void cptrs(const Field5& f, const double* out[5]) { for (int v = 0; v < 5; v++) out[v] = f.p(v); }
 
-// lam <- (dq+/dq)^T lam for the step from state q.
-int adjoint_step(nsem_dg* d, const double* const q[5], Field5& lam, AdjointWork& w) {
-    double *d1[5], *d2[5], *t[5], *ad1[5], *ad2[5], *acc[5], *lp[5];
-    ptrs(w.d1, d1); ptrs(w.d2, d2); ptrs(w.t, t); ptrs(w.ad1, ad1); ptrs(w.ad2, ad2); ptrs(w.acc, acc);
-    ptrs(lam, lp);
-    const uint64_t nb = d->nb;
+// lam <- (dq+/dq)^T lam for the step from state q whose stage increments
+// d1, d2 were stored by the forward recomputation.
+int adjoint_step(nsem_dg* d, const double* const q[5], const double* const d1[5], const double* const d2[5],
+                 Field5& lam, AdjointWork& w) {
+    double *ad1[5], *ad2[5], *acc[5], *lp[5];
+    ptrs(w.ad1, ad1); ptrs(w.ad2, ad2); ptrs(w.acc, acc); ptrs(lam, lp);
     const double dt = d->dev.dt;
-    HIP_TRY(fwd_stage(d, 0, q, nullptr, d1, nullptr));
-    HIP_TRY(fwd_stage(d, 1, q, d1, d2, nullptr));
-    auto vjp = [&](const double* const din[5], double* const wv[5]) -> hipError_t {
+    auto vjp = [&](const double* const din[5], double* const wv[5], double* const ad_out[5],
+                   double* const acc_in[5], double* const acc_out[5], double c1, double c2) -> hipError_t {
         VjpArgs a{};
         for (int v = 0; v < 5; v++) {
-            a.q[v] = q[v]; a.din[v] = din ? din[v] : nullptr; a.w[v] = wv[v]; a.t[v] = t[v];
+            a.q[v] = q[v]; a.din[v] = din ? din[v] : nullptr; a.w[v] = wv[v];
+            a.ad_out[v] = ad_out ? ad_out[v] : nullptr;
+            a.acc_in[v] = acc_in[v]; a.acc_out[v] = acc_out[v];
         }
+        a.dt = dt; a.c1 = c1; a.c2 = c2;
         return launch_rk_vjp(d->dev, a, d->stream);
     };
-    auto combine = [&](double* const ad_in[5], double* const ad_out[5], double* const acc_in[5],
-                       double* const acc_out[5], double c1, double c2) -> hipError_t {
-        AdjCombineArgs c{};
-        for (int v = 0; v < 5; v++) {
-            c.t[v] = t[v];
-            c.ad_in[v] = ad_in ? ad_in[v] : nullptr;
-            c.ad_out[v] = ad_out ? ad_out[v] : nullptr;
-            c.acc_in[v] = acc_in[v];
-            c.acc_out[v] = acc_out[v];
-        }
-        c.dt = dt; c.c1 = c1; c.c2 = c2;
-        return launch_adj_combine(nb, c, d->stream);
-    };
-    HIP_TRY(vjp(d2, lp));
-    HIP_TRY(combine(lp, ad2, lp, acc, 2.0 / 3.0, 2.0 / 3.0));
-    HIP_TRY(vjp(d1, ad2));
-    HIP_TRY(combine(ad2, ad1, acc, acc, 0.25, 0.25));
-    HIP_TRY(vjp(nullptr, ad1));
-    HIP_TRY(combine(nullptr, nullptr, acc, lp, 0.0, 1.0));
+    // t3 = J(q+d2)^T lam: ad2 = 2/3 (lam + dt t3), acc = lam + 2/3 dt t3
+    HIP_TRY(vjp(d2, lp, ad2, lp, acc, 2.0 / 3.0, 2.0 / 3.0));
+    // t2 = J(q+d1)^T ad2: ad1 = 1/4 (ad2 + dt t2), acc += 1/4 dt t2
+    HIP_TRY(vjp(d1, ad2, ad1, acc, acc, 0.25, 0.25));
+    // t1 = J(q)^T ad1: lam = acc + dt t1
+    HIP_TRY(vjp(nullptr, ad1, nullptr, acc, lp, 0.0, 1.0));
     return NSEM_OK;
 }
 
@@ -369,14 +427,15 @@ int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t) {
     int rc = check_adjoint_support(d);
     if (rc) return rc;
     const uint64_t nb = d->nb;
-    Field5 W, T;
-    HIP_TRY(W.alloc(nb));
-    HIP_TRY(T.alloc(nb));
+    std::vector<Field5> ws;
+    HIP_TRY(workspace(d, 2, ws));
+    const Field5 &W = ws[0], &T = ws[1];
     HIP_TRY(upload5(W, w, nb, d->stream));
     VjpArgs a{};
     for (int v = 0; v < 5; v++) {
         a.q[v] = d->mq[d->mcur][v].as<double>(); a.din[v] = nullptr; a.w[v] = W.p(v); a.t[v] = T.p(v);
     }
+    // (no epilogue arrays: t only)
     HIP_TRY(launch_rk_vjp(d->dev, a, d->stream));
     HIP_TRY(download5(t, T, nb, d->stream));
     return NSEM_OK;
@@ -385,10 +444,9 @@ int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t) {
 int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
     if (!d || !dq || !L || !dL) return nsem_fail(NSEM_EINVAL, "null argument");
     const uint64_t nb = d->nb;
-    Field5 TQ, Lout, TL;
-    HIP_TRY(TQ.alloc(nb));
-    HIP_TRY(Lout.alloc(nb));
-    HIP_TRY(TL.alloc(nb));
+    std::vector<Field5> ws;
+    HIP_TRY(workspace(d, 3, ws));
+    const Field5 &TQ = ws[0], &Lout = ws[1], &TL = ws[2];
     HIP_TRY(upload5(TQ, dq, nb, d->stream));
     RkArgs<double> a{};
     for (int v = 0; v < 5; v++) {
@@ -412,8 +470,9 @@ int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
 int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq) {
     if (!d || !dq || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
     const uint64_t nb = d->nb;
-    Field5 tq[2], td[2];
-    for (int k = 0; k < 2; k++) { HIP_TRY(tq[k].alloc(nb)); HIP_TRY(td[k].alloc(nb)); }
+    std::vector<Field5> ws;
+    HIP_TRY(workspace(d, 4, ws));
+    const Field5 tq[2] = {ws[0], ws[1]}, td[2] = {ws[2], ws[3]};
     HIP_TRY(upload5(tq[0], dq, nb, d->stream));
     int tc = 0;
     for (int32_t s = 0; s < nsteps; s++) {
@@ -454,42 +513,74 @@ int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* 
     if (rc) return rc;
     const uint64_t nb = d->nb;
     const int nseg = (nsteps + checkpoint - 1) / checkpoint;
-    std::vector<Field5> ck(nseg);
-    for (auto& f : ck) HIP_TRY(f.alloc(nb));
-    std::vector<Field5> seg(checkpoint);
-    for (auto& f : seg) HIP_TRY(f.alloc(nb));
-    Field5 L;
-    AdjointWork w;
-    HIP_TRY(L.alloc(nb));
-    HIP_TRY(w.alloc(nb));
+    // checkpoints; one segment's states and each step's stage increments
+    // d1, d2; the adjoint and its stage work arrays
+    std::vector<Field5> ws;
+    HIP_TRY(workspace(d, (size_t)nseg + 3 * (size_t)checkpoint + 4, ws));
+    const Field5* ck = ws.data();
+    const Field5* seg = ck + nseg;
+    const Field5* sd1 = seg + checkpoint;
+    const Field5* sd2 = sd1 + checkpoint;
+    Field5 L = sd2[checkpoint];
+    AdjointWork w{sd2[checkpoint + 1], sd2[checkpoint + 2], sd2[checkpoint + 3]};
     HIP_TRY(upload5(L, lam, nb, d->stream));
-    // forward sweep, checkpoints at the start of every segment
-    for (int s = 0; s < nsteps; s++) {
+    // one segment forward from its first state seg[0], recording each step's
+    // state and stage increments; the state after the segment goes to `last`
+    // (if given)
+    auto record = [&](int s0, int s1, double* const last[5]) -> int {
+        for (int s = s0; s < s1; s++) {
+            const double* q[5];
+            double *d1[5], *d2[5];
+            cptrs(seg[s - s0], q); ptrs(sd1[s - s0], d1); ptrs(sd2[s - s0], d2);
+            HIP_TRY(fwd_stage(d, 0, q, nullptr, d1, nullptr));
+            HIP_TRY(fwd_stage(d, 1, q, d1, d2, nullptr));
+            if (s + 1 < s1) {
+                double* qn[5];
+                ptrs(seg[s - s0 + 1], qn);
+                HIP_TRY(fwd_stage(d, 2, q, d2, nullptr, qn));
+            } else if (last) {
+                HIP_TRY(fwd_stage(d, 2, q, d2, nullptr, last));
+            }
+        }
+        return NSEM_OK;
+    };
+    // forward sweep: checkpoints at the start of every segment; the last
+    // segment is recorded as it runs, so it needs no recomputation
+    const int last0 = (nseg - 1) * checkpoint;
+    for (int s = 0; s < last0; s++) {
         if (s % checkpoint == 0) {
             const double* src[5];
             for (int v = 0; v < 5; v++) src[v] = d->mq[d->mcur][v].as<double>();
             HIP_TRY(copy5(ck[s / checkpoint], src, nb, d->stream));
         }
-        if ((rc = nsem_dg_rk3_steps(d, 1, 64))) return rc;
+        if ((rc = nsem_rk3_fp64_queue(d, 1))) return rc;
     }
-    // backward sweep: recompute each segment's states from its checkpoint
+    if (nsteps > 0) {
+        const double* src[5];
+        double* fin[5];
+        for (int v = 0; v < 5; v++) {
+            src[v] = d->mq[d->mcur][v].as<double>();
+            fin[v] = d->mq[1 - d->mcur][v].as<double>();
+        }
+        HIP_TRY(copy5(seg[0], src, nb, d->stream));
+        if ((rc = record(last0, nsteps, fin))) return rc;
+        d->mcur = 1 - d->mcur;
+        d->mol_current = true;
+    }
+    // backward sweep, segment by segment (recomputed from its checkpoint
+    // except the last, recorded above)
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int s0 = sg * checkpoint, s1 = std::min(nsteps, s0 + checkpoint);
-        const double* src[5];
-        cptrs(ck[sg], src);
-        HIP_TRY(copy5(seg[0], src, nb, d->stream));
-        for (int s = s0; s + 1 < s1; s++) {
-            const double* q[5];
-            double *d1[5], *d2[5], *qn[5];
-            cptrs(seg[s - s0], q); ptrs(w.d1, d1); ptrs(w.d2, d2); ptrs(seg[s - s0 + 1], qn);
-            HIP_TRY(fwd_stage(d, 0, q, nullptr, d1, nullptr));
-            HIP_TRY(fwd_stage(d, 1, q, d1, d2, nullptr));
-            HIP_TRY(fwd_stage(d, 2, q, d2, nullptr, qn));
+        if (sg < nseg - 1) {
+            const double* src[5];
+            cptrs(ck[sg], src);
+            HIP_TRY(copy5(seg[0], src, nb, d->stream));
+            if ((rc = record(s0, s1, nullptr))) return rc;
         }
         for (int s = s1 - 1; s >= s0; s--) {
-            const double* q[5];
-            cptrs(seg[s - s0], q);
-            if ((rc = adjoint_step(d, q, L, w))) return rc;
+            const double *q[5], *d1[5], *d2[5];
+            cptrs(seg[s - s0], q); cptrs(sd1[s - s0], d1); cptrs(sd2[s - s0], d2);
+            if ((rc = adjoint_step(d, q, d1, d2, L, w))) return rc;
         }
     }
     HIP_TRY(download5(lam, L, nb, d->stream));

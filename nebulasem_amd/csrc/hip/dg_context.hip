@@ -377,6 +377,8 @@ int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
 
 }  // namespace
 
+int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double, double>(d, nsteps, d->d64); }
+
 extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {

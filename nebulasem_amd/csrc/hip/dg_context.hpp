@@ -48,6 +48,8 @@ struct nsem_dg {
     DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
     // diagnostics: g.x per node, block partials, result
     DevBuf gh, diag_slots, diag_out;
+    // workspace of the derivative entry points (grown on demand, kept)
+    DevBuf adj_ws;
     int mcur = 0;
     bool mol_current = false;
     // profiling of the dominant kernels
@@ -62,3 +64,6 @@ struct nsem_dg {
     }
 };
 
+// nsteps FP64 SSP-RK3 steps of the method-of-lines state, queued on the
+// handle's stream without waiting or checking the non-finite flag.
+int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps);

@@ -45,28 +45,25 @@ struct RkArgs {
 };
 // Vector-Jacobian product of the method-of-lines residual at Q = q + din:
 // t = (dL/dQ)^T w, all FP64 SoA node arrays (din may be null).
+// Epilogue (the adjoint's stage algebra, fused): with t = J^T w at a node,
+//   t_out = t                        (when t_out is set)
+//   ad_out = c1 (w + dt t)           (when ad_out is set)
+//   acc_out = acc_in + c2 dt t       (when acc_out is set)
+// ad_out and acc_out must not alias w (other work-groups read w).
 struct VjpArgs {
     int epb;      // elements per work-group (set by the launcher)
     const double* q[5];
     const double* din[5];
     const double* w[5];
     double* t[5];
-};
-hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st);
-size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb);
-
-// The discrete adjoint's stage algebra on node arrays:
-//   ad_out = c1 (ad_in + dt t)   (skipped when ad_out is null)
-//   acc_out = acc_in + c2 dt t
-struct AdjCombineArgs {
-    const double* t[5];
-    const double* ad_in[5];
     double* ad_out[5];
     const double* acc_in[5];
     double* acc_out[5];
     double dt, c1, c2;
 };
-hipError_t launch_adj_combine(uint64_t nb, const AdjCombineArgs& a, hipStream_t st);
+hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st);
+size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb);
+
 
 // LDS of the RK stage kernel, in values of the residual type: per element
 // max(15 flux rows x NP, 10 padded primitive rows + 5 face-flux rows x NFN),
