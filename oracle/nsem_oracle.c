@@ -9,7 +9,7 @@
  * by gather (cds / rusanov), a cell-by-cell surface integral and a weak-form
  * volume integral, then explicit boundary-condition passes.
  *
- *   om_euler_split_steps  apps/euler/euler.cpp:179-287 (diffusion NO, BDF1):
+ *   om_euler_split_steps  apps/euler/euler.cpp:179-287 (BDF1; diffusion NO or YES):
  *       T += T0; Fc = rho*U; lambda = (cds|U| + cds sqrt(g R T))/2;
  *       rho  <- Solve(divf(U*rho) + ddt)               field.h:3416-3478,3740-3920
  *       p    <- P0 (rho T R / P0)^gamma - p_ref        euler.cpp:211-213
@@ -67,6 +67,10 @@ typedef struct {
     /* per patch boundary kinds and values (values: 1 or 3 per patch) */
     const int32_t *bc_rho, *bc_U, *bc_T, *bc_p;
     const double *val_rho, *val_U, *val_T, *val_p;
+    /* artificial viscosity (euler block `diffusion YES`): mu = rho viscosity,
+     * heat diffusivity mu / Pr (Fluid::viscosity, Fluid::Pr) */
+    int32_t diffusion;
+    double viscosity, Pr;
 } om_euler;
 
 static double dot3(const double* p, const double* q) { /* right-nested, tensor.h:117 */
@@ -267,6 +271,100 @@ static void cds_scalar(const om_mesh* m, const double* f, double* out) {
     }
 }
 
+/* Strong-form DG gradient of a node field with `rows` components,
+ * gradf<true>(p, true) (field.h:3327-3372 with grad_flux<true>,
+ * field.h:3050-3082): the surface lift fN (x) (cds(p) - p) summed cell by
+ * cell over the cell's faces (every cell, ghosts included), then the local
+ * derivative (Jinv cV grad psi_index1(x_index)) (x) p[index1] along the three
+ * GLL lines, divided by cV, ghost nodes copied from their owners and then
+ * set on the field's NEUMANN and SYMMETRY patches as below.  Output
+ * G[(node * 3 + a) * rows + b] = d_a p_b (the reference's mul(Vector, .)). */
+static void gradf_strong(const om_mesh* m, int rows, const double* p, const int32_t* kind,
+                         const double* val, double* G) {
+    const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    double* fF = (double*)calloc(nfn * rows, sizeof(double));
+    for (uint64_t k = 0; k < nfn; k++) {
+        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        if (c2 >= m->naf) continue;
+        for (int b = 0; b < rows; b++)
+            fF[k * rows + b] = p[(uint64_t)c1 * rows + b] * m->fI[k] + p[(uint64_t)c2 * rows + b] * (1 - m->fI[k]);
+    }
+    memset(G, 0, sizeof(double) * m->naf * 3 * rows);
+    for (uint32_t i = 0; i < m->ncells; i++)
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++)
+            for (int32_t n = 0; n < m->npf; n++) {
+                uint64_t k = (uint64_t)m->allFaces[f] * m->npf + n;
+                uint32_t c1 = m->FO[k], c2 = m->FN[k];
+                const double* nv = &m->fN[3 * k];
+                if (c1 >= (uint64_t)i * m->np && c1 < (uint64_t)(i + 1) * m->np) {
+                    for (int b = 0; b < rows; b++) {
+                        const double dp = fF[k * rows + b] - p[(uint64_t)c1 * rows + b];
+                        for (int a = 0; a < 3; a++) G[((uint64_t)c1 * 3 + a) * rows + b] += nv[a] * dp;
+                    }
+                } else if (c2 < m->naf) {
+                    for (int b = 0; b < rows; b++) {
+                        const double dp = fF[k * rows + b] - p[(uint64_t)c2 * rows + b];
+                        for (int a = 0; a < 3; a++) G[((uint64_t)c2 * 3 + a) * rows + b] -= nv[a] * dp;
+                    }
+                }
+            }
+    free(fF);
+    const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
+    for (uint32_t ci = 0; ci < m->nbcs; ci++)
+        for (int ii = 0; ii < NPX; ii++)
+            for (int jj = 0; jj < NPY; jj++)
+                for (int kk = 0; kk < NPZ; kk++) {
+                    uint64_t index = (uint64_t)ci * m->np + ii * NPY * NPZ + jj * NPZ + kk;
+                    double Jin[9];
+                    for (int c = 0; c < 9; c++) Jin[c] = m->Jinv[9 * index + c] * m->cV[index];
+                    for (int line = 0; line < 3; line++) {
+                        int npts = line == 0 ? NPX : (line == 1 ? NPY : NPZ);
+                        for (int t = 0; t < npts; t++) {
+                            int im = ii, jm = jj, km = kk;
+                            if (line == 0) im = t;
+                            else if (line == 1) { if (t == jj) continue; jm = t; }
+                            else { if (t == kk) continue; km = t; }
+                            uint64_t index1 = (uint64_t)ci * m->np + im * NPY * NPZ + jm * NPZ + km;
+                            double d[3];
+                            d[0] = m->dpsi[0][ii * NPX + im] * m->psi[1][jj * NPY + jm] * m->psi[2][kk * NPZ + km];
+                            d[1] = m->psi[0][ii * NPX + im] * m->dpsi[1][jj * NPY + jm] * m->psi[2][kk * NPZ + km];
+                            d[2] = m->psi[0][ii * NPX + im] * m->psi[1][jj * NPY + jm] * m->dpsi[2][kk * NPZ + km];
+                            double dj[3] = {Jin[XX] * d[0] + Jin[XY] * d[1] + Jin[XZ] * d[2],
+                                            Jin[YX] * d[0] + Jin[YY] * d[1] + Jin[YZ] * d[2],
+                                            Jin[ZX] * d[0] + Jin[ZY] * d[1] + Jin[ZZ] * d[2]};
+                            for (int a = 0; a < 3; a++)
+                                for (int b = 0; b < rows; b++)
+                                    G[(index * 3 + a) * rows + b] += dj[a] * p[index1 * rows + b];
+                        }
+                    }
+                }
+    for (uint64_t i = 0; i < m->naf * 3 * rows; i++) G[i] = G[i] / m->cV[i / (3 * rows)];
+    fill_bcs(m, G, 3 * rows);
+    /* fillBCs(r, p.fIndex) (field.h:2745-2768): on the field's own NEUMANN
+     * patches the ghost gradient is the patch's BCondition value read as
+     * the gradient type, on SYMMETRY patches it is zero.  The value read as
+     * a Vector (scalar field) is (value, shape, tvalue), as a Tensor (vector
+     * field) it is (value[3], shape, tvalue[3], tshape, zMin) in the order
+     * XX YY ZZ XY YZ XZ YX ZY ZX; shape, tvalue, tshape and zMin are 0 for
+     * the boundary types handled here, so only value survives, on the
+     * diagonal of the tensor. */
+    for (int32_t b = 0; b < m->nbnd; b++) {
+        if (kind[b] != OM_NEUMANN && kind[b] != OM_SYMMETRY) continue;
+        double gv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (kind[b] == OM_NEUMANN) {
+            if (rows == 1) gv[0] = val[b];
+            else for (int c = 0; c < 3; c++) gv[3 * c + c] = val[3 * b + c];
+        }
+        for (uint32_t j = m->boff[b]; j < m->boff[b + 1]; j++)
+            for (int32_t n = 0; n < m->npf; n++) {
+                uint64_t k = (uint64_t)m->bfaces[j] * m->npf + n;
+                uint32_t c2 = m->FN[k];
+                if (c2 >= m->naf) continue;
+                for (int c = 0; c < 3 * rows; c++) G[(uint64_t)c2 * 3 * rows + c] = gv[c];
+            }
+    }
+}
+
 /* Hydrostatic background and initial density from (T', p')
  * (euler.cpp:104-149 with buoyancy, non-spherical). */
 void om_euler_hydrostatic(const om_mesh* m, const om_euler* e, const double* T, const double* p,
@@ -327,6 +425,8 @@ int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_re
     double* fq = (double*)malloc(sizeof(double) * N * 9);
     double* q = (double*)malloc(sizeof(double) * N * 3);
     double* r = (double*)malloc(sizeof(double) * N * 3);
+    double* mu = (double*)malloc(sizeof(double) * N);
+    double* G = e->diffusion ? (double*)malloc(sizeof(double) * N * 9) : NULL;
     int status = 0;
     for (int step = 0; step < nsteps; step++) {
         for (uint64_t i = 0; i < N; i++) T[i] += e->T0;
@@ -338,6 +438,8 @@ int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_re
         for (uint64_t i = 0; i < N; i++) w[i] = sqrt(gam * R * T[i]);
         cds_scalar(m, w, t1);
         for (uint64_t k = 0; k < nfn; k++) lam[k] = (lam[k] + t1[k]) / 2;
+        /* artificial viscosity mu = rho viscosity (rho^n) */
+        for (uint64_t i = 0; i < N; i++) mu[i] = e->diffusion ? rho[i] * e->viscosity : 0.0;
         memcpy(rhof, rho, sizeof(double) * N);
 
         /* rho equation */
@@ -356,11 +458,15 @@ int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_re
         bc_scalar(m, e->bc_p, e->val_p, p);
         for (uint64_t i = 0; i < N; i++) p[i] -= p_ref[i];
 
-        /* momentum equation: flux Fc (x) U + I p, Rusanov on rho^{n+1} U^n */
+        /* momentum equation: flux Fc (x) U + I p - mu gradf<true>(U),
+         * Rusanov on rho^{n+1} U^n */
+        if (e->diffusion) gradf_strong(m, 3, U, e->bc_U, e->val_U, G);
         for (uint64_t i = 0; i < N; i++) {
             for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++)
+                for (int b = 0; b < 3; b++) {
                     fq[9 * i + 3 * a + b] = Fc[3 * i + a] * U[3 * i + b] + (a == b ? p[i] : 0.0 * p[i]);
+                    if (e->diffusion) fq[9 * i + 3 * a + b] = fq[9 * i + 3 * a + b] - mu[i] * G[9 * i + 3 * a + b];
+                }
             for (int c = 0; c < 3; c++) q[3 * i + c] = rho[i] * U[3 * i + c];
         }
         divf_dg(m, 3, fq, q, lam, r);
@@ -375,9 +481,14 @@ int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_re
         }
         bc_vector(m, e->bc_U, e->val_U, U);
 
-        /* potential temperature equation: flux Fc T, Rusanov on rho^{n+1} T^n */
+        /* potential temperature equation: flux Fc T - (mu/Pr) gradf<true>(T),
+         * Rusanov on rho^{n+1} T^n */
+        if (e->diffusion) gradf_strong(m, 1, T, e->bc_T, e->val_T, G);
         for (uint64_t i = 0; i < N; i++) {
-            for (int c = 0; c < 3; c++) fq[3 * i + c] = Fc[3 * i + c] * T[i];
+            for (int c = 0; c < 3; c++) {
+                fq[3 * i + c] = Fc[3 * i + c] * T[i];
+                if (e->diffusion) fq[3 * i + c] = fq[3 * i + c] - (mu[i] * (1 / e->Pr)) * G[3 * i + c];
+            }
             q[i] = rho[i] * T[i];
         }
         divf_dg(m, 1, fq, q, lam, r);
@@ -392,7 +503,7 @@ int om_euler_split_steps(const om_mesh* m, const om_euler* e, const double* p_re
             if (!isfinite(rho[i]) || !isfinite(T[i])) status = 5;
         if (status) break;
     }
-    free(Fc); free(lam); free(t1); free(w); free(rhof); free(fq); free(q); free(r);
+    free(Fc); free(lam); free(t1); free(w); free(rhof); free(fq); free(q); free(r); free(mu); free(G);
     return status;
 }
 

@@ -51,13 +51,15 @@ def euler_controls(case):
     npx                          %d
     npy                          %d
     npz                          %d
+    viscosity                    %r
 }
 euler
 {
-    diffusion                    NO
+    diffusion                    %s
     buoyancy                     YES
 }
-""" % (max(case.get("check_steps", [1])), case.get("dt", 0.005), o[0], o[1], o[2])
+""" % (max(case.get("check_steps", [1])), case.get("dt", 0.005), o[0], o[1], o[2],
+       case.get("viscosity", 1.568e-5), "NO" if case.get("viscosity") is None else "YES")
 
 
 def field_file(ncomp, gens, bcs):
@@ -148,6 +150,12 @@ EULER_CASES = [
     dict(name="e3_n7", ndiv=(2, 2, 2), size=(1000, 1000, 500), order=(7, 7, 7),
          patches=PATCHES_3D, dt=0.002, check_steps=[1, 3],
          T=["gaussian 0 0.5 400 600 200 300 300 300"]),
+    dict(name="e3_visc_n3", ndiv=(3, 2, 2), size=(3000, 2000, 2000), order=(3, 3, 3),
+         patches=PATCHES_3D, dt=0.01, check_steps=[1, 4], viscosity=1.5,
+         T=["gaussian 0 1.0 1500 1000 700 800 800 800"]),
+    dict(name="cfg0_2d_visc", ndiv=(16, 1, 16), size=(1000, 100, 1000), order=(4, 0, 4),
+         patches=PATCHES_2D, dt=0.005, check_steps=[1, 10], viscosity=1.5,
+         T=["gaussian 0 0.5 500 50 350 250 1000 250"]),
     dict(name="e3_aniso_bubbles", ndiv=(4, 3, 2), size=(1000, 900, 500), order=(2, 3, 4),
          patches=PATCHES_3D, dt=0.004, check_steps=[2],
          T=["gaussian 0 0.7 300 300 200 250 250 250", "gaussian 0 0.3 700 600 300 200 200 200",

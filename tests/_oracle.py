@@ -32,10 +32,15 @@ _lib = None
 _lib_ulp = None
 
 
+def _stale(so):
+    src = os.path.join(ROOT, "oracle", "nsem_oracle.c")
+    return not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src)
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(ORACLE_SO):
+        if _stale(ORACLE_SO):
             build_oracle()
         _lib = C.CDLL(ORACLE_SO)
     return _lib
@@ -45,7 +50,7 @@ def lib_ulp():
     """The oracle built with the 1-ulp pow perturbation (see nsem_oracle.c)."""
     global _lib_ulp
     if _lib_ulp is None:
-        if not os.path.exists(ORACLE_ULP_SO):
+        if _stale(ORACLE_ULP_SO):
             build_oracle()
         _lib_ulp = C.CDLL(ORACLE_ULP_SO)
     return _lib_ulp
@@ -71,7 +76,8 @@ class OmEuler(C.Structure):
     _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
                 ("cv", C.c_double), ("g", C.c_double * 3), ("buoyancy", C.c_int32),
                 ("bc_rho", PI), ("bc_U", PI), ("bc_T", PI), ("bc_p", PI),
-                ("val_rho", PD), ("val_U", PD), ("val_T", PD), ("val_p", PD)]
+                ("val_rho", PD), ("val_U", PD), ("val_T", PD), ("val_p", PD),
+                ("diffusion", C.c_int32), ("viscosity", C.c_double), ("Pr", C.c_double)]
 
 
 def _p(a, t):
@@ -129,12 +135,16 @@ def euler_diagnostics(om, params, p_ref, rho, U, T, p):
 
 
 def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -9.80606),
-                buoyancy=1):
-    """bcs: dict field -> (kinds int32 array, values float64 array)."""
+                buoyancy=1, viscosity=None, Pr=0.9):
+    """bcs: dict field -> (kinds int32 array, values float64 array);
+    viscosity: None for `diffusion NO`, else the artificial viscosity."""
     e = OmEuler()
     e.dt, e.P0, e.T0, e.cp, e.cv = dt, P0, T0, cp, cv
     e.g[:] = list(g)
     e.buoyancy = buoyancy
+    e.diffusion = 0 if viscosity is None else 1
+    e.viscosity = 0.0 if viscosity is None else viscosity
+    e.Pr = Pr
     keep = []
     for f in ("rho", "U", "T", "p"):
         kinds, vals = bcs[f]

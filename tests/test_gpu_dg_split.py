@@ -33,7 +33,7 @@ def setup(oracle, tmp_path, case):
     mesh, f = load_case(tmp_path, case)
     nb = mesh.info.nbcs_field
     om = oracle.OracleMesh(mesh)
-    prm = oracle.make_params(case["dt"], bc_table(f))
+    prm = oracle.make_params(case["dt"], bc_table(f), viscosity=case.get("viscosity"))
     T = f["T"][0].reshape(-1).copy()
     p = f["p"][0].reshape(-1).copy()
     U = f["U"][0].reshape(-1).copy()
@@ -49,6 +49,8 @@ def setup(oracle, tmp_path, case):
 
 @pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
 def test_split_steps_match_oracle(oracle, tmp_path, case):
+    if case.get("viscosity") is not None:
+        pytest.skip("artificial viscosity is not on the device split path yet")
     print(f"[gpu] split steps {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field

@@ -18,7 +18,7 @@ FX = fixtures()
 def run_oracle_euler(oracle, tmp_path, case):
     mesh, f = load_case(tmp_path, case)
     om = oracle.OracleMesh(mesh)
-    prm = oracle.make_params(case["dt"], bc_table(f))
+    prm = oracle.make_params(case["dt"], bc_table(f), viscosity=case.get("viscosity"))
     T = f["T"][0].reshape(-1).copy()
     p = f["p"][0].reshape(-1).copy()
     U = f["U"][0].reshape(-1).copy()
