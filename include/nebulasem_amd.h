@@ -99,13 +99,19 @@ int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const d
 
 /* -------------------------------------------------------------- DG Euler --
  * Device-resident dGSEM compressible Euler problem on one GPU (gfx950).
- * Constants as apps/utils/properties.cpp:14-33 (P0 T0 cp cv) and the controls
- * keys dt / gravity; buoyancy as the euler block (euler.cpp:104-131).      */
+ * Constants as apps/utils/properties.cpp:14-33 (P0 T0 cp cv viscosity Pr)
+ * and the controls keys dt / gravity; buoyancy and diffusion as the euler
+ * block (euler.cpp:23, 42-45, 104-131, 188-190).  diffusion != 0 adds the
+ * artificial viscosity mu = rho viscosity (theta diffusivity mu / Pr) to the
+ * split step; the method-of-lines entry points then fail with
+ * NSEM_EUNSUPPORTED.                                                       */
 typedef struct {
     double dt;
     double P0, T0, cp, cv;
     double gravity[3];
     int32_t buoyancy;
+    int32_t diffusion;
+    double viscosity, Pr;
 } nsem_euler_params;
 typedef struct nsem_dg nsem_dg;
 

@@ -149,18 +149,21 @@ def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
 
 class EulerParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
-                ("cv", C.c_double), ("gravity", C.c_double * 3), ("buoyancy", C.c_int32)]
+                ("cv", C.c_double), ("gravity", C.c_double * 3), ("buoyancy", C.c_int32),
+                ("diffusion", C.c_int32), ("viscosity", C.c_double), ("Pr", C.c_double)]
 
 
 BC_NEUMANN, BC_SYMMETRY, BC_DIRICHLET = 0, 1, 2
 
 
 def euler_params(dt, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, gravity=(0.0, 0.0, -9.80606),
-                 buoyancy=1):
+                 buoyancy=1, diffusion=0, viscosity=1.568e-5, Pr=0.9):
+    """nsem_euler_params; viscosity and Pr default to apps/utils/properties.cpp:15-16."""
     p = EulerParams()
     p.dt, p.P0, p.T0, p.cp, p.cv = dt, P0, T0, cp, cv
     p.gravity[:] = list(gravity)
     p.buoyancy = buoyancy
+    p.diffusion, p.viscosity, p.Pr = diffusion, viscosity, Pr
     return p
 
 

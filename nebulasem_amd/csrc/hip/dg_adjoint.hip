@@ -412,6 +412,7 @@ int adjoint_step(nsem_dg* d, const double* const q[5], const double* const d1[5]
 }
 
 int check_adjoint_support(const nsem_dg* d) {
+    if (int rc = nsem_mol_supported(d)) return rc;
     const DgDev& v = d->dev;
     if (v.np > 512 || rk_vjp_lds_bytes(v.npx, v.npy, v.npz, 1) > 65536)
         return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the adjoint kernel");
@@ -443,6 +444,7 @@ int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t) {
 
 int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
     if (!d || !dq || !L || !dL) return nsem_fail(NSEM_EINVAL, "null argument");
+    if (int rc0 = nsem_mol_supported(d)) return rc0;
     const uint64_t nb = d->nb;
     std::vector<Field5> ws;
     HIP_TRY(workspace(d, 3, ws));
@@ -469,6 +471,7 @@ int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
 
 int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq) {
     if (!d || !dq || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    if (int rc0 = nsem_mol_supported(d)) return rc0;
     const uint64_t nb = d->nb;
     std::vector<Field5> ws;
     HIP_TRY(workspace(d, 4, ws));

@@ -54,6 +54,12 @@ struct DgDev {
     const double* D[3];               // [n*n] per direction
     const double* w[3];               // [n] per direction
     const BcTable* bc;
+    // artificial viscosity of the split step (euler.cpp:188-190): mu =
+    // rho visc, theta diffusivity mu iPr; grad [12][nb] holds the strong
+    // gradients of U (3a+b = d_a U_b) and theta (9+a = d_a theta)
+    int diffusion;
+    double visc, iPr;
+    const double* grad;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

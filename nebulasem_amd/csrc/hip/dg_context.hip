@@ -49,6 +49,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the LDS staging (order <= 7 in 3D)");
     }
     if (!(prm->dt > 0)) return nsem_fail(NSEM_EINVAL, "dt must be positive");
+    if (prm->diffusion && !(prm->viscosity >= 0 && prm->Pr > 0))
+        return nsem_fail(NSEM_EINVAL, "diffusion needs viscosity >= 0 and Pr > 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return nsem_fail(NSEM_EDEVICE, "no HIP device available");
@@ -179,6 +181,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             HIP_TRY(d->d64[a][v].alloc(nb * sizeof(double)));
             HIP_TRY(d->d32[a][v].alloc(nb * sizeof(float)));
         }
+    if (prm->diffusion) HIP_TRY(d->grad.alloc(12 * nb * sizeof(double)));
     HIP_TRY(d->diag_slots.alloc(kDiagBlocks * 6 * sizeof(double)));
     HIP_TRY(d->diag_out.alloc(6 * sizeof(double)));
     HIP_TRY(d->flag.alloc(sizeof(int)));
@@ -199,6 +202,10 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.eface = d->eface.as<EFace>(); dv.gFN = d->gFN.as<double>();
     for (int dir = 0; dir < 3; dir++) { dv.D[dir] = d->D[dir].as<double>(); dv.w[dir] = d->w[dir].as<double>(); }
     dv.bc = d->bc.as<BcTable>();
+    dv.diffusion = prm->diffusion ? 1 : 0;
+    dv.visc = prm->viscosity;
+    dv.iPr = 1 / prm->Pr;
+    dv.grad = d->grad.as<double>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -329,7 +336,7 @@ int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
         SplitState in{A[0].as<double>(), A[1].as<double>(), A[2].as<double>(), A[3].as<double>(), A[4].as<double>()};
         SplitOut out{B[1].as<double>(), B[2].as<double>(), B[3].as<double>(), B[4].as<double>(), B[5].as<double>()};
         if (d->profile) HIP_TRY(hipEventRecord(d->ev0, d->stream));
-        HIP_TRY(launch_split_step(d->dev, in, B[0].as<double>(), out, d->stream));
+        HIP_TRY(launch_split_step(d->dev, in, B[0].as<double>(), out, d->grad.as<double>(), d->stream));
         if (d->profile) {
             HIP_TRY(hipEventRecord(d->ev1, d->stream));
             HIP_TRY(hipEventSynchronize(d->ev1));
@@ -379,11 +386,18 @@ int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
 
 int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double, double>(d, nsteps, d->d64); }
 
+int nsem_mol_supported(const nsem_dg* d) {
+    if (d->prm.diffusion)
+        return nsem_fail(NSEM_EUNSUPPORTED, "artificial viscosity is carried by the split step only");
+    return NSEM_OK;
+}
+
 extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    int rc;
+    int rc = nsem_mol_supported(d);
+    if (rc) return rc;
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
     else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
@@ -401,8 +415,9 @@ int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
 
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
     if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    int rc = nsem_mol_supported(d);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
-    int rc;
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
     else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");

@@ -48,6 +48,8 @@ struct nsem_dg {
     DevBuf mq[2][5], d64[2][5], d32[2][5], flag;
     // diagnostics: g.x per node, block partials, result
     DevBuf gh, diag_slots, diag_out;
+    // strong gradients of U and theta of the viscous split step [12][nb]
+    DevBuf grad;
     // workspace of the derivative entry points (grown on demand, kept)
     DevBuf adj_ws;
     int mcur = 0;
@@ -67,3 +69,7 @@ struct nsem_dg {
 // nsteps FP64 SSP-RK3 steps of the method-of-lines state, queued on the
 // handle's stream without waiting or checking the non-finite flag.
 int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps);
+
+// NSEM_OK, or NSEM_EUNSUPPORTED when the problem has artificial viscosity,
+// which only the split step carries.
+int nsem_mol_supported(const nsem_dg* d);

@@ -100,8 +100,9 @@ constexpr int kDiagBlocks = 1024;
 hipError_t launch_diagnostics(const DgDev& d, const DiagState& s, const double* gh, double* slots,
                               double* out, hipStream_t st);
 
-// One split forward-Euler step (two fused element kernels).
+// One split forward-Euler step (two fused element kernels; with
+// d.diffusion a gradient kernel first, writing grad, which d.grad reads).
 hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
-                             hipStream_t st);
+                             double* grad, hipStream_t st);
 
 }  // namespace nsem

@@ -1,12 +1,16 @@
-// The reference's split forward-Euler step of apps/euler/euler.cpp:179-287
-// (diffusion NO), as two fused element kernels:
+// The reference's split forward-Euler step of apps/euler/euler.cpp:179-287,
+// as two fused element kernels (three with artificial viscosity):
 //
+//   split_grad: (diffusion YES) strong gradients of U^n and theta^n
+//               (gradf<true>(., true)), stored per node
 //   split_rho : rho^{n+1} = (r + rho^n ap) / ap,  ap = -cV/dt,
 //               r = surface(Rusanov) - volume(weak) of F = U rho
 //   split_ut  : p' = P0 (rho^{n+1} theta R / P0)^gamma - p_ref
-//               U^{n+1}     from F = (rho^n U) (x) U + p' I, q = rho^{n+1} U,
-//                           source (rho^{n+1} - rho_ref) g
-//               theta^{n+1} from F = (rho^n U) theta,      q = rho^{n+1} theta
+//               U^{n+1}     from F = (rho^n U) (x) U + p' I - mu grad U,
+//                           q = rho^{n+1} U, source (rho^{n+1} - rho_ref) g
+//               theta^{n+1} from F = (rho^n U) theta - mu iPr grad theta,
+//                           q = rho^{n+1} theta;  mu = rho^n visc (0 without
+//                           diffusion)
 //
 // One work-group holds EPB elements (EPB*NP threads, one per node).  A node
 // computes its own flux, writes its contravariant components to LDS, then
@@ -70,7 +74,7 @@ __device__ __forceinline__ void volume_reference_order(double* r, const double* 
 
 // ---------------------------------------------------------------------------
 template <int NXc, int NYc, int NZc>
-__global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out) {
+__global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int epb = blockDim.x / NP;
@@ -150,9 +154,133 @@ __global__ void __launch_bounds__(1024) split_rho_kernel(DgDev d, SplitState s, 
 }
 
 // ---------------------------------------------------------------------------
+// Strong-form gradients of U and theta at every interior node, as
+// gradf<true>(X, true) builds them (field.h:3327-3372 with grad_flux<true>,
+// field.h:3050-3082): the surface lift fN (x) (cds(X) - X) over the faces
+// through the node, in the cell's face order, added on the owner side and
+// subtracted on the neighbour side; then (Jin grad psi_s)(x_t) (x) X_s with
+// Jin = Jinv cV of the node itself, over the node's x line (itself included),
+// its y line and its z line (itself excluded), in that order; then / cV.
+// Ghost traces as in the flux kernels: U by its boundary condition, theta a
+// copy.  grad[c][nb]: c = 3a+b for d_a U_b, 9+a for d_a theta.
 template <int NXc, int NYc, int NZc>
-__global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, const double* __restrict__ rho1,
-                                                       SplitOut o) {
+__global__ void __launch_bounds__(512) split_grad_kernel(DgDev d, SplitState s, double* __restrict__ grad) {
+    const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
+    const int NP = NX * NY * NZ;
+    const int epb = blockDim.x / NP;
+    const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
+    const uint32_t e = blockIdx.x * epb + le;
+    const bool active = le < epb && e < d.nelem;
+    extern __shared__ double lds[];
+    double* P = lds + (size_t)le * 4 * NP;  // U_x, U_y, U_z, theta rows
+    const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
+    const uint64_t n = (uint64_t)e * NP + q;
+
+    double u[3] = {0, 0, 0}, th = 0;
+    if (active) {
+        u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
+        th = s.T[n] + d.T0;
+        P[q] = u[0]; P[NP + q] = u[1]; P[2 * NP + q] = u[2]; P[3 * NP + q] = th;
+    }
+    __syncthreads();
+    if (!active) return;
+
+    double gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gt[3] = {0, 0, 0};
+    for (int g = 0; g < 6; g++) {
+        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+        const EFace ef = d.eface[(size_t)e * 6 + g];
+        if (!((ef.meta >> 4) & 1)) continue;
+        const bool own = (ef.meta >> 3) & 1;
+        const bool ghost = (ef.meta >> 5) & 1;
+        const int og = ef.meta & 7;
+        int a, b;
+        face_ab(g, i, j, k, a, b);
+        const double wgt = face_weight(d, own ? g : og, a, b);
+        const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                              d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+        double ou[3], oth;
+        if (!ghost) {
+            const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+            ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+            oth = s.T[m] + d.T0;
+        } else {
+            ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
+            oth = th;
+        }
+        const double fI = ghost ? 0.0 : 0.5;
+        // cds with the owner's value weighted by fI, minus the node's own
+        double du[3];
+        for (int c = 0; c < 3; c++)
+            du[c] = (own ? u[c] * fI + ou[c] * (1 - fI) : ou[c] * fI + u[c] * (1 - fI)) - u[c];
+        const double dth = (own ? th * fI + oth * (1 - fI) : oth * fI + th * (1 - fI)) - th;
+        for (int aa = 0; aa < 3; aa++) {
+            for (int c = 0; c < 3; c++)
+                gu[3 * aa + c] = own ? gu[3 * aa + c] + nv[aa] * du[c] : gu[3 * aa + c] - nv[aa] * du[c];
+            gt[aa] = own ? gt[aa] + nv[aa] * dth : gt[aa] - nv[aa] * dth;
+        }
+    }
+    double J[9];
+    for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * d.nb + n];
+    auto add = [&](int src, double w0, double w1, double w2) {
+        const double p0 = P[src], p1 = P[NP + src], p2 = P[2 * NP + src], p3 = P[3 * NP + src];
+        gu[0] += w0 * p0; gu[1] += w0 * p1; gu[2] += w0 * p2;
+        gu[3] += w1 * p0; gu[4] += w1 * p1; gu[5] += w1 * p2;
+        gu[6] += w2 * p0; gu[7] += w2 * p1; gu[8] += w2 * p2;
+        gt[0] += w0 * p3; gt[1] += w1 * p3; gt[2] += w2 * p3;
+    };
+    // off the node, grad psi_s(x_t) has one non-zero component, so Jin grad
+    // psi is one column of Jin (XX0 YY1 ZZ2 XY3 YZ4 XZ5 YX6 ZY7 ZX8) times it
+    const int syz = NY * NZ;
+    const double* D0 = d.D[0];
+    const double* D1 = d.D[1];
+    const double* D2 = d.D[2];
+    for (int t = 0; t < NX; t++) {
+        const int src = t * syz + j * NZ + k;
+        if (t == i) {
+            const double d0 = D0[i * NX + i], d1 = D1[j * NY + j], d2 = D2[k * NZ + k];
+            add(src, J[0] * d0 + J[3] * d1 + J[5] * d2, J[6] * d0 + J[1] * d1 + J[4] * d2,
+                J[8] * d0 + J[7] * d1 + J[2] * d2);
+        } else {
+            const double D = D0[i * NX + t];
+            add(src, J[0] * D, J[6] * D, J[8] * D);
+        }
+    }
+    for (int t = 0; t < NY; t++) {
+        if (t == j) continue;
+        const double D = D1[j * NY + t];
+        add(i * syz + t * NZ + k, J[3] * D, J[1] * D, J[7] * D);
+    }
+    for (int t = 0; t < NZ; t++) {
+        if (t == k) continue;
+        const double D = D2[k * NZ + t];
+        add(i * syz + j * NZ + t, J[5] * D, J[4] * D, J[2] * D);
+    }
+    const double cv = d.cV[n];
+    for (int c = 0; c < 9; c++) grad[(size_t)c * d.nb + n] = gu[c] / cv;
+    for (int c = 0; c < 3; c++) grad[(size_t)(9 + c) * d.nb + n] = gt[c] / cv;
+}
+
+// Ghost-node gradients (fillBCs(r, X.fIndex), field.h:2745-2768): the
+// owner's copy, except on the field's own NEUMANN patches, where it is the
+// patch value read as the gradient type (with the default shape, tvalue and
+// zMin of 0, the value on the tensor diagonal, or (value, 0, 0) for theta),
+// and on its SYMMETRY patches, where it is zero.
+__device__ __forceinline__ void ghost_gradient(const BcTable& bt, int pi, const double* own, double* out) {
+    const int ku = bt.kind[1][pi], kt = bt.kind[2][pi];
+    for (int c = 0; c < 9; c++) out[c] = own[c];
+    if (ku == BC_SYMMETRY || ku == BC_NEUMANN)
+        for (int c = 0; c < 9; c++) out[c] = 0.0;
+    if (ku == BC_NEUMANN) { out[0] = bt.value[1][pi]; out[4] = bt.value[2][pi]; out[8] = bt.value[3][pi]; }
+    for (int c = 9; c < 12; c++) out[c] = own[c];
+    if (kt == BC_SYMMETRY || kt == BC_NEUMANN)
+        for (int c = 9; c < 12; c++) out[c] = 0.0;
+    if (kt == BC_NEUMANN) out[9] = bt.value[4][pi];
+}
+
+// ---------------------------------------------------------------------------
+template <int NXc, int NYc, int NZc, bool VISC>
+__global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, const double* __restrict__ rho1,
+                                                      SplitOut o) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int epb = blockDim.x / NP;
@@ -168,6 +296,7 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
 
     double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
     double fr[4][3];  // own flux rows: U_x U_y U_z (tensor rows), theta (vector)
+    double gown[VISC ? 12 : 1];
     if (active) {
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
@@ -177,6 +306,13 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
         for (int c = 0; c < 3; c++) fr[3][c] = fc[c] * th;
+        if constexpr (VISC) {
+            for (int c = 0; c < 12; c++) gown[c] = d.grad[(size_t)c * d.nb + n];
+            const double mu = rho0 * d.visc, imu = mu * d.iPr;
+            for (int row = 0; row < 3; row++)
+                for (int c = 0; c < 3; c++) fr[row][c] = fr[row][c] - mu * gown[3 * row + c];
+            for (int c = 0; c < 3; c++) fr[3][c] = fr[3][c] - imu * gown[9 + c];
+        }
         for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
     }
 
@@ -196,18 +332,22 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
             const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
                                   d.gFN[3 * (size_t)ef.facet + 2] * wgt};
             double orho0, orho, ou[3], oth, opp;
+            double go[VISC ? 12 : 1];
             if (!ghost) {
                 const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
                 orho0 = s.rho[m]; orho = rho1[m];
                 ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
                 oth = s.T[m] + d.T0;
                 opp = d.P0 * pow((orho * oth * R) / d.P0, gam) - d.p_ref[m];
+                if constexpr (VISC)
+                    for (int c = 0; c < 12; c++) go[c] = d.grad[(size_t)c * d.nb + m];
             } else {
                 ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
                 orho0 = rho0;
                 orho = rho;
                 oth = th;
                 opp = pp;
+                if constexpr (VISC) ghost_gradient(*d.bc, ef.meta >> 8, gown, go);
             }
             const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
             const double rO = own ? rho : orho, rN = own ? orho : rho;
@@ -224,11 +364,25 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
             const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
             const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
             const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
+            // viscous parts: mu = rho^n visc on each side, gradients G
+            double muO = 0, muN = 0;
+            const double* gO = nullptr;
+            const double* gN = nullptr;
+            if constexpr (VISC) {
+                muO = r0O * d.visc;
+                muN = r0N * d.visc;
+                gO = own ? gown : go;
+                gN = own ? go : gown;
+            }
             for (int row = 0; row < 3; row++) {
                 double fF[3];
                 for (int c = 0; c < 3; c++) {
-                    const double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
-                    const double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
+                    double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
+                    double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
+                    if constexpr (VISC) {
+                        FO = FO - muO * gO[3 * row + c];
+                        FN = FN - muN * gN[3 * row + c];
+                    }
                     const double dq = lam * (rN * uN[c] - rO * uO[c]);
                     fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
                 }
@@ -239,7 +393,14 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
             {
                 const double dq = lam * (rN * thN - rO * thO);
                 double fF[3];
-                for (int c = 0; c < 3; c++) fF[c] = (fcO[c] * thO) * fI + (fcN[c] * thN) * (1 - fI) - en[c] * dq;
+                for (int c = 0; c < 3; c++) {
+                    double FO = fcO[c] * thO, FN = fcN[c] * thN;
+                    if constexpr (VISC) {
+                        FO = FO - (muO * d.iPr) * gO[9 + c];
+                        FN = FN - (muN * d.iPr) * gN[9 + c];
+                    }
+                    fF[c] = FO * fI + FN * (1 - fI) - en[c] * dq;
+                }
                 const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
                 if (own) res[3] += flux;
                 else res[3] -= flux;
@@ -275,28 +436,37 @@ __global__ void __launch_bounds__(1024) split_ut_kernel(DgDev d, SplitState s, c
 // ---------------------------------------------------------------------------
 template <int X, int Y, int Z>
 static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
-                                 hipStream_t st) {
+                                 double* grad, hipStream_t st) {
     const int NP = d.np;
-    if (NP > 1024) return hipErrorInvalidValue;
+    if (NP > 512) return hipErrorInvalidValue;
     int epb = NP >= 256 ? 1 : 256 / NP;
     int threads = epb * NP;
     threads = (threads + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
     size_t lds_a = (size_t)epb * 12 * NP * sizeof(double);
     size_t lds_b = (size_t)epb * 15 * NP * sizeof(double);
+    size_t lds_g = (size_t)epb * 4 * NP * sizeof(double);
+    hipError_t err;
+    if (d.diffusion) {
+        if (!grad || grad != d.grad) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((split_grad_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_g, st, d, s, grad);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
     hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL((split_ut_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_b, st, d, s, rho1, o);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (d.diffusion)
+        hipLaunchKernelGGL((split_ut_kernel<X, Y, Z, true>), dim3(blocks), dim3(threads), lds_b, st, d, s, rho1, o);
+    else
+        hipLaunchKernelGGL((split_ut_kernel<X, Y, Z, false>), dim3(blocks), dim3(threads), lds_b, st, d, s, rho1, o);
     return hipGetLastError();
 }
 
 hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
-                             hipStream_t st) {
-    if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_split_t<8, 8, 8>(d, s, rho1, o, st);
-    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_split_t<5, 1, 5>(d, s, rho1, o, st);
-    if (d.npx == 4 && d.npy == 4 && d.npz == 4) return launch_split_t<4, 4, 4>(d, s, rho1, o, st);
-    return launch_split_t<0, 0, 0>(d, s, rho1, o, st);
+                             double* grad, hipStream_t st) {
+    if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_split_t<8, 8, 8>(d, s, rho1, o, grad, st);
+    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_split_t<5, 1, 5>(d, s, rho1, o, grad, st);
+    if (d.npx == 4 && d.npy == 4 && d.npz == 4) return launch_split_t<4, 4, 4>(d, s, rho1, o, grad, st);
+    return launch_split_t<0, 0, 0>(d, s, rho1, o, grad, st);
 }
 
 }  // namespace nsem

@@ -38,7 +38,10 @@ def setup(oracle, tmp_path, case):
     p = f["p"][0].reshape(-1).copy()
     U = f["U"][0].reshape(-1).copy()
     p_ref, rho_ref, rho = oracle.euler_hydrostatic(om, prm, T, p)
-    dg = DGEuler.from_fields(mesh, euler_params(case["dt"]), f)
+    visc = case.get("viscosity")
+    prm_dev = euler_params(case["dt"], diffusion=int(visc is not None),
+                           **({} if visc is None else {"viscosity": visc}))
+    dg = DGEuler.from_fields(mesh, prm_dev, f)
     pr, rr = dg.background()
     assert np.array_equal(pr, p_ref[:nb]) and np.array_equal(rr, rho_ref[:nb])
     rho_gpu = dg.initial_density(T[:nb], p[:nb])
@@ -49,8 +52,6 @@ def setup(oracle, tmp_path, case):
 
 @pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
 def test_split_steps_match_oracle(oracle, tmp_path, case):
-    if case.get("viscosity") is not None:
-        pytest.skip("artificial viscosity is not on the device split path yet")
     print(f"[gpu] split steps {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field
@@ -89,3 +90,17 @@ def test_resting_hydrostatic_state_stays_at_rest(oracle, tmp_path):
     rho, U, T, p = dg.get_state()
     assert np.abs(U).max() < 1e-9
     assert np.abs(T).max() < 1e-9
+
+
+def test_viscous_problem_rejects_method_of_lines(tmp_path):
+    """Artificial viscosity is carried by the split step only; the
+    method-of-lines entry points say so instead of dropping the term."""
+    from nebulasem_amd import NsemError
+    case = [c for c in EULER_CASES if c.get("viscosity") is not None][0]
+    mesh, f = load_case(tmp_path, case)
+    dg = DGEuler.from_fields(mesh, euler_params(case["dt"], diffusion=1, viscosity=case["viscosity"]), f)
+    with pytest.raises(NsemError, match="split step only"):
+        dg.rk3_steps(1, precision=64)
+    with pytest.raises(NsemError, match="split step only"):
+        dg.residual_vjp(np.zeros((5, mesh.info.nbcs_field)))
+    dg.close()
