@@ -34,11 +34,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_per_node(s_in, s_base, s_out, s_geo, N):
+def algorithmic_bytes_per_node(s_in, s_base, s_out, s_geo, N, affine=False):
     """Bytes per node per stage of the DG residual+update (the path's
     algorithmic figure): n_q (s_in + s_base + s_out) + 10 s_geo + 2*8
-    + 3/(N+1) (3 s_geo + 8)."""
-    return NQ * (s_in + s_base + s_out) + 10 * s_geo + 2 * 8 + 3.0 / (N + 1) * (3 * s_geo + 8)
+    + 3/(N+1) (3 s_geo + 8).  With affine elements the metric (JcV, cV:
+    10 s_geo) is one 80-byte record per element instead of per node."""
+    metric = 80.0 / (N + 1) ** 3 if affine else 10 * s_geo
+    return NQ * (s_in + s_base + s_out) + metric + 2 * 8 + 3.0 / (N + 1) * (3 * s_geo + 8)
 
 
 # Box corner points and the block's hex vertex order over them (x, then y,
@@ -401,14 +403,15 @@ def main():
     # the reference's split forward-Euler step (1 stage/step)
     dg.set_state(rho, U, T, p)
     ks = max(5, K // 5)
+    affine = dg.affine()
     ts = dg.time_split(ks)
     split_rate = NQ * nb * ks / ts
     log(f"split step fp64: {ts / ks * 1e3:.2f} ms/step, {split_rate / 1e9:.2f} G DOF*stage/s")
     dg.close()
     del res[64]["q"], res[32]["q"]
 
-    b32 = algorithmic_bytes_per_node(4, 8, 8, 4, N)
-    b64 = algorithmic_bytes_per_node(8, 8, 8, 8, N)
+    b32 = algorithmic_bytes_per_node(4, 8, 8, 4, N, affine)
+    b64 = algorithmic_bytes_per_node(8, 8, 8, 8, N, affine)
     achieved = b32 * nb / res[32]["stage_s"]
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK,
@@ -416,6 +419,7 @@ def main():
                 "traffic_source": "profiles/rk_stage_64x64x32_n7_hbm_bytes.md (mean of the 3 stages)",
                 "kernel": "rk_stage_kernel_n7<float,float,6>",
                 "bytes_per_node": b32, "nodes_per_launch": nb,
+                "metric_layout": "per element (affine)" if affine else "per node",
                 "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
                 "fp64_frac": b64 * nb / res[64]["stage_s"] / HBM_PEAK,
                 "stream_copy_GBps": stream_bw / 1e9,

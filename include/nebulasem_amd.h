@@ -153,9 +153,11 @@ int nsem_dg_diagnostics(nsem_dg* d, double* out);
 int nsem_dg_get_background(const nsem_dg* d, double* p_ref, double* rho_ref);
 /* Initial density from theta' and p' as euler.cpp:136-149. */
 int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* p, double* rho);
-/* nsteps of the reference's split forward-Euler update (euler.cpp:179-287,
- * diffusion NO): rho, then p, then U and theta, each a Rusanov dGSEM residual
- * with the explicit ddt/Solve update (field.h:3740-3920, solve.cpp:563-570). */
+/* nsteps of the reference's split forward-Euler update (euler.cpp:179-287):
+ * rho, then p, then U and theta, each a Rusanov dGSEM residual with the
+ * explicit ddt/Solve update (field.h:3740-3920, solve.cpp:563-570); with
+ * prm.diffusion the U and theta fluxes carry -mu gradf<true>(U) and
+ * -mu/Pr gradf<true>(theta) (field.h:3327-3372). */
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps);
 /* nsteps of SSP-RK3 (Shu-Osher) on the method-of-lines form of the same
  * spatial operator, state = conserved perturbations (rho', rho U,
@@ -173,6 +175,11 @@ int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p)
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds);
 int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds);
 int nsem_dg_synchronize(nsem_dg* d);
+/* *all_affine = 1 when every element's Jacobian is constant (to 1e-12
+ * relative), so the method-of-lines kernels read one metric per element
+ * instead of one per node; 0 otherwise.  No reference counterpart (the
+ * reference stores Jinv per node, field.cpp:171-384). */
+int nsem_dg_affine(const nsem_dg* d, int32_t* all_affine);
 
 /* ------------------------------------------------------------ FV Poisson --
  * Pressure Poisson of the FV solvers: Solve(lap(p, mu, true) == divf(U))

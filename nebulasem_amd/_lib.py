@@ -208,6 +208,12 @@ class DGEuler:
         check(lib().nsem_dg_get_state(self._h, _pd(rho), _pd(U), _pd(T), _pd(p)))
         return rho, U, T, p
 
+    def affine(self):
+        """True when the method-of-lines kernels use one metric per element."""
+        v = C.c_int32()
+        check(lib().nsem_dg_affine(self._h, C.byref(v)))
+        return bool(v.value)
+
     def background(self):
         pr, rr = np.empty(self.nb), np.empty(self.nb)
         check(lib().nsem_dg_get_background(self._h, _pd(pr), _pd(rr)))

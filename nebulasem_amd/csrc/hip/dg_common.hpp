@@ -5,7 +5,8 @@
 // face flux needs them):
 //   node n = element*NP + i*NPY*NPZ + j*NPZ + k       (as the reference)
 //   state      SoA double arrays  rho, ux, uy, uz, T (theta'), p (p')
-//   metric     SoA double JcV[9][nb] = Jinv*cV, cV[nb]
+//   metric     SoA double JcV[9][nb] = Jinv*cV, cV[nb]; per element
+//              Jinv_e (9) and the cell volume when the elements are affine
 //   background SoA double p_ref[nb], rho_ref[nb]
 //   faces      per element, per face group g (0..5): EFace {facet, other, meta}
 //              per facet: area normal gFN (3 doubles)
@@ -60,6 +61,13 @@ struct DgDev {
     int diffusion;
     double visc, iPr;
     const double* grad;
+    // affine elements (every element's Jinv constant to 1e-12 relative): the
+    // method-of-lines kernels form JcV = Jinv_e cV and cV = vol_e W(node)
+    // from emet [nelem][10] (Jinv_e in tensor order, then the cell volume)
+    // and wnode [NP] = w_i w_j w_k / 8 instead of reading the node arrays
+    int affine;
+    const double* emet;
+    const double* wnode;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

@@ -2,6 +2,7 @@
 // and the C-ABI entry points that run steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -153,6 +154,37 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         }
     }
     });
+    // per-element metric of affine elements: Jinv_e from the element's first
+    // node, accepted when every node's Jinv agrees to 1e-12 of its largest
+    // component (an affine map has a constant Jacobian; curved or trilinear
+    // elements fail); cV is the cell volume times w_i w_j w_k / 8 on every
+    // element (field.cpp:171-384 with dg.cpp:175-325)
+    const int NPX = g.npx, NPY = g.npy, NPZ = g.npz, NPn = g.np;
+    std::vector<double> wnode(NPn);
+    for (int i = 0; i < NPX; i++)
+        for (int j = 0; j < NPY; j++)
+            for (int k = 0; k < NPZ; k++)
+                wnode[(i * NPY + j) * NPZ + k] = g.basis[0].w[i] * g.basis[1].w[j] * g.basis[2].w[k] / 8;
+    std::vector<double> emet((size_t)g.nbcs * 10);
+    std::vector<uint8_t> aff(g.nbcs, 1);
+    parallel_for(g.nbcs, [&](uint64_t cb, uint64_t ce) {
+    for (uint64_t c = cb; c < ce; c++) {
+        const uint64_t n0 = c * NPn;
+        double scale = 0;
+        for (int q = 0; q < 9; q++) {
+            emet[c * 10 + q] = g.Jinv[n0][q];
+            scale = std::max(scale, std::fabs(g.Jinv[n0][q]));
+        }
+        emet[c * 10 + 9] = lm.cv[c];
+        for (int t = 0; t < NPn && aff[c]; t++) {
+            for (int q = 0; q < 9; q++)
+                if (!(std::fabs(g.Jinv[n0 + t][q] - g.Jinv[n0][q]) <= 1e-12 * scale)) aff[c] = 0;
+            if (g.cV[n0 + t] != lm.cv[c] * wnode[t]) aff[c] = 0;
+        }
+    }
+    });
+    bool all_affine = true;
+    for (uint8_t a : aff) all_affine = all_affine && a;
     std::vector<double> gfn(3 * (size_t)g.nfacets);
     for (Int f = 0; f < g.nfacets; f++)
         for (int c = 0; c < 3; c++) gfn[3 * (size_t)f + c] = lm.fn[f][c];
@@ -161,7 +193,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         (e = upload(d->JcV32, JcV32, s)) != hipSuccess || (e = upload(d->cV32, cV32, s)) != hipSuccess ||
         (e = upload(d->p_ref, pref, s)) != hipSuccess || (e = upload(d->rho_ref, rref, s)) != hipSuccess ||
         (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess ||
-        (e = upload(d->gh, ghv, s)) != hipSuccess)
+        (e = upload(d->gh, ghv, s)) != hipSuccess || (e = upload(d->emet, emet, s)) != hipSuccess ||
+        (e = upload(d->wnode, wnode, s)) != hipSuccess)
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     for (int dir = 0; dir < 3; dir++) {
         if ((e = upload(d->D[dir], g.basis[dir].dpsi, s)) != hipSuccess ||
@@ -206,6 +239,9 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.visc = prm->viscosity;
     dv.iPr = 1 / prm->Pr;
     dv.grad = d->grad.as<double>();
+    dv.affine = all_affine ? 1 : 0;
+    dv.emet = d->emet.as<double>();
+    dv.wnode = d->wnode.as<double>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -446,6 +482,12 @@ int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     *seconds = ms * 1e-3;
+    return NSEM_OK;
+}
+
+int nsem_dg_affine(const nsem_dg* d, int32_t* all_affine) {
+    if (!d || !all_affine) return nsem_fail(NSEM_EINVAL, "null argument");
+    *all_affine = d->dev.affine;
     return NSEM_OK;
 }
 

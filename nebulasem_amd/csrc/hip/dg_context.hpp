@@ -39,7 +39,7 @@ struct nsem_dg {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t nb = 0;
     // geometry
-    DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc;
+    DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc, emet, wnode;
     // double-buffered state: rho ux uy uz T p
     DevBuf st[2][6];
     int cur = 0;

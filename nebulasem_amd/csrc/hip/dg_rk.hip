@@ -150,13 +150,17 @@ constexpr int kPrimRows = 10;
 //      one face per wavefront): Rusanov flux -> LDS
 //   5. each node subtracts/adds the fluxes of its faces in face-group order,
 //      then the stage update.
-template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN>
+// AFF: the elements are affine, so the metric is formed from one Jinv and
+// cell volume per element (JcV = Jinv_e (vol_e W), cV = vol_e W with W the
+// node's LGL weight product) instead of being read per node.
+template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN, bool AFF>
 __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int NFN = 2 * (NX * NY + NX * NZ + NY * NZ);
     const int epb = a.epb;
-    const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
+    // 512 nodes: one element per work-group, so the element index is uniform
+    const int le = NXc * NYc * NZc == 512 ? 0 : threadIdx.x / NP, q = threadIdx.x - le * NP;
     // XCD-contiguous element mapping: work-group b runs on XCD b % 8
     const uint32_t per_xcd = (gridDim.x + 7) / 8;
     const uint32_t bmap = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -196,13 +200,20 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
         }
         rref = (R)ldn<O32>(d.rho_ref, n);
         pref = (R)ldn<O32>(d.p_ref, n);
-        if constexpr (sizeof(R) == 4) cvn = ldn<O32>(d.cV32, n);
-        else cvn = (R)ldn<O32>(d.cV, n);
         R J[9];
-        if constexpr (sizeof(R) == 4)
-            for (int c = 0; c < 9; c++) J[c] = ldn<O32>(d.JcV32 + (size_t)c * nb, n);
-        else
-            for (int c = 0; c < 9; c++) J[c] = (R)ldn<O32>(d.JcV + (size_t)c * nb, n);
+        if constexpr (AFF) {
+            const double* em = d.emet + (size_t)e * 10;
+            const double cv64 = em[9] * d.wnode[q];
+            cvn = (R)cv64;
+            for (int c = 0; c < 9; c++) J[c] = (R)(em[c] * cv64);
+        } else {
+            if constexpr (sizeof(R) == 4) cvn = ldn<O32>(d.cV32, n);
+            else cvn = (R)ldn<O32>(d.cV, n);
+            if constexpr (sizeof(R) == 4)
+                for (int c = 0; c < 9; c++) J[c] = ldn<O32>(d.JcV32 + (size_t)c * nb, n);
+            else
+                for (int c = 0; c < 9; c++) J[c] = (R)ldn<O32>(d.JcV + (size_t)c * nb, n);
+        }
         if (q < NFN) prepare_face_task<R, S, O32, DIN>(d, a, e, q, NX, NY, NZ, ft);
         for (int v = 0; v < 5; v++) {
             pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
@@ -334,32 +345,40 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     }
 }
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN>
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF>
 __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z, false, DIN>(d, a);
+    rk_stage_body<R, S, X, Y, Z, false, DIN, AFF>(d, a);
 }
 
 // N = 7 (512 nodes, one element per work-group), node offsets in 32 bits,
 // registers capped at W waves per SIMD: measured on the 64x64x32 N=7
 // workload, FP32 runs fastest at 6 (three work-groups per CU), FP64 at 4
 // (two work-groups per CU).
-template <typename R, typename S, int W, bool DIN>
+template <typename R, typename S, int W, bool DIN, bool AFF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
 rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, 8, 8, 8, true, DIN>(d, a);
+    rk_stage_body<R, S, 8, 8, 8, true, DIN, AFF>(d, a);
 }
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN>
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF>
 static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
     constexpr int W = sizeof(R) == 4 ? 6 : 4;
     if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
         if (d.nb * sizeof(double) < (1ull << 32)) {
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN>), dim3(blocks), dim3(threads), lds, st, d, a);
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN, AFF>), dim3(blocks), dim3(threads), lds, st, d, a);
             return;
         }
     }
-    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN>), dim3(blocks), dim3(threads), lds, st, d, a);
+    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN, AFF>), dim3(blocks), dim3(threads), lds, st, d, a);
+}
+
+template <typename R, typename S, int X, int Y, int Z, bool DIN>
+static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
+                             hipStream_t st) {
+    // the tangent-linear instantiation keeps the per-node metric
+    if (d.affine && !is_dual<R>::value) launch_rk_kernel<R, S, X, Y, Z, DIN, !is_dual<R>::value>(d, a, blocks, threads, lds, st);
+    else launch_rk_kernel<R, S, X, Y, Z, DIN, false>(d, a, blocks, threads, lds, st);
 }
 
 template <typename R, typename S, int X, int Y, int Z>
@@ -376,8 +395,8 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
     const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
     if (lds > 65536) return hipErrorInvalidValue;
     a.epb = epb;
-    if (a.din[0]) launch_rk_kernel<R, S, X, Y, Z, true>(d, a, blocks, threads, lds, st);
-    else launch_rk_kernel<R, S, X, Y, Z, false>(d, a, blocks, threads, lds, st);
+    if (a.din[0]) launch_rk_metric<R, S, X, Y, Z, true>(d, a, blocks, threads, lds, st);
+    else launch_rk_metric<R, S, X, Y, Z, false>(d, a, blocks, threads, lds, st);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ BOX_CORNERS = lambda lx, ly, lz: [(0, 0, 0), (lx, 0, 0), (lx, 0, lz), (0, 0, lz)
 def _spec(case):
     lx, ly, lz = case["size"]
     nx, ny, nz = case["ndiv"]
-    pts = "\n".join("%r %r %r" % c for c in BOX_CORNERS(lx, ly, lz))
+    pts = "\n".join("%r %r %r" % tuple(c) for c in case.get("corners", BOX_CORNERS(lx, ly, lz)))
     spacing = case.get("spacing", "linear 3{%d %d %d}" % (nx, ny, nz))
     lines = ["8", "{", pts, "}", "8{0 1 5 4 3 2 6 7} " + spacing]
     for name, groups in case["patches"]:
@@ -160,6 +160,14 @@ EULER_CASES = [
          patches=PATCHES_3D, dt=0.004, check_steps=[2],
          T=["gaussian 0 0.7 300 300 200 250 250 250", "gaussian 0 0.3 700 600 300 200 200 200",
             "gaussian 0 1.0 500 450 250 150 150 150"]),
+    # non-affine elements: the top face tilted (z = 600 at x = 0, 700 at
+    # x = 1000), so every element is a trapezoid in x-z and Jinv varies
+    # inside it
+    dict(name="e3_trilinear_n3", ndiv=(3, 3, 2), size=(1000, 800, 600), order=(3, 3, 3),
+         corners=[(0, 0, 0), (1000, 0, 0), (1000, 0, 700), (0, 0, 600),
+                  (0, 800, 0), (1000, 800, 0), (1000, 800, 700), (0, 800, 600)],
+         patches=PATCHES_3D, dt=0.004, check_steps=[1, 3],
+         T=["gaussian 0 0.5 500 400 250 200 200 200"]),
     dict(name="e3_wall_n3", ndiv=(5, 4, 6), size=(1000, 800, 600), order=(3, 3, 3),
          spacing="wall 3{5 4 6} 3{2 3 0.5}", patches=PATCHES_3D, dt=0.002, check_steps=[2],
          T=["gaussian 0 0.5 500 400 300 200 200 200"]),

@@ -8,7 +8,21 @@ from test_gpu_dg_split import relerr, setup
 
 pytestmark = pytest.mark.gpu
 
-CASES = [c for c in EULER_CASES if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_n7", "e3_aniso_bubbles")]
+CASES = [c for c in EULER_CASES
+         if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_n7", "e3_aniso_bubbles", "e3_trilinear_n3")]
+
+
+@pytest.mark.parametrize("name,affine", [("cfg0_2d_n4", True), ("e3_n3", True), ("e3_wall_n3", False),
+                                         ("e3_trilinear_n3", False)])
+def test_metric_layout(oracle, tmp_path, name, affine):
+    """Uniform box elements are affine and take the per-element metric; the
+    tilted-top block's trapezoids and the wall-graded block (whose node
+    placement follows the grading inside each element: Jinv varies by 13%)
+    keep the per-node metric."""
+    case = [c for c in EULER_CASES if c["name"] == name][0]
+    mesh, om, prm, dg, state = setup(oracle, tmp_path, case)
+    assert dg.affine() == affine
+    dg.close()
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
