@@ -34,13 +34,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_per_node(s_in, s_base, s_out, s_geo, N, affine=False):
-    """Bytes per node per stage of the DG residual+update (the path's
-    algorithmic figure): n_q (s_in + s_base + s_out) + 10 s_geo + 2*8
-    + 3/(N+1) (3 s_geo + 8).  With affine elements the metric (JcV, cV:
-    10 s_geo) is one 80-byte record per element instead of per node."""
-    metric = 80.0 / (N + 1) ** 3 if affine else 10 * s_geo
-    return NQ * (s_in + s_base + s_out) + metric + 2 * 8 + 3.0 / (N + 1) * (3 * s_geo + 8)
+def algorithmic_bytes_per_node(s_inc, s_geo, N, affine=False):
+    """Compulsory HBM bytes per node of one SSP-RK3 stage launch, averaged
+    over the three stages (each array element read or written once):
+    FP64 base state 5 x 8 every stage; increment in 5 s_inc on stages 2 and 3;
+    output 5 s_inc (stages 1, 2) or the FP64 state 5 x 8 (stage 3);
+    background p_ref, rho_ref 2 x 8; metric JcV, cV 10 s_geo per node, or one
+    80-byte record per element with affine elements; per element the six
+    face entries (6 x 12 B) and its three owned facet normals (3 x 24 B).
+    Neighbour traces are re-reads of state counted once already."""
+    npe = (N + 1) ** 3
+    state = 5 * 8 + 5 * s_inc * 2 / 3 + (2 * 5 * s_inc + 5 * 8) / 3
+    metric = 80.0 / npe if affine else 10 * s_geo
+    return state + 2 * 8 + metric + (6 * 12 + 3 * 24) / npe
 
 
 # Box corner points and the block's hex vertex order over them (x, then y,
@@ -388,13 +394,18 @@ def main():
     nb = mesh.info.nbcs_field
     K, W = args.steps, args.warmup
     res = {}
+    xfer = {}
     for prec in (64, 32):
+        t0 = time.time()
         dg.set_state(rho, U, T, p)
+        xfer.setdefault("set_state_s", time.time() - t0)
         dg.rk3_steps(W, prec)
         dg.synchronize()
         t = dg.time_rk3(K, prec)
-        res[prec] = {"seconds": t, "rate": NQ * nb * 3 * K / t, "stage_s": t / (3 * K),
-                     "q": dg.get_conserved()}
+        t0 = time.time()
+        q = dg.get_conserved()
+        xfer.setdefault("get_conserved_s", time.time() - t0)
+        res[prec] = {"seconds": t, "rate": NQ * nb * 3 * K / t, "stage_s": t / (3 * K), "q": q}
         log(f"rk3 fp{prec}: {t * 1e3:.1f} ms for {K} steps, {res[prec]['rate'] / 1e9:.2f} G DOF*stage/s")
     err = {}
     for name, a, b in zip(("rho_p", "m", "Theta_p"), res[32]["q"], res[64]["q"]):
@@ -410,14 +421,14 @@ def main():
     dg.close()
     del res[64]["q"], res[32]["q"]
 
-    b32 = algorithmic_bytes_per_node(4, 8, 8, 4, N, affine)
-    b64 = algorithmic_bytes_per_node(8, 8, 8, 8, N, affine)
+    b32 = algorithmic_bytes_per_node(4, 4, N, affine)
+    b64 = algorithmic_bytes_per_node(8, 8, N, affine)
     achieved = b32 * nb / res[32]["stage_s"]
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK,
                 "traffic": profiled_traffic("traffic_bytes_per_launch_fp32") if ndiv == (64, 64, 32) else None,
                 "traffic_source": "profiles/rk_stage_64x64x32_n7_hbm_bytes.md (mean of the 3 stages)",
-                "kernel": "rk_stage_kernel_n7<float,float,6>",
+                "kernel": "rk_stage_kernel_n7<float,float,6,DIN,%s>" % ("true" if affine else "false"),
                 "bytes_per_node": b32, "nodes_per_launch": nb,
                 "metric_layout": "per element (affine)" if affine else "per node",
                 "fp64_kernel_GBps": b64 * nb / res[64]["stage_s"] / 1e9,
@@ -453,6 +464,11 @@ def main():
         "fp32_over_fp64": res[32]["rate"] / res[64]["rate"],
         "fp32_vs_fp64_rel_l2": err,
         "split_reference_step_fp64": {"value": split_rate, "ms_per_step": ts / ks * 1e3},
+        # host <-> device state transfers through the C-ABI (host arrays,
+        # pageable memory, layout conversion included), once per run
+        "host_transfer": dict(xfer, state_bytes=6 * 8 * nb,
+                              fp32_rate_incl_one_round_trip=NQ * nb * 3 * K / (
+                                  res[32]["seconds"] + xfer["set_state_s"] + xfer["get_conserved_s"])),
         "roofline": roofline,
         "pcg": fv,
         "adjoint": adj,

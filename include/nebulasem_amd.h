@@ -175,7 +175,7 @@ int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p)
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds);
 int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds);
 int nsem_dg_synchronize(nsem_dg* d);
-/* *all_affine = 1 when every element's Jacobian is constant (to 1e-12
+/* *all_affine = 1 when every element's Jacobian is constant (to 1e-10
  * relative), so the method-of-lines kernels read one metric per element
  * instead of one per node; 0 otherwise.  No reference counterpart (the
  * reference stores Jinv per node, field.cpp:171-384). */

@@ -182,7 +182,14 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         }
         rref = d.rho_ref[n];
         pref = d.p_ref[n];
-        for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * nb + n];
+        if (d.affine) {
+            // the forward stage's per-element metric (JcV = Jinv_e (vol_e W))
+            const double* em = d.emet + (size_t)e * 10;
+            const double cv64 = em[9] * d.wnode[q];
+            for (int c = 0; c < 9; c++) J[c] = em[c] * cv64;
+        } else {
+            for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * nb + n];
+        }
     }
     __syncthreads();
 

@@ -61,7 +61,7 @@ struct DgDev {
     int diffusion;
     double visc, iPr;
     const double* grad;
-    // affine elements (every element's Jinv constant to 1e-12 relative): the
+    // affine elements (every element's Jinv constant to 1e-10 relative): the
     // method-of-lines kernels form JcV = Jinv_e cV and cV = vol_e W(node)
     // from emet [nelem][10] (Jinv_e in tensor order, then the cell volume)
     // and wnode [NP] = w_i w_j w_k / 8 instead of reading the node arrays

@@ -155,10 +155,12 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     }
     });
     // per-element metric of affine elements: Jinv_e from the element's first
-    // node, accepted when every node's Jinv agrees to 1e-12 of its largest
-    // component (an affine map has a constant Jacobian; curved or trilinear
-    // elements fail); cV is the cell volume times w_i w_j w_k / 8 on every
-    // element (field.cpp:171-384 with dg.cpp:175-325)
+    // node, accepted when every node's Jinv agrees to 1e-10 of its largest
+    // component.  An affine map has a constant Jacobian, and the per-node
+    // values then differ by the round-off of their computation only (about
+    // 1e-12 at 10 km / 156 m elements); trapezoidal, graded or curved
+    // elements differ by 1e-2 and more.  cV is the cell volume times
+    // w_i w_j w_k / 8 on every element (field.cpp:171-384, dg.cpp:175-325).
     const int NPX = g.npx, NPY = g.npy, NPZ = g.npz, NPn = g.np;
     std::vector<double> wnode(NPn);
     for (int i = 0; i < NPX; i++)
@@ -178,7 +180,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         emet[c * 10 + 9] = lm.cv[c];
         for (int t = 0; t < NPn && aff[c]; t++) {
             for (int q = 0; q < 9; q++)
-                if (!(std::fabs(g.Jinv[n0 + t][q] - g.Jinv[n0][q]) <= 1e-12 * scale)) aff[c] = 0;
+                if (!(std::fabs(g.Jinv[n0 + t][q] - g.Jinv[n0][q]) <= 1e-10 * scale)) aff[c] = 0;
             if (g.cV[n0 + t] != lm.cv[c] * wnode[t]) aff[c] = 0;
         }
     }

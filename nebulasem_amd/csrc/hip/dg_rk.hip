@@ -376,8 +376,7 @@ static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks
 template <typename R, typename S, int X, int Y, int Z, bool DIN>
 static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
-    // the tangent-linear instantiation keeps the per-node metric
-    if (d.affine && !is_dual<R>::value) launch_rk_kernel<R, S, X, Y, Z, DIN, !is_dual<R>::value>(d, a, blocks, threads, lds, st);
+    if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true>(d, a, blocks, threads, lds, st);
     else launch_rk_kernel<R, S, X, Y, Z, DIN, false>(d, a, blocks, threads, lds, st);
 }
 
