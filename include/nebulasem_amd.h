@@ -219,6 +219,12 @@ int nsem_fv_apply(nsem_fv* fv, const double* x, double* y);
  * Returns NSEM_ENOTCONVERGED when max_it is reached first. */
 int nsem_fv_solve(nsem_fv* fv, double* x, int32_t mode, double tol, int32_t max_it,
                   nsem_fv_result* res);
+/* Velocity correction after the solve, U -= gradf(p, true) on the interior
+ * cells (apps/potential/potential.cpp:66-67; FV gradf field.h:3363-3371):
+ * p interior cells (ghosts from the pressure boundary conditions given to
+ * nsem_fv_create, as Solve's applyExplicitBCs sets them), U interior cells
+ * xyz, updated in place. */
+int nsem_fv_correct_velocity(nsem_fv* fv, const double* p, double* U);
 
 /* ----------------------------------------------------------- measurement --
  * Device-to-device copy of `bytes` (a multiple of 16) through a plain

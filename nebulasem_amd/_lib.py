@@ -351,6 +351,14 @@ class FVPoisson:
         check(lib().nsem_fv_apply(self._h, _pd(x), _pd(y)))
         return y
 
+    def correct_velocity(self, p, U):
+        """U - gradf(p, true) on the interior cells (nsem_fv_correct_velocity);
+        U: (n, 3), returned as a new array."""
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)[:self.n]
+        out = np.array(U, dtype=np.float64).reshape(-1)[:3 * self.n].copy()
+        check(lib().nsem_fv_correct_velocity(self._h, _pd(p), _pd(out)))
+        return out.reshape(-1, 3)
+
     def solve(self, x0=None, mode=0, tol=1e-10, max_it=100000, raise_on_error=True):
         x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
         res = FvResult()

@@ -63,6 +63,9 @@ struct nsem_fv {
     Buf su_bc, su;                       // boundary part of the source, full source
     // divergence inputs
     Buf fi0, fi1, allFaces, FO, FN, fN, fI, patch_of_ghost, ubc_kind, ubc_value, U;
+    // velocity correction: pressure ghost rule per ghost cell, cell volumes
+    Buf pg_kind, pg_off, cV;
+    uint64_t nghost = 0;
     FvDivInput div{};
     // CG work
     Buf x, r, z, p, Ap, x32, r32, z32, p32, Ap32, e32, b32, y64, slots, sc, tmp;
@@ -114,7 +117,9 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
             if (c1 == i) ap[c1] += ano[k];
             else if (c2 < nc) ap[c2] += ann[k];
         }
-    std::vector<int32_t> patch_of_ghost(nc - n, 0);
+    d->nghost = nc - n;
+    std::vector<int32_t> patch_of_ghost(nc - n, 0), pg_kind(nc - n, 0);
+    std::vector<double> pg_off(nc - n, 0.0);
     {
         size_t b = 0;
         for (const auto& kv : lm.boundaries) {
@@ -126,8 +131,12 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
                     const V3 dv(g.cC[c2][0] - g.cC[c1][0], g.cC[c2][1] - g.cC[c1][1], g.cC[c2][2] - g.cC[c1][2]);
                     ap[c1] -= ann[k];
                     subc[c1] += ann[k] * (bc_value[b] * std::sqrt(dot3(dv, dv)));
+                    pg_kind[c2 - n] = 0;
+                    pg_off[c2 - n] = bc_value[b] * std::sqrt(dot3(dv, dv));
                 } else {
                     subc[c1] += ann[k] * bc_value[b];
+                    pg_kind[c2 - n] = 2;
+                    pg_off[c2 - n] = bc_value[b];
                 }
                 ann[k] = 0;
             }
@@ -251,7 +260,9 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
     for (uint64_t k = 0; k < nf; k++) for (int c = 0; c < 3; c++) fnv[3 * k + c] = g.fN[k][c];
     if ((e = put(d->fi0, g.faceIndices0, s)) || (e = put(d->fi1, g.faceIndices1, s)) ||
         (e = put(d->allFaces, g.allFaces, s)) || (e = put(d->FO, g.FO, s)) || (e = put(d->FN, g.FN, s)) ||
-        (e = put(d->fN, fnv, s)) || (e = put(d->fI, g.fI, s)) || (e = put(d->patch_of_ghost, patch_of_ghost, s)))
+        (e = put(d->fN, fnv, s)) || (e = put(d->fI, g.fI, s)) || (e = put(d->patch_of_ghost, patch_of_ghost, s)) ||
+        (e = put(d->pg_kind, pg_kind, s)) || (e = put(d->pg_off, pg_off, s)) ||
+        (e = put(d->cV, std::vector<double>(g.cV.begin(), g.cV.begin() + n), s)))
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     HIP_TRY(d->ubc_kind.alloc(names.size() * sizeof(int32_t) + 8));
     HIP_TRY(d->ubc_value.alloc(names.size() * 3 * sizeof(double) + 8));
@@ -310,6 +321,18 @@ int nsem_fv_get_rhs(nsem_fv* d, double* su) {
     if (!d || !su) return nsem_fail(NSEM_EINVAL, "null argument");
     HIP_TRY(hipMemcpyAsync(su, d->su.p, d->n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
+    return NSEM_OK;
+}
+
+int nsem_fv_correct_velocity(nsem_fv* d, const double* p, double* U) {
+    if (!d || !p || !U) return nsem_fail(NSEM_EINVAL, "null argument");
+    hipStream_t s = d->stream;
+    HIP_TRY(hipMemcpyAsync(d->y64.p, p, d->n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d->U.p, U, 3 * d->n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(fv_gradcorr(d->div, d->y64.as<double>(), d->pg_kind.as<int32_t>(), d->pg_off.as<double>(),
+                        d->cV.as<double>(), d->nghost, d->U.as<double>(), s));
+    HIP_TRY(hipMemcpyAsync(U, d->U.p, 3 * d->n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return NSEM_OK;
 }
 

@@ -53,6 +53,9 @@ hipError_t fv_residual64(const FvOp<double>& A, const double* b, const double* x
                          double* out, double* slots, hipStream_t s);
 hipError_t fv_correct(uint64_t n, double* x, const float* e, hipStream_t s);
 hipError_t fv_divu(const FvDivInput& in, double* su, hipStream_t s);
+// U -= gradf(p) on the interior cells; gkind/goff per ghost cell (nghost)
+hipError_t fv_gradcorr(const FvDivInput& in, const double* p, const int32_t* gkind, const double* goff,
+                       const double* cV, uint64_t nghost, double* U, hipStream_t s);
 template <typename R>
 hipError_t fv_convert(uint64_t n, const double* a, R* b, hipStream_t s);
 template <typename R>

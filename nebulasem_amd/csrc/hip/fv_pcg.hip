@@ -387,6 +387,44 @@ __global__ void __launch_bounds__(kThreads) divu_kernel(FvDivInput in, double* _
     }
 }
 
+// U -= gradf(p, true) on the interior cells (potential.cpp:66-67; the FV
+// branch of gradf, field.h:3363-3371, with grad_flux<false>,
+// field.h:3050-3082): cds(p) at each face of the cell, + fN cds on the faces
+// it owns and - on the others, in the cell's face order, then / cV.  Ghost p
+// from the pressure boundary condition as applyExplicitBCs sets it after the
+// solve: DIRICHLET the patch value, NEUMANN the owner plus value |dv|
+// (goff holds the value, or value |dv|, per ghost cell).
+__global__ void __launch_bounds__(kThreads) gradcorr_kernel(FvDivInput in, const double* __restrict__ p,
+                                                           const int32_t* __restrict__ gkind,
+                                                           const double* __restrict__ goff,
+                                                           const double* __restrict__ cV, uint64_t nghost,
+                                                           double* __restrict__ U) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x; c < in.ncells; c += stride) {
+        double g0 = 0, g1 = 0, g2 = 0;
+        for (uint32_t f = in.fi0[c]; f < in.fi1[c]; f++) {
+            const uint32_t k = in.allFaces[f];
+            const uint32_t c1 = in.FO[k], c2 = in.FN[k];
+            const double* nv = &in.fN[3 * (size_t)k];
+            const double fi = in.fI[k];
+            const double p1 = p[c1];
+            double p2;
+            if (c2 < in.ncells) p2 = p[c2];
+            else {
+                const uint64_t gi = c2 - in.ncells;
+                p2 = gi >= nghost ? p1 : (gkind[gi] == 0 ? p1 + goff[gi] : goff[gi]);
+            }
+            const double fF = p1 * fi + p2 * (1 - fi);
+            if (c1 == c) { g0 += nv[0] * fF; g1 += nv[1] * fF; g2 += nv[2] * fF; }
+            else { g0 -= nv[0] * fF; g1 -= nv[1] * fF; g2 -= nv[2] * fF; }
+        }
+        const double v = cV[c];
+        U[3 * c] = U[3 * c] - g0 / v;
+        U[3 * c + 1] = U[3 * c + 1] - g1 / v;
+        U[3 * c + 2] = U[3 * c + 2] - g2 / v;
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kThreads) convert_kernel(uint64_t n, const double* __restrict__ a,
                                                           R* __restrict__ b) {
@@ -454,6 +492,13 @@ hipError_t fv_residual64(const FvOp<double>& A, const double* b, const double* x
 
 hipError_t fv_correct(uint64_t n, double* x, const float* e, hipStream_t s) {
     hipLaunchKernelGGL(correct_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, e);
+    return hipGetLastError();
+}
+
+hipError_t fv_gradcorr(const FvDivInput& in, const double* p, const int32_t* gkind, const double* goff,
+                       const double* cV, uint64_t nghost, double* U, hipStream_t s) {
+    hipLaunchKernelGGL(gradcorr_kernel, dim3(grid_for(in.ncells)), dim3(kThreads), 0, s, in, p, gkind, goff, cV,
+                       nghost, U);
     return hipGetLastError();
 }
 

@@ -618,6 +618,31 @@ int om_fv_poisson(const om_mesh* m, const int32_t* bc_p, const double* val_p, co
     return status;
 }
 
+/* Velocity correction of the potential app, U -= gradf(p, true)
+ * (potential.cpp:66-67) on the interior cells: the FV branch of gradf
+ * (field.h:3363-3371) is cds(p) at every face, grad_flux<false>
+ * (field.h:3050-3082: + fN cds(p) on the cell's owned faces, - on the others,
+ * in the cell's face order), divided by cV.  p has naf entries: its ghosts are
+ * first set by p's boundary conditions, as Solve's closing applyExplicitBCs
+ * does (solve.cpp:580).  U: naf*3, interior entries updated. */
+void om_fv_correct_velocity(const om_mesh* m, const int32_t* bc_p, const double* val_p, double* p, double* U) {
+    bc_scalar(m, bc_p, val_p, p);
+    for (uint32_t i = 0; i < m->nbf; i++) {
+        double g[3] = {0, 0, 0};
+        for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++) {
+            uint64_t k = m->allFaces[f];
+            uint32_t c1 = m->FO[k], c2 = m->FN[k];
+            const double fF = p[c1] * m->fI[k] + p[c2] * (1 - m->fI[k]);
+            const double* nv = &m->fN[3 * k];
+            for (int a = 0; a < 3; a++) {
+                if (c1 == i) g[a] += nv[a] * fF;
+                else g[a] -= nv[a] * fF;
+            }
+        }
+        for (int a = 0; a < 3; a++) U[3 * (uint64_t)i + a] = U[3 * (uint64_t)i + a] - g[a] / m->cV[i];
+    }
+}
+
 /* ---------------------------------------------- method-of-lines (new) -- */
 /* No reference counterpart: the reference's explicit schemes are forward
  * Euler on the split update.  This composes the pinned operators above into

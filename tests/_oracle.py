@@ -174,6 +174,14 @@ def fv_poisson(om, bc_kind, bc_val, U, p, tol, max_it):
     return st, it.value, r0.value, r.value
 
 
+def fv_correct_velocity(om, bc_kind, bc_val, p, U):
+    """U -= gradf(p, true) on the interior cells (p: all cells, ghosts set
+    from bc_kind/bc_val first; U: all cells xyz, updated in place)."""
+    bc_kind = np.ascontiguousarray(bc_kind, dtype=np.int32)
+    bc_val = np.ascontiguousarray(bc_val, dtype=np.float64)
+    lib().om_fv_correct_velocity(C.byref(om.s), _p(bc_kind, PI), _p(bc_val, PD), _p(p, PD), _p(U, PD))
+
+
 def euler_ssprk3_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps):
     return lib().om_euler_ssprk3_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
                                        _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))

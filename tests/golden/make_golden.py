@@ -117,11 +117,15 @@ def main():
             it = int(toks[[t.endswith("Iterations") for t in toks].index(True) + 1])
             res = float(toks[toks.index("Final") + 2])
             p = read_field_bin(os.path.join(d, "p1.bin")).reshape(-1)
+            # the corrected velocity U -= gradf(p) (potential.cpp:66-67)
+            u = read_field_bin(os.path.join(d, "U1.bin")).reshape(-1)
             fixtures["potential"][case["name"]] = {
                 "iterations": it, "final_residual": res, "log_line": line.split("]", 1)[-1].strip(),
-                "p_sha256": sha(p), "p_l2": float(np.linalg.norm(p)), "n": int(p.size)}
+                "p_sha256": sha(p), "p_l2": float(np.linalg.norm(p)), "n": int(p.size),
+                "U_sha256": sha(u), "U_l2": float(np.linalg.norm(u)), "U_n": int(u.size)}
             if case.get("keep_fields"):
                 arrays[f"potential_{case['name']}_p"] = p
+                arrays[f"potential_{case['name']}_U"] = u
             print("potential", case["name"], it, res)
     finally:
         shutil.rmtree(tmp)

@@ -28,6 +28,17 @@ def test_fv_reference_mode_matches_oracle(oracle, tmp_path, case):
     assert abs(res.iterations - it) <= 1
     err = np.linalg.norm(x - p_or[:n]) / np.linalg.norm(p_or[:n])
     assert err <= 1e-9, err
+    # velocity correction U -= gradf(p): bit-identical to the oracle on the
+    # same p (the oracle's matches the reference's output bit for bit)
+    u_or = U.reshape(-1).copy()
+    oracle.fv_correct_velocity(om, kp, vp.reshape(-1), p_or.copy(), u_or)
+    u_gpu = fv.correct_velocity(p_or[:n], U[:n])
+    assert np.array_equal(u_gpu.reshape(-1), u_or[:3 * n])
+    # and on the device's own solution, within its 1e-9
+    u_own = fv.correct_velocity(x, U[:n])
+    du = np.linalg.norm(u_own.reshape(-1) - u_or[:3 * n]) / np.linalg.norm(u_or[:3 * n])
+    print(f"  corrected U: bit-exact on the oracle's p; {du:.1e} on the device's own p", flush=True)
+    assert du <= 1e-8
     fv.close()
 
 

@@ -80,6 +80,42 @@ def test_oracle_poisson_matches_reference(oracle, tmp_path, case):
     assert it == ref["iterations"]
     assert abs(r - ref["final_residual"]) <= 1e-5 * ref["final_residual"]
     assert sha(p[:mesh.info.nbcs]) == ref["p_sha256"]
+    # the velocity correction U -= gradf(p, true) the potential app closes with
+    oracle.fv_correct_velocity(om, kp, vp.reshape(-1), p, U.reshape(-1))
+    assert sha(U.reshape(-1)[:3 * mesh.info.nbcs]) == ref["U_sha256"]
+
+
+def test_fv_gradient_closed_forms(oracle, tmp_path):
+    """The FV Gauss gradient is exact for a linear p on cells whose faces are
+    all interior (uniform grid: cds is the face-centre value), and it
+    telescopes: sum cV grad p = sum over boundary faces of fN cds(p)."""
+    case = dict(name="grad_lin", ndiv=(6, 5, 4), seed=5)
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, case)
+    om = oracle.OracleMesh(mesh)
+    n = mesh.info.nbcs
+    cC = mesh.array("cC").reshape(-1, 3)
+    a = np.array([0.7, -1.3, 2.1])
+    p = (cC @ a).copy()
+    kn = np.zeros_like(kp)   # all NEUMANN: ghosts copy their owners
+    vn = np.zeros(len(kp))
+    U0 = np.zeros((len(p), 3))
+    U1 = U0.copy().reshape(-1)
+    oracle.fv_correct_velocity(om, kn, vn, p.copy(), U1)
+    grad = -U1.reshape(-1, 3)[:n]
+    FO, FN, fI = mesh.array("FO"), mesh.array("FN"), mesh.array("fI")
+    fi0, fi1, af = mesh.array("faceIndices0"), mesh.array("faceIndices1"), mesh.array("allFaces")
+    inner = [c for c in range(n) if all(FN[af[f]] < n for f in range(fi0[c], fi1[c]))]
+    assert len(inner) > 0
+    assert np.abs(grad[inner] - a).max() <= 1e-12 * np.abs(a).max() * 10
+    # telescoping with the ghost values the correction used (owner copies)
+    pg = p.copy()
+    pg[FN[FN >= n]] = pg[FO[FN >= n]]
+    fN = mesh.array("fN").reshape(-1, 3)
+    cV = mesh.array("cV")
+    bnd = np.where(FN >= n)[0]
+    lhs = (cV[:n, None] * grad).sum(axis=0)
+    rhs = (fN[bnd] * (pg[FO[bnd]] * fI[bnd] + pg[FN[bnd]] * (1 - fI[bnd]))[:, None]).sum(axis=0)
+    assert np.abs(lhs - rhs).max() <= 1e-9 * np.abs(rhs).max()
 
 
 def test_poisson_against_dense_solve(oracle, tmp_path):
