@@ -203,6 +203,10 @@ typedef struct {
 int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_kind,
                    const double* bc_value, nsem_fv** out);
 void nsem_fv_destroy(nsem_fv* fv);
+/* *structured: 0 general mesh (six stored couplings per cell), 1 structured
+ * block numbering (three stored coupling arrays), 2 structured with every
+ * coupling of a direction bitwise equal (uniform grid and coefficient: the
+ * SpMV reads no coupling array). */
 int nsem_fv_info(const nsem_fv* fv, uint64_t* ncells, int32_t* structured);
 /* Source = BC part - divf(U) (field.h:3470-3477): U interior cells xyz; the
  * velocity boundary conditions per patch (NEUMANN, SYMMETRY or DIRICHLET). */

@@ -327,6 +327,7 @@ class FVPoisson:
         check(lib().nsem_fv_info(self._h, C.byref(n), C.byref(st)))
         self.n = n.value
         self.structured = bool(st.value)
+        self.uniform = st.value == 2
 
     def set_rhs_divergence(self, U, bc_kind, bc_value):
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)

@@ -248,6 +248,25 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
         A.nx = B.nx = (int)nx; A.ny = B.ny = (int)ny; A.nz = B.nz = (int)nz;
         A.sx = B.sx = ny * nz; A.sy = B.sy = nz; A.sz = B.sz = 1;
         A.cpl = d->cpl.as<double>(); B.cpl = d->cpl32.as<float>();
+        // uniform couplings: every interior adjacency of a direction carries
+        // the same value, bit for bit (a uniform grid, constant mu)
+        bool uni = true;
+        double cu[3] = {0, 0, 0};
+        bool seen[3] = {false, false, false};
+        const int64_t st[3] = {ny * nz, nz, 1};
+        for (uint64_t k = 0; k < nf && uni; k++) {
+            const Int c1 = g.FO[k], c2 = g.FN[k];
+            if (c2 >= n) continue;
+            const int64_t lo = std::min<int64_t>(c1, c2), dlt = std::max<int64_t>(c1, c2) - lo;
+            const int dir = dlt == st[0] ? 0 : dlt == st[1] ? 1 : 2;
+            const double v = cpl[(size_t)dir * n + lo];
+            if (!seen[dir]) { cu[dir] = v; seen[dir] = true; }
+            else if (std::memcmp(&v, &cu[dir], sizeof(double)) != 0) uni = false;
+        }
+        if (uni) {
+            A.uniform = B.uniform = 1;
+            for (int q = 0; q < 3; q++) { A.cu[q] = cu[q]; B.cu[q] = (float)cu[q]; }
+        }
     } else {
         std::vector<float> o32(off.begin(), off.end());
         if ((e = put(d->nbr, nbr, s)) || (e = put(d->off, off, s)) || (e = put(d->off32, o32, s)))
@@ -292,7 +311,8 @@ void nsem_fv_destroy(nsem_fv* d) { delete d; }
 int nsem_fv_info(const nsem_fv* d, uint64_t* ncells, int32_t* structured) {
     if (!d || !ncells || !structured) return nsem_fail(NSEM_EINVAL, "null argument");
     *ncells = d->n;
-    *structured = d->A64.structured;
+    // 0 general, 1 structured block numbering, 2 structured with uniform couplings
+    *structured = d->A64.structured ? (d->A64.uniform ? 2 : 1) : 0;
     return NSEM_OK;
 }
 

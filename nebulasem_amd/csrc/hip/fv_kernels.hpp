@@ -18,6 +18,10 @@ struct FvOp {
     const R* cpl = nullptr;    // [3][n] structured couplings (0 at walls)
     const uint32_t* nbr = nullptr;  // [6][n] neighbour or 0xffffffff (unstructured)
     const R* off = nullptr;         // [6][n]
+    // structured with every x (y, z) coupling bitwise equal (a uniform grid
+    // and coefficient): the marching SpMV uses cu[d] instead of cpl[d][.]
+    int uniform = 0;
+    R cu[3] = {0, 0, 0};
 };
 
 // Device-resident CG scalars.

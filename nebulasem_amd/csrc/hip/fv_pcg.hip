@@ -139,7 +139,9 @@ inline MarchGrid march_grid(int nx, int ny, int nz) {
     return m;
 }
 
-template <typename R>
+// UNI: the couplings are the three constants A.cu (bitwise equal to every
+// stored coupling), so only p, ap and Ap move.
+template <typename R, bool UNI>
 __global__ void __launch_bounds__(kThreads) spmv_dot_march_kernel(FvOp<R> A, MarchGrid mg, const R* __restrict__ p,
                                                                  R* __restrict__ Ap, const CgScalars* sc,
                                                                  double* slots) {
@@ -157,19 +159,19 @@ __global__ void __launch_bounds__(kThreads) spmv_dot_march_kernel(FvOp<R> A, Mar
             const R* c1 = A.cpl + A.n;
             const R* c2 = A.cpl + 2 * A.n;
             uint64_t n = (uint64_t)i0 * sx + col;
-            R pm = i0 > 0 ? p[n - sx] : R(0), c0m = i0 > 0 ? c0[n - sx] : R(0);
+            R pm = i0 > 0 ? p[n - sx] : R(0), c0m = UNI ? A.cu[0] : (i0 > 0 ? c0[n - sx] : R(0));
             R pc = p[n];
             for (int i = i0; i < i1; i++, n += sx) {
                 const bool hx = i + 1 < A.nx;
                 const R pn = hx ? p[n + sx] : R(0);
-                const R c0c = c0[n];
+                const R c0c = UNI ? A.cu[0] : c0[n];
                 R acc = pc * A.diag[n];
                 if (hx) acc -= pn * c0c;
                 if (i > 0) acc -= pm * c0m;
-                if (j + 1 < A.ny) acc -= p[n + A.nz] * c1[n];
-                if (j > 0) acc -= p[n - A.nz] * c1[n - A.nz];
-                if (k + 1 < A.nz) acc -= p[n + 1] * c2[n];
-                if (k > 0) acc -= p[n - 1] * c2[n - 1];
+                if (j + 1 < A.ny) acc -= p[n + A.nz] * (UNI ? A.cu[1] : c1[n]);
+                if (j > 0) acc -= p[n - A.nz] * (UNI ? A.cu[1] : c1[n - A.nz]);
+                if (k + 1 < A.nz) acc -= p[n + 1] * (UNI ? A.cu[2] : c2[n]);
+                if (k > 0) acc -= p[n - 1] * (UNI ? A.cu[2] : c2[n - 1]);
                 Ap[n] = acc;
                 v[0] += (double)pc * (double)acc;
                 pm = pc; pc = pn; c0m = c0c;
@@ -460,8 +462,12 @@ hipError_t fv_cg_iterations(const FvOp<R>& A, int count, R* x, R* r, R* z, R* p,
     const bool march = A.structured && mg.groups <= (unsigned)kCgSlots;
     for (int it = 0; it < count; it++) {
         if (march) {
-            hipLaunchKernelGGL(spmv_dot_march_kernel<R>, dim3(mg.groups), dim3(kThreads), 0, s, A, mg, p, Ap, sc,
-                               slots);
+            if (A.uniform)
+                hipLaunchKernelGGL((spmv_dot_march_kernel<R, true>), dim3(mg.groups), dim3(kThreads), 0, s, A, mg, p,
+                                   Ap, sc, slots);
+            else
+                hipLaunchKernelGGL((spmv_dot_march_kernel<R, false>), dim3(mg.groups), dim3(kThreads), 0, s, A, mg,
+                                   p, Ap, sc, slots);
             hipLaunchKernelGGL(alpha_kernel, dim3(1), dim3(kThreads), 0, s, sc, slots, mg.groups);
         } else {
             hipLaunchKernelGGL(spmv_dot_kernel<R>, dim3(g), dim3(kThreads), 0, s, A, p, Ap, sc, slots);
