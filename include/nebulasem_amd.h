@@ -96,6 +96,14 @@ void nsem_mesh_destroy(nsem_mesh* m);
 enum { NSEM_BC_NEUMANN = 0, NSEM_BC_SYMMETRY = 1, NSEM_BC_DIRICHLET = 2 };
 int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
                     double* values, int32_t* bc_kind, double* bc_value);
+/* Write a field file as MeshField::write_ does (field.h:1588-1663, boundary
+ * blocks as BCondition's operator<<, field.h:187-216): ".bin" paths in the
+ * binary token stream (util.h:191-222), others as text with 12 significant
+ * digits.  values: nbcs_field*ncomp interior values (xyz per node for ncomp
+ * 3); one boundary block per patch whose bc_kind is not -1 (NSEM_BC_*),
+ * value from bc_value[ncomp*patch], written when its magnitude exceeds 1e-7. */
+int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
+                     const int32_t* bc_kind, const double* bc_value);
 
 /* -------------------------------------------------------------- DG Euler --
  * Device-resident dGSEM compressible Euler problem on one GPU (gfx950).

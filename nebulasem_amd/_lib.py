@@ -147,6 +147,17 @@ def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
     return vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp)
 
 
+def write_field(mesh, path, values, bc_kind, bc_value):
+    """Write a reference field file (nsem_field_write): values (nbcs_field, ncomp)
+    interior values; bc_kind (nb,), bc_value (nb, ncomp) per mesh patch."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    ncomp = 1 if v.ndim == 1 else v.shape[1]
+    k = np.ascontiguousarray(bc_kind, dtype=np.int32).reshape(-1)
+    bv = np.ascontiguousarray(bc_value, dtype=np.float64).reshape(-1)
+    check(lib().nsem_field_write(mesh.handle, path.encode(), ncomp, v.ctypes.data_as(C.c_void_p),
+                                 k.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p)))
+
+
 class EulerParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
                 ("cv", C.c_double), ("gravity", C.c_double * 3), ("buoyancy", C.c_int32),

@@ -42,3 +42,26 @@ extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t nco
     }
     return NSEM_OK;
 }
+
+extern "C" int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
+                                const int32_t* bc_kind, const double* bc_value) {
+    if (!m || !path || !values || !bc_kind || !bc_value) return nsem_fail(NSEM_EINVAL, "null argument");
+    if (ncomp != 1 && ncomp != 3) return nsem_fail(NSEM_EINVAL, "ncomp must be 1 or 3");
+    const nsem::Geometry& g = nsem_mesh_geometry(m);
+    const std::vector<std::string>& names = nsem_mesh_boundary_names(m);
+    std::vector<nsem::BoundaryCondition> bcs;
+    for (size_t b = 0; b < names.size(); b++) {
+        if (bc_kind[b] < 0) continue;
+        nsem::BoundaryCondition bc;
+        bc.patch = names[b];
+        if (bc_kind[b] == NSEM_BC_NEUMANN) bc.type = "NEUMANN";
+        else if (bc_kind[b] == NSEM_BC_SYMMETRY) bc.type = "SYMMETRY";
+        else if (bc_kind[b] == NSEM_BC_DIRICHLET) bc.type = "DIRICHLET";
+        else return nsem_fail(NSEM_EINVAL, "unknown boundary kind for " + names[b]);
+        bc.value.assign(bc_value + b * ncomp, bc_value + (b + 1) * ncomp);
+        bcs.push_back(bc);
+    }
+    std::string err;
+    if (!nsem::write_field_file(path, ncomp, g.nbcs_field, values, bcs, &err)) return nsem_fail(NSEM_EIO, err);
+    return NSEM_OK;
+}

@@ -56,12 +56,15 @@ __device__ __forceinline__ T ldn(const T* base, uint64_t i) {
     else
         return base[i];
 }
+// Stores of the stage output bypass the caches' retention (non-temporal):
+// they are not read again in this launch, and the L2 keeps the neighbour
+// traces instead.
 template <bool O32, typename T>
 __device__ __forceinline__ void stn(T* base, uint64_t i, T v) {
     if constexpr (O32)
-        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off(i, sizeof(T))) = v;
+        __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off(i, sizeof(T))));
     else
-        base[i] = v;
+        __builtin_nontemporal_store(v, base + i);
 }
 
 // Increment of variable v at node n (DIN: the stage has one; a compile-time

@@ -7,6 +7,7 @@
 
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 
@@ -199,6 +200,87 @@ bool read_field_file(const std::string& path, const FieldContext& ctx, int ncomp
         out.bcs.push_back(bc);
     }
     return true;
+}
+
+namespace {
+
+// Token sink of either format.  The binary stream (Util::ofstream_bin,
+// util.h:191-222) writes a string as a length byte and its characters with
+// all whitespace removed (nothing when that leaves it empty), an Int as 4
+// bytes and a Scalar as 8, little-endian; the text stream writes them as they
+// are, numbers as a C++ stream of precision 12 does (%.12g).
+struct Sink {
+    bool bin = false;
+    std::string out;
+    void str(const std::string& s) {
+        if (!bin) { out += s; return; }
+        std::string t;
+        for (char c : s)
+            if (!std::isspace((unsigned char)c)) t += c;
+        if (t.empty()) return;
+        out += (char)(unsigned char)t.size();
+        out += t;
+    }
+    void u32(uint32_t v) {
+        if (!bin) { out += std::to_string(v); return; }
+        out.append(reinterpret_cast<const char*>(&v), 4);
+    }
+    void f64(double v) {
+        if (!bin) {
+            char buf[64];
+            std::snprintf(buf, sizeof(buf), "%.12g", v);
+            out += buf;
+            return;
+        }
+        out.append(reinterpret_cast<const char*>(&v), 8);
+    }
+};
+
+// A field component block: a Scalar as "v", a Vector as "x y z " (the
+// tensor writer puts a space after every component, tensor.h:386-390).
+void put_value(Sink& o, const double* v, int ncomp) {
+    if (ncomp == 1) { o.f64(v[0]); return; }
+    for (int c = 0; c < ncomp; c++) { o.f64(v[c]); o.str(" "); }
+}
+
+}  // namespace
+
+bool write_field_file(const std::string& path, int ncomp, uint64_t n, const double* values,
+                      const std::vector<BoundaryCondition>& bcs, std::string* err) {
+    Sink o;
+    o.bin = path.size() > 4 && path.compare(path.size() - 4, 4, ".bin") == 0;
+    o.str("size "); o.u32((uint32_t)ncomp); o.str("\n");
+    o.str("internal "); o.u32((uint32_t)n); o.str("\n{\n");
+    for (uint64_t i = 0; i < n; i++) {
+        put_value(o, values + i * ncomp, ncomp);
+        o.str("\n");
+    }
+    o.str("}\n");
+    o.str("boundary "); o.u32((uint32_t)bcs.size()); o.str("\n{\n");
+    for (const BoundaryCondition& bc : bcs) {
+        o.str(bc.patch); o.str("\n{\n");
+        o.str("\ttype "); o.str(bc.type); o.str("\n");
+        // written unless equal(mag(value), 0) (tensor.h:471-476, EqualEpsilon 1e-7)
+        double m2 = 0;
+        for (int c = 0; c < ncomp; c++) m2 += bc.value[c] * bc.value[c];
+        if (!(std::sqrt(m2) <= 1e-7)) {
+            o.str("\tvalue "); put_value(o, bc.value.data(), ncomp); o.str("\n");
+        }
+        if (!(std::fabs(bc.shape) <= 1e-7)) {
+            o.str("\tshape "); o.f64(bc.shape); o.str("\n");
+        }
+        o.str("}\n");
+    }
+    o.str("}\n");
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        if (err) *err = "cannot write " + path;
+        return false;
+    }
+    const bool ok = std::fwrite(o.out.data(), 1, o.out.size(), f) == o.out.size();
+    std::fclose(f);
+    if (!ok && err) *err = "short write to " + path;
+    return ok;
 }
 
 }  // namespace nsem

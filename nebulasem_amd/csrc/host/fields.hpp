@@ -28,4 +28,11 @@ struct FieldContext {
 bool read_field_file(const std::string& path, const FieldContext& ctx, int ncomp, FieldData& out,
                      std::string* err);
 
+// Writer of the same formats (MeshField::write_, field.h:1588-1663, with
+// BCondition's operator<<, field.h:187-216): ".bin" paths get the binary
+// token stream, others the text form with 12 significant digits.  values:
+// n interior entries of ncomp; one boundary block per condition in bcs.
+bool write_field_file(const std::string& path, int ncomp, uint64_t n, const double* values,
+                      const std::vector<BoundaryCondition>& bcs, std::string* err);
+
 }  // namespace nsem

@@ -52,6 +52,7 @@ def euler_controls(case):
     npy                          %d
     npz                          %d
     viscosity                    %r
+    write_format                 %s
 }
 euler
 {
@@ -59,7 +60,8 @@ euler
     buoyancy                     YES
 }
 """ % (max(case.get("check_steps", [1])), case.get("dt", 0.005), o[0], o[1], o[2],
-       case.get("viscosity", 1.568e-5), "NO" if case.get("viscosity") is None else "YES")
+       case.get("viscosity", 1.568e-5), case.get("write_format", "BINARY"),
+       "NO" if case.get("viscosity") is None else "YES")
 
 
 def field_file(ncomp, gens, bcs):

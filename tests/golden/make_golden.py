@@ -31,6 +31,7 @@ from _cases import EULER_CASES, GEOM_CASES, POTENTIAL_CASES, write_euler_case, w
 from _fieldio import read_field_bin  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
+FIELDS = os.path.join(HERE, "fields")
 GEOM_SKIP = {"header", "bnames"}
 
 
@@ -106,6 +107,25 @@ def main():
                         arrays[f"euler_{case['name']}_{name}{s}"] = v
             rec["log_tail"] = log.strip().splitlines()[-3:]
             fixtures["euler"][case["name"]] = rec
+            if case.get("keep_fields"):
+                # the reference's own field files, binary and text (the same
+                # deck rerun with write_format TEXT), for the writer's tests
+                s1 = case["check_steps"][0]
+                os.makedirs(FIELDS, exist_ok=True)
+                for name in ("T", "U"):
+                    shutil.copy(os.path.join(d, f"{name}{s1}.bin"),
+                                os.path.join(FIELDS, f"{case['name']}_{name}{s1}.bin"))
+                dt = os.path.join(tmp, "t_" + case["name"])
+                write_euler_case(dt, dict(case, write_format="TEXT", check_steps=[s1]))
+                run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], dt)
+                run([os.path.join(REF, "euler"), "controls"], dt)
+                for name in ("T", "U"):
+                    shutil.copy(os.path.join(dt, f"{name}{s1}.txt"),
+                                os.path.join(FIELDS, f"{case['name']}_{name}{s1}.txt"))
+                # the mesh tool's grid file, binary and text
+                shutil.copy(os.path.join(d, "grid_0.bin"), os.path.join(FIELDS, f"{case['name']}_grid_0.bin"))
+                run([os.path.join(REF, "mesh"), "spec", "-o", "grid_t.txt"], dt)
+                shutil.copy(os.path.join(dt, "grid_t.txt"), os.path.join(FIELDS, f"{case['name']}_grid_0.txt"))
             print("euler", case["name"], "ok")
         for case in POTENTIAL_CASES:
             d = os.path.join(tmp, "p_" + case["name"])

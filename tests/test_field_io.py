@@ -1,0 +1,68 @@
+"""The reference's on-disk formats: files the reference itself wrote
+(tests/golden/fields: the e3_n3 deck's T and U after one step, as text and
+as the binary token stream, and its mesh tool's grid file in both forms) are
+read and written back, or regenerated from the block description, byte for
+byte."""
+import os
+
+import pytest
+
+from _cases import EULER_CASES
+from _euler_setup import load_case
+from nebulasem_amd import read_field, write_field
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = os.path.join(HERE, "golden", "fields")
+
+
+@pytest.mark.parametrize("name,ncomp", [("T", 1), ("U", 3)])
+@pytest.mark.parametrize("ext", ["txt", "bin"])
+def test_field_round_trip_is_byte_identical(tmp_path, name, ncomp, ext):
+    case = [c for c in EULER_CASES if c["name"] == "e3_n3"][0]
+    mesh, _ = load_case(tmp_path, case)
+    src = os.path.join(FIELDS, f"e3_n3_{name}1.{ext}")
+    vals, kinds, bv = read_field(mesh, src, ncomp)
+    out = str(tmp_path / f"{name}1.{ext}")
+    write_field(mesh, out, vals[:mesh.info.nbcs_field], kinds, bv)
+    assert open(out, "rb").read() == open(src, "rb").read()
+
+
+def test_field_writer_boundary_values(tmp_path):
+    """A patch value is written when its magnitude exceeds 1e-7 (the
+    reference's equal(mag(value), 0)), and read back."""
+    case = [c for c in EULER_CASES if c["name"] == "e3_n3"][0]
+    mesh, _ = load_case(tmp_path, case)
+    nb = len(mesh.boundaries)
+    import numpy as np
+    vals = np.linspace(-1, 1, 3 * mesh.info.nbcs_field).reshape(-1, 3)
+    kinds = np.array([2] + [1] * (nb - 1), dtype=np.int32)
+    bv = np.zeros((nb, 3))
+    bv[0] = (1.5, 0.0, -2.0)
+    bv[1] = (5e-8, 0, 0)   # below the threshold: no value line
+    for ext in ("txt", "bin"):
+        out = str(tmp_path / f"U0.{ext}")
+        write_field(mesh, out, vals, kinds, bv)
+        v2, k2, b2 = read_field(mesh, out, 3)
+        assert np.array_equal(k2, kinds)
+        assert np.array_equal(b2[0], bv[0]) and not b2[1].any()
+        if ext == "txt":
+            assert np.allclose(v2[:mesh.info.nbcs_field], vals, rtol=1e-11, atol=1e-12)
+            text = open(out).read()
+            assert "\tvalue 1.5 0 -2 \n" in text and text.count("\tvalue") == 1
+        else:
+            assert np.array_equal(v2[:mesh.info.nbcs_field], vals)
+
+
+@pytest.mark.parametrize("ext", ["txt", "bin"])
+def test_grid_file_is_byte_identical_to_the_mesh_tool(tmp_path, ext):
+    """Our block generator and grid writer against the reference mesh tool's
+    output for the same block description (meshApp.cpp, mesh.h:158-220)."""
+    from _cases import write_euler_case
+    from nebulasem_amd import Mesh
+    case = [c for c in EULER_CASES if c["name"] == "e3_n3"][0]
+    write_euler_case(str(tmp_path), case)
+    m = Mesh.from_spec(str(tmp_path / "spec"), order=(0, 0, 0))
+    out = str(tmp_path / f"grid_0.{ext}")
+    m.write_grid(out)
+    ref = os.path.join(FIELDS, f"e3_n3_grid_0.{ext}")
+    assert open(out, "rb").read() == open(ref, "rb").read()
