@@ -412,6 +412,12 @@ int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
             a.last = stage == 2;
             a.check = (stage == 2);
             a.flag = d->flag.as<int>();
+            if (d->dev.diffusion) {
+                // the stage state's gradients for the viscous fluxes
+                R* grad = d->grad.as<R>();
+                HIP_TRY((launch_mol_gradient<R, S>(d->dev, a.q, a.din, grad, d->stream)));
+                a.grad = grad;
+            }
             HIP_TRY((launch_rk_stage<R, S>(d->dev, a, d->stream)));
         }
         d->mcur = 1 - d->mcur;
@@ -426,7 +432,7 @@ int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double,
 
 int nsem_mol_supported(const nsem_dg* d) {
     if (d->prm.diffusion)
-        return nsem_fail(NSEM_EUNSUPPORTED, "artificial viscosity is carried by the split step only");
+        return nsem_fail(NSEM_EUNSUPPORTED, "the derivative entry points do not carry artificial viscosity");
     return NSEM_OK;
 }
 
@@ -434,8 +440,7 @@ extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    int rc = nsem_mol_supported(d);
-    if (rc) return rc;
+    int rc;
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
     else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
@@ -453,8 +458,7 @@ int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
 
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
     if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
-    int rc = nsem_mol_supported(d);
-    if (rc) return rc;
+    int rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);

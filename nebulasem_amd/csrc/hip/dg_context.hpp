@@ -71,5 +71,5 @@ struct nsem_dg {
 int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps);
 
 // NSEM_OK, or NSEM_EUNSUPPORTED when the problem has artificial viscosity,
-// which only the split step carries.
+// which the derivative (tangent-linear, adjoint) entry points do not carry.
 int nsem_mol_supported(const nsem_dg* d);

@@ -42,7 +42,18 @@ struct RkArgs {
     const double* tdin[5];
     double* tdout[5];
     double* tqout[5];
+    // artificial viscosity: the stage state's strong gradients, [12][nb] in
+    // the residual type (3a+b = d_a U_b, 9+a = d_a theta'), from
+    // launch_mol_gradient; null without diffusion
+    const void* grad;
 };
+
+// Strong gradients of U and theta' at the stage state q + din of the
+// method-of-lines form (the split step's gradf<true> rule, evaluated in the
+// residual type R), written to grad [12][nb] of R.
+template <typename R, typename S>
+hipError_t launch_mol_gradient(const DgDev& d, const double* const q[5], const S* const din[5], R* grad,
+                               hipStream_t st);
 // Vector-Jacobian product of the method-of-lines residual at Q = q + din:
 // t = (dL/dQ)^T w, all FP64 SoA node arrays (din may be null).
 // Epilogue (the adjoint's stage algebra, fused): with t = J^T w at a node,

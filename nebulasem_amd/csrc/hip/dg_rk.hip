@@ -100,6 +100,7 @@ template <typename R>
 struct FaceTask {
     uint32_t meta;
     int g, fa, fb, fnode;
+    uint64_t node;    // the node the state was read from (neighbour, or own on a boundary)
     R nv[3];
     R rp, m[3], tp, rref, pref;
 };
@@ -127,6 +128,7 @@ __device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S
     F.nv[2] = (R)(d.gFN[f3 + 2] * wgt);
     const uint64_t m = interior ? face_node(ef.other, og, F.fa, F.fb, NX, NY, NZ)
                                 : (uint64_t)e * (NX * NY * NZ) + F.fnode;
+    F.node = m;
     F.rp = load_state<R, S, O32, DIN>(a, 0, m);
     F.m[0] = load_state<R, S, O32, DIN>(a, 1, m);
     F.m[1] = load_state<R, S, O32, DIN>(a, 2, m);
@@ -134,6 +136,28 @@ __device__ __forceinline__ void prepare_face_task(const DgDev& d, const RkArgs<S
     F.tp = load_state<R, S, O32, DIN>(a, 4, m);
     F.rref = (R)ldn<O32>(d.rho_ref, m);
     F.pref = (R)ldn<O32>(d.p_ref, m);
+}
+
+// Ghost-node gradients of the viscous terms, as the split step forms them
+// (fillBCs(r, X.fIndex), field.h:2745-2768): the owner's copy, zero on the
+// field's SYMMETRY patches, the patch value on the tensor diagonal (theta:
+// (value, 0, 0)) on its NEUMANN patches.
+template <typename R>
+__device__ __forceinline__ void ghost_gradient_r(const BcTable& bt, int pi, const R* own, R* out) {
+    const int ku = bt.kind[1][pi], kt = bt.kind[2][pi];
+    const bool zu = ku == BC_SYMMETRY || ku == BC_NEUMANN, zt = kt == BC_SYMMETRY || kt == BC_NEUMANN;
+    for (int c = 0; c < 9; c++) out[c] = zu ? R(0) : own[c];
+    if (ku == BC_NEUMANN) { out[0] = (R)bt.value[1][pi]; out[4] = (R)bt.value[2][pi]; out[8] = (R)bt.value[3][pi]; }
+    for (int c = 9; c < 12; c++) out[c] = zt ? R(0) : own[c];
+    if (kt == BC_NEUMANN) out[9] = (R)bt.value[4][pi];
+}
+
+// Normal components of the viscous fluxes mu grad U (rows 0-2) and
+// mu/Pr grad theta' (row 3) through the area-weighted normal nv.
+template <typename R>
+__device__ __forceinline__ void viscous_normal(const R* g, R mu, R iPr, const R* nv, R* v) {
+    for (int r = 0; r < 3; r++) v[r] = mu * (g[3 * r] * nv[0] + g[3 * r + 1] * nv[1] + g[3 * r + 2] * nv[2]);
+    v[3] = (mu * iPr) * (g[9] * nv[0] + g[10] * nv[1] + g[11] * nv[2]);
 }
 
 // Primitive rows kept in LDS for the face tasks (the owner side of each face
@@ -156,7 +180,9 @@ constexpr int kPrimRows = 10;
 // AFF: the elements are affine, so the metric is formed from one Jinv and
 // cell volume per element (JcV = Jinv_e (vol_e W), cV = vol_e W with W the
 // node's LGL weight product) instead of being read per node.
-template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN, bool AFF>
+// VISC: artificial viscosity: the flux rows carry -mu grad U and -mu/Pr grad
+// theta' (mu = rho visc) from a.grad, on both sides of every face.
+template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN, bool AFF, bool VISC>
 __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -193,6 +219,11 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     double tpre[5] = {0, 0, 0, 0, 0};  // its tangent (tangent-linear instantiation)
     R rref = 0, pref = 0, cvn = 1;
     FaceTask<R> ft{};   // global reads of this thread's first face task
+    constexpr int NG = VISC ? 12 : 1;
+    R gown[NG];          // this node's gradients (VISC)
+    R gA[NG], gB[NG];    // the face task's own-side and far-side gradients (VISC)
+    R rrefA = 0;         // the face task's own-side background density (VISC)
+    const R* grd = static_cast<const R*>(a.grad);
     if (active) {
         // all of this phase's global reads first (own node, then the face
         // task), straight-line, so they are in flight together
@@ -218,6 +249,17 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
                 for (int c = 0; c < 9; c++) J[c] = (R)ldn<O32>(d.JcV + (size_t)c * nb, n);
         }
         if (q < NFN) prepare_face_task<R, S, O32, DIN>(d, a, e, q, NX, NY, NZ, ft);
+        if constexpr (VISC) {
+            for (int c = 0; c < 12; c++) gown[c] = ldn<O32>(grd + (size_t)c * nb, n);
+            if (q < NFN) {
+                const uint64_t na = (uint64_t)e * NP + ft.fnode;
+                for (int c = 0; c < 12; c++) {
+                    gA[c] = ldn<O32>(grd + (size_t)c * nb, na);
+                    gB[c] = ldn<O32>(grd + (size_t)c * nb, ft.node);
+                }
+                rrefA = (R)ldn<O32>(d.rho_ref, na);
+            }
+        }
         for (int v = 0; v < 5; v++) {
             pre[v] = a.last ? base[v] + a.b * din[v] : a.b * din[v];
             if constexpr (is_dual<R>::value) {
@@ -237,10 +279,23 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             P.tp = (R)(base[4] + din[4]);
         }
         make_prim(P, rref, pref, T0, gam, gR);
+        R mu = 0, imu = 0;
+        if constexpr (VISC) {
+            mu = (rref + P.rp) * (R)d.visc;
+            imu = mu * (R)d.iPr;
+        }
         // rows: rho, m_x, m_y, m_z, Theta
         for (int row = 0; row < 5; row++) {
             R f0, f1, f2, g0, g1, g2;
             flux_row(P, row, T0, f0, f1, f2);
+            if constexpr (VISC) {
+                if (row >= 1 && row <= 3) {
+                    const int r3 = 3 * (row - 1);
+                    f0 = f0 - mu * gown[r3]; f1 = f1 - mu * gown[r3 + 1]; f2 = f2 - mu * gown[r3 + 2];
+                } else if (row == 4) {
+                    f0 = f0 - imu * gown[9]; f1 = f1 - imu * gown[10]; f2 = f2 - imu * gown[11];
+                }
+            }
             contravariant(J, f0, f1, f2, g0, g1, g2);
             G[(row * 3 + 0) * NP + q] = g0;
             G[(row * 3 + 1) * NP + q] = g1;
@@ -274,7 +329,17 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
 
     if (active) {
         for (int t = q; t < NFN; t += NP) {
-            if (t != q) prepare_face_task<R, S, O32, DIN>(d, a, e, t, NX, NY, NZ, ft);
+            if (t != q) {
+                prepare_face_task<R, S, O32, DIN>(d, a, e, t, NX, NY, NZ, ft);
+                if constexpr (VISC) {
+                    const uint64_t na = (uint64_t)e * NP + ft.fnode;
+                    for (int c = 0; c < 12; c++) {
+                        gA[c] = ldn<O32>(grd + (size_t)c * nb, na);
+                        gB[c] = ldn<O32>(grd + (size_t)c * nb, ft.node);
+                    }
+                    rrefA = (R)ldn<O32>(d.rho_ref, na);
+                }
+            }
             const FaceTask<R>& F = ft;
             if (!((F.meta >> 4) & 1)) continue;
             const bool own = (F.meta >> 3) & 1;
@@ -304,6 +369,25 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             R flux[5];
             if (own) rusanov(A, Q, nv, ghost ? R(0) : R(0.5), T0, flux);
             else rusanov(Q, A, nv, ghost ? R(0) : R(0.5), T0, flux);
+            if constexpr (VISC) {
+                // central average of -mu grad . n; the far side of a boundary
+                // face takes the ghost gradient and the own density
+                R gG[12];
+                const R* gFar = gB;
+                if (ghost) {
+                    ghost_gradient_r(*d.bc, (int)(F.meta >> 8), gA, gG);
+                    gFar = gG;
+                }
+                const R muA = (rrefA + A.rp) * (R)d.visc;
+                const R muB = ghost ? muA : (F.rref + F.rp) * (R)d.visc;
+                R vA[4], vB[4];
+                viscous_normal(gA, muA, (R)d.iPr, nv, vA);
+                viscous_normal(gFar, muB, (R)d.iPr, nv, vB);
+                const R fI = ghost ? R(0) : R(0.5), fJ = R(1) - fI;
+                const R* vO = own ? vA : vB;
+                const R* vN = own ? vB : vA;
+                for (int r = 0; r < 4; r++) flux[1 + r] = flux[1 + r] - (vO[r] * fI + vN[r] * fJ);
+            }
             for (int v = 0; v < 5; v++) FL[v * NFN + t] = flux[v];
         }
     }
@@ -348,39 +432,205 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     }
 }
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF>
+// ---------------------------------------------------------------------------
+// Strong gradients of U and theta' at the stage state Q = q + din of the
+// method-of-lines form, as the split step's gradf<true> forms them (surface
+// lift fN (x) (cds(X) - X) over the faces through the node in face-group
+// order, then (Jin grad psi_s)(x_t) (x) X_s over the node's x line and its y
+// and z lines without the node, then / cV), in the residual type R.  Ghost
+// traces: U by its boundary condition (ghost_state), theta' a copy.
+template <typename S>
+struct MolGradIn {
+    const double* q[5];
+    const S* din[5];
+};
+
+template <typename R, typename S, int NXc, int NYc, int NZc, bool DIN, bool AFF>
+__global__ void __launch_bounds__(512) mol_grad_kernel(DgDev d, MolGradIn<S> in, R* __restrict__ grad) {
+    const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
+    const int NP = NX * NY * NZ;
+    const int epb = blockDim.x / NP;
+    const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
+    const uint32_t e = blockIdx.x * epb + le;
+    const bool active = le < epb && e < d.nelem;
+    extern __shared__ unsigned char lds_raw[];
+    R* Pl = reinterpret_cast<R*>(lds_raw) + (size_t)le * 4 * NP;   // U_x U_y U_z theta'
+    const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
+    const uint64_t n = (uint64_t)e * NP + q;
+    const R T0 = (R)d.T0;
+    auto prims = [&](uint64_t m, R* u, R& th) {
+        R Qv[5];
+        for (int v = 0; v < 5; v++) Qv[v] = (R)(in.q[v][m] + (DIN ? (double)in.din[v][m] : 0.0));
+        const R rho = (R)d.rho_ref[m] + Qv[0];
+        const R ir = r_div(R(1), rho);
+        u[0] = Qv[1] * ir; u[1] = Qv[2] * ir; u[2] = Qv[3] * ir;
+        th = (Qv[4] - T0 * Qv[0]) * ir;
+    };
+    R u[3] = {0, 0, 0}, th = 0;
+    if (active) {
+        prims(n, u, th);
+        Pl[q] = u[0]; Pl[NP + q] = u[1]; Pl[2 * NP + q] = u[2]; Pl[3 * NP + q] = th;
+    }
+    __syncthreads();
+    if (!active) return;
+
+    R gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gt[3] = {0, 0, 0};
+    for (int g = 0; g < 6; g++) {
+        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+        const EFace ef = d.eface[(size_t)e * 6 + g];
+        if (!((ef.meta >> 4) & 1)) continue;
+        const bool own = (ef.meta >> 3) & 1;
+        const bool ghost = (ef.meta >> 5) & 1;
+        const int og = ef.meta & 7;
+        int a, b;
+        face_ab(g, i, j, k, a, b);
+        const double wgt = face_weight(d, own ? g : og, a, b);
+        const R nv[3] = {(R)(d.gFN[3 * (size_t)ef.facet] * wgt), (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
+                         (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
+        R ou[3], oth;
+        if (!ghost) {
+            prims(face_node(ef.other, og, a, b, NX, NY, NZ), ou, oth);
+        } else {
+            const BcTable& bt = *d.bc;
+            const int pi = ef.meta >> 8;
+            const int kind = bt.kind[1][pi];
+            if (kind == BC_SYMMETRY) sym_r(u, nv, ou);
+            else if (kind == BC_DIRICHLET) { ou[0] = (R)bt.value[1][pi]; ou[1] = (R)bt.value[2][pi]; ou[2] = (R)bt.value[3][pi]; }
+            else { ou[0] = u[0]; ou[1] = u[1]; ou[2] = u[2]; }
+            oth = th;
+        }
+        const R fI = ghost ? R(0) : R(0.5), fJ = R(1) - fI;
+        R du[3];
+        for (int c = 0; c < 3; c++) du[c] = (own ? u[c] * fI + ou[c] * fJ : ou[c] * fI + u[c] * fJ) - u[c];
+        const R dth = (own ? th * fI + oth * fJ : oth * fI + th * fJ) - th;
+        const R sg = own ? R(1) : R(-1);
+        for (int aa = 0; aa < 3; aa++) {
+            for (int c = 0; c < 3; c++) gu[3 * aa + c] += sg * (nv[aa] * du[c]);
+            gt[aa] += sg * (nv[aa] * dth);
+        }
+    }
+    R J[9], cv;
+    if constexpr (AFF) {
+        const double* em = d.emet + (size_t)e * 10;
+        const double cv64 = em[9] * d.wnode[q];
+        cv = (R)cv64;
+        for (int c = 0; c < 9; c++) J[c] = (R)(em[c] * cv64);
+    } else {
+        cv = (R)d.cV[n];
+        for (int c = 0; c < 9; c++) J[c] = (R)d.JcV[(size_t)c * d.nb + n];
+    }
+    auto add = [&](int src, R w0, R w1, R w2) {
+        const R p0 = Pl[src], p1 = Pl[NP + src], p2 = Pl[2 * NP + src], p3 = Pl[3 * NP + src];
+        gu[0] += w0 * p0; gu[1] += w0 * p1; gu[2] += w0 * p2;
+        gu[3] += w1 * p0; gu[4] += w1 * p1; gu[5] += w1 * p2;
+        gu[6] += w2 * p0; gu[7] += w2 * p1; gu[8] += w2 * p2;
+        gt[0] += w0 * p3; gt[1] += w1 * p3; gt[2] += w2 * p3;
+    };
+    const int syz = NY * NZ;
+    for (int t = 0; t < NX; t++) {
+        const int src = t * syz + j * NZ + k;
+        if (t == i) {
+            const R d0 = (R)d.D[0][i * NX + i], d1 = (R)d.D[1][j * NY + j], d2 = (R)d.D[2][k * NZ + k];
+            add(src, J[0] * d0 + J[3] * d1 + J[5] * d2, J[6] * d0 + J[1] * d1 + J[4] * d2,
+                J[8] * d0 + J[7] * d1 + J[2] * d2);
+        } else {
+            const R D = (R)d.D[0][i * NX + t];
+            add(src, J[0] * D, J[6] * D, J[8] * D);
+        }
+    }
+    for (int t = 0; t < NY; t++) {
+        if (t == j) continue;
+        const R D = (R)d.D[1][j * NY + t];
+        add(i * syz + t * NZ + k, J[3] * D, J[1] * D, J[7] * D);
+    }
+    for (int t = 0; t < NZ; t++) {
+        if (t == k) continue;
+        const R D = (R)d.D[2][k * NZ + t];
+        add(i * syz + j * NZ + t, J[5] * D, J[4] * D, J[2] * D);
+    }
+    const R icv = r_div(R(1), cv);
+    for (int c = 0; c < 9; c++) grad[(size_t)c * d.nb + n] = gu[c] * icv;
+    for (int c = 0; c < 3; c++) grad[(size_t)(9 + c) * d.nb + n] = gt[c] * icv;
+}
+
+template <typename R, typename S, int X, int Y, int Z>
+static hipError_t launch_mol_gradient_t(const DgDev& d, const MolGradIn<S>& in, R* grad, hipStream_t st) {
+    const int NP = d.np;
+    if (NP > 512) return hipErrorInvalidValue;
+    const int epb = NP >= 256 ? 1 : 256 / NP;
+    const int threads = (epb * NP + 63) / 64 * 64;
+    const unsigned blocks = (d.nelem + epb - 1) / epb;
+    const size_t lds = (size_t)epb * 4 * NP * sizeof(R);
+    const bool din = in.din[0] != nullptr;
+    if (d.affine) {
+        if (din) hipLaunchKernelGGL((mol_grad_kernel<R, S, X, Y, Z, true, true>), dim3(blocks), dim3(threads), lds, st, d, in, grad);
+        else hipLaunchKernelGGL((mol_grad_kernel<R, S, X, Y, Z, false, true>), dim3(blocks), dim3(threads), lds, st, d, in, grad);
+    } else {
+        if (din) hipLaunchKernelGGL((mol_grad_kernel<R, S, X, Y, Z, true, false>), dim3(blocks), dim3(threads), lds, st, d, in, grad);
+        else hipLaunchKernelGGL((mol_grad_kernel<R, S, X, Y, Z, false, false>), dim3(blocks), dim3(threads), lds, st, d, in, grad);
+    }
+    return hipGetLastError();
+}
+
+template <typename R, typename S>
+hipError_t launch_mol_gradient(const DgDev& d, const double* const q[5], const S* const din[5], R* grad,
+                               hipStream_t st) {
+    MolGradIn<S> in{};
+    for (int v = 0; v < 5; v++) { in.q[v] = q[v]; in.din[v] = din[v]; }
+    if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_mol_gradient_t<R, S, 8, 8, 8>(d, in, grad, st);
+    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_mol_gradient_t<R, S, 5, 1, 5>(d, in, grad, st);
+    if (d.npx == 4 && d.npy == 4 && d.npz == 4) return launch_mol_gradient_t<R, S, 4, 4, 4>(d, in, grad, st);
+    return launch_mol_gradient_t<R, S, 0, 0, 0>(d, in, grad, st);
+}
+
+template hipError_t launch_mol_gradient<double, double>(const DgDev&, const double* const[5], const double* const[5],
+                                                        double*, hipStream_t);
+template hipError_t launch_mol_gradient<float, float>(const DgDev&, const double* const[5], const float* const[5],
+                                                      float*, hipStream_t);
+
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC>
 __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z, false, DIN, AFF>(d, a);
+    rk_stage_body<R, S, X, Y, Z, false, DIN, AFF, VISC>(d, a);
 }
 
 // N = 7 (512 nodes, one element per work-group), node offsets in 32 bits,
 // registers capped at W waves per SIMD: measured on the 64x64x32 N=7
 // workload, FP32 runs fastest at 6 (three work-groups per CU), FP64 at 4
 // (two work-groups per CU).
-template <typename R, typename S, int W, bool DIN, bool AFF>
+template <typename R, typename S, int W, bool DIN, bool AFF, bool VISC>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
 rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, 8, 8, 8, true, DIN, AFF>(d, a);
+    rk_stage_body<R, S, 8, 8, 8, true, DIN, AFF, VISC>(d, a);
 }
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF>
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC>
 static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
-    constexpr int W = sizeof(R) == 4 ? 6 : 4;
+    constexpr int W = sizeof(R) == 4 ? (VISC ? 4 : 6) : (VISC ? 2 : 4);
     if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
         if (d.nb * sizeof(double) < (1ull << 32)) {
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN, AFF>), dim3(blocks), dim3(threads), lds, st, d, a);
+            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN, AFF, VISC>), dim3(blocks), dim3(threads), lds, st,
+                               d, a);
             return;
         }
     }
-    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN, AFF>), dim3(blocks), dim3(threads), lds, st, d, a);
+    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN, AFF, VISC>), dim3(blocks), dim3(threads), lds, st, d, a);
 }
 
 template <typename R, typename S, int X, int Y, int Z, bool DIN>
 static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
-    if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true>(d, a, blocks, threads, lds, st);
-    else launch_rk_kernel<R, S, X, Y, Z, DIN, false>(d, a, blocks, threads, lds, st);
+    // the tangent-linear instantiation has no viscous variant (the
+    // derivative entry points refuse diffusion)
+    if constexpr (!is_dual<R>::value) {
+        if (a.grad) {
+            if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, true>(d, a, blocks, threads, lds, st);
+            else launch_rk_kernel<R, S, X, Y, Z, DIN, false, true>(d, a, blocks, threads, lds, st);
+            return;
+        }
+    }
+    if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, false>(d, a, blocks, threads, lds, st);
+    else launch_rk_kernel<R, S, X, Y, Z, DIN, false, false>(d, a, blocks, threads, lds, st);
 }
 
 template <typename R, typename S, int X, int Y, int Z>

@@ -649,7 +649,10 @@ void om_fv_correct_velocity(const om_mesh* m, const int32_t* bc_p, const double*
  * the method-of-lines right-hand side with every flux at one state
  *   L_rho = -r_rho/cV,  L_m = -r_m/cV + (rho - rho_ref) g,  L_Th = -r_Th/cV
  * (r from divf_dg with F = rho U, rho U (x) U + p' I, rho U theta) and
- * SSP-RK3 (Shu & Osher 1988) on (rho, rho U, rho theta). */
+ * SSP-RK3 (Shu & Osher 1988) on (rho, rho U, rho theta).  With diffusion the
+ * momentum and theta fluxes carry -mu grad U and -mu/Pr grad theta' (mu =
+ * rho viscosity) at the stage state, from the split step's pinned strong
+ * gradient gradf_strong and its ghost rule. */
 static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, const double* rho_ref,
                     double* rho, double* U, double* T, double* L /* naf*5 */) {
     const uint64_t N = m->naf, nfn = (uint64_t)m->nfacets * m->npf;
@@ -664,6 +667,14 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
     double* q = (double*)malloc(sizeof(double) * N * 3);
     double* r = (double*)malloc(sizeof(double) * N * 3);
     double* p = (double*)malloc(sizeof(double) * N);
+    /* artificial viscosity at the stage state: mu = rho viscosity, the strong
+     * gradients of U and theta' with the split step's ghost rule */
+    double* GU = e->diffusion ? (double*)malloc(sizeof(double) * N * 9) : NULL;
+    double* GT = e->diffusion ? (double*)malloc(sizeof(double) * N * 3) : NULL;
+    if (e->diffusion) {
+        gradf_strong(m, 3, U, e->bc_U, e->val_U, GU);
+        gradf_strong(m, 1, T, e->bc_T, e->val_T, GT);
+    }
     for (uint64_t i = 0; i < N; i++) w[i] = sqrt(dot3(&U[3 * i], &U[3 * i]));
     cds_scalar(m, w, lam);
     for (uint64_t i = 0; i < N; i++) w[i] = sqrt(gam * R * (T[i] + e->T0));
@@ -680,6 +691,8 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++)
                 fq[9 * i + 3 * a + b] = rho[i] * U[3 * i + a] * U[3 * i + b] + (a == b ? p[i] : 0.0);
+        if (e->diffusion)
+            for (int c = 0; c < 9; c++) fq[9 * i + c] = fq[9 * i + c] - (rho[i] * e->viscosity) * GU[9 * i + c];
         for (int c = 0; c < 3; c++) q[3 * i + c] = rho[i] * U[3 * i + c];
     }
     divf_dg(m, 3, fq, q, lam, r);
@@ -689,11 +702,14 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
     /* rho theta */
     for (uint64_t i = 0; i < N; i++) {
         for (int c = 0; c < 3; c++) fq[3 * i + c] = rho[i] * U[3 * i + c] * (T[i] + e->T0);
+        if (e->diffusion)
+            for (int c = 0; c < 3; c++)
+                fq[3 * i + c] = fq[3 * i + c] - ((rho[i] * e->viscosity) * (1 / e->Pr)) * GT[3 * i + c];
         q[i] = rho[i] * (T[i] + e->T0);
     }
     divf_dg(m, 1, fq, q, lam, r);
     for (uint64_t i = 0; i < N; i++) L[5 * i + 4] = -r[i] / m->cV[i];
-    free(lam); free(t1); free(w); free(fq); free(q); free(r); free(p);
+    free(lam); free(t1); free(w); free(fq); free(q); free(r); free(p); free(GU); free(GT);
 }
 
 /* SSP-RK3 steps of the method-of-lines form; state as the split steps use

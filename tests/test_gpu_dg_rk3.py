@@ -9,7 +9,8 @@ from test_gpu_dg_split import relerr, setup
 pytestmark = pytest.mark.gpu
 
 CASES = [c for c in EULER_CASES
-         if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_n7", "e3_aniso_bubbles", "e3_trilinear_n3")]
+         if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_n7", "e3_aniso_bubbles", "e3_trilinear_n3",
+                          "e3_visc_n3", "cfg0_2d_visc")]
 
 
 @pytest.mark.parametrize("name,affine", [("cfg0_2d_n4", True), ("e3_n3", True), ("e3_wall_n3", False),
@@ -84,3 +85,21 @@ def test_rk3_conserves_mass_on_device(oracle, tmp_path):
 
 def rho_ref_of(dg):
     return dg.background()[1]
+
+
+@pytest.mark.parametrize("name", ["e3_visc_n3", "cfg0_2d_visc"])
+def test_rk3_fp32_viscous_against_fp64(oracle, tmp_path, name):
+    """The FP32-perturbation form with artificial viscosity against FP64."""
+    case = [c for c in EULER_CASES if c["name"] == name][0]
+    mesh, om, prm, dg, state = setup(oracle, tmp_path, case)
+    dg.rk3_steps(10, 64)
+    q64 = dg.get_conserved()
+    mesh2, om2, prm2, dg2, state2 = setup(oracle, tmp_path / "b", case)
+    dg2.rk3_steps(10, 32)
+    q32 = dg2.get_conserved()
+    for nm, a, b in zip(("rho'", "m", "Theta'"), q32, q64):
+        l2, linf = relerr(a, b)
+        print(f"  viscous fp32 vs fp64 {nm}: rel L2 {l2:.2e} rel max {linf:.2e}", flush=True)
+        assert l2 <= 1e-5, (nm, l2)
+    dg.close()
+    dg2.close()

@@ -92,15 +92,15 @@ def test_resting_hydrostatic_state_stays_at_rest(oracle, tmp_path):
     assert np.abs(T).max() < 1e-9
 
 
-def test_viscous_problem_rejects_method_of_lines(tmp_path):
-    """Artificial viscosity is carried by the split step only; the
-    method-of-lines entry points say so instead of dropping the term."""
+def test_viscous_problem_rejects_derivatives(tmp_path):
+    """Artificial viscosity runs through the split step and the method-of-
+    lines steps; the derivative entry points do not carry it and say so
+    instead of dropping the term."""
     from nebulasem_amd import NsemError
     case = [c for c in EULER_CASES if c.get("viscosity") is not None][0]
     mesh, f = load_case(tmp_path, case)
     dg = DGEuler.from_fields(mesh, euler_params(case["dt"], diffusion=1, viscosity=case["viscosity"]), f)
-    with pytest.raises(NsemError, match="split step only"):
-        dg.rk3_steps(1, precision=64)
-    with pytest.raises(NsemError, match="split step only"):
+    dg.rk3_steps(1, precision=64)
+    with pytest.raises(NsemError, match="do not carry artificial viscosity"):
         dg.residual_vjp(np.zeros((5, mesh.info.nbcs_field)))
     dg.close()
