@@ -439,7 +439,7 @@ def main():
                 "frac": achieved / HBM_PEAK,
                 "traffic": profiled_traffic("traffic_bytes_per_launch_fp32") if ndiv == (64, 64, 32) else None,
                 "traffic_source": "profiles/rk_stage_64x64x32_n7_hbm_bytes.md (mean of the 3 stages)",
-                "kernel": "rk_stage_kernel_n7<float,float,6,DIN,%s>" % ("true" if affine else "false"),
+                "kernel": "rk_stage_kernel_fx<float,float,8,8,8,6,DIN,%s,false>" % ("true" if affine else "false"),
                 "bytes_per_node": b32, "nodes_per_launch": nb,
                 # the exact mean over the three stages of what must move
                 # (per-element metric on affine meshes), beside the contract

@@ -187,14 +187,17 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int NFN = 2 * (NX * NY + NX * NZ + NY * NZ);
-    const int epb = a.epb;
-    // 512 nodes: one element per work-group, so the element index is uniform
-    const int le = NXc * NYc * NZc == 512 ? 0 : threadIdx.x / NP, q = threadIdx.x - le * NP;
+    // a compile-time shape of more than 128 nodes runs one element per
+    // work-group (the launcher guarantees epb = 1), so the element index is
+    // uniform across the work-group and lives in scalar registers
+    constexpr bool ONE = NXc * NYc * NZc > 128;
+    const int epb = ONE ? 1 : a.epb;
+    const int le = ONE ? 0 : threadIdx.x / NP, q = threadIdx.x - le * NP;
     // XCD-contiguous element mapping: work-group b runs on XCD b % 8
     const uint32_t per_xcd = (gridDim.x + 7) / 8;
     const uint32_t bmap = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     const uint32_t e = bmap * epb + le;
-    const bool active = le < epb && e < d.nelem;
+    const bool active = (ONE ? (int)threadIdx.x < NP : le < epb) && e < d.nelem;
     extern __shared__ unsigned char lds_raw[];
     const int per_elem = rk_lds_per_element(NX, NY, NZ);
     R* G = reinterpret_cast<R*>(lds_raw) + (size_t)le * per_elem;
@@ -593,24 +596,35 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
     rk_stage_body<R, S, X, Y, Z, false, DIN, AFF, VISC>(d, a);
 }
 
-// N = 7 (512 nodes, one element per work-group), node offsets in 32 bits,
-// registers capped at W waves per SIMD: measured on the 64x64x32 N=7
-// workload, FP32 runs fastest at 6 (three work-groups per CU), FP64 at 4
-// (two work-groups per CU).
-template <typename R, typename S, int W, bool DIN, bool AFF, bool VISC>
+// Compile-time element shape (X x Y x Z nodes), node offsets in 32 bits,
+// registers capped at W waves per SIMD.  Measured on the 64x64x32 N=7
+// workload, FP32 runs fastest at 6 (three work-groups per CU) and FP64 at 4
+// (two work-groups per CU).  Shapes of up to 128 nodes hold several elements
+// per work-group, whose element index is not uniform and costs registers, so
+// they take one wave less (FP64 at 4^3 nodes two less: at 3 waves it spills
+// 56 bytes per lane and ran 8% slower than at 2); the viscous variant's extra
+// gradient state takes two less.
+template <typename R, int X, int Y, int Z, bool VISC>
+constexpr int rk_waves_per_simd() {
+    constexpr int np = X * Y * Z;
+    if constexpr (sizeof(R) == 4) return VISC ? 4 : (np > 128 ? 6 : 5);
+    return VISC ? 2 : (np > 128 ? 4 : np > 64 ? 3 : 2);
+}
+
+template <typename R, typename S, int X, int Y, int Z, int W, bool DIN, bool AFF, bool VISC>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
-rk_stage_kernel_n7(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, 8, 8, 8, true, DIN, AFF, VISC>(d, a);
+rk_stage_kernel_fx(DgDev d, RkArgs<S> a) {
+    rk_stage_body<R, S, X, Y, Z, true, DIN, AFF, VISC>(d, a);
 }
 
 template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC>
 static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
-    constexpr int W = sizeof(R) == 4 ? (VISC ? 4 : 6) : (VISC ? 2 : 4);
-    if constexpr (X == 8 && Y == 8 && Z == 8 && !is_dual<R>::value) {
+    constexpr int W = rk_waves_per_simd<R, X, Y, Z, VISC>();
+    if constexpr (X > 0 && !is_dual<R>::value) {
         if (d.nb * sizeof(double) < (1ull << 32)) {
-            hipLaunchKernelGGL((rk_stage_kernel_n7<R, S, W, DIN, AFF, VISC>), dim3(blocks), dim3(threads), lds, st,
-                               d, a);
+            hipLaunchKernelGGL((rk_stage_kernel_fx<R, S, X, Y, Z, W, DIN, AFF, VISC>), dim3(blocks), dim3(threads),
+                               lds, st, d, a);
             return;
         }
     }
@@ -646,6 +660,7 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
     blocks = (blocks + 7) / 8 * 8;  // whole XCD chunks
     const size_t lds = rk_lds_bytes(d.npx, d.npy, d.npz, epb, sizeof(R));
     if (lds > 65536) return hipErrorInvalidValue;
+    if (X * Y * Z > 128 && epb != 1) return hipErrorInvalidValue;  // the body assumes one element
     a.epb = epb;
     if (a.din[0]) launch_rk_metric<R, S, X, Y, Z, true>(d, a, blocks, threads, lds, st);
     else launch_rk_metric<R, S, X, Y, Z, false>(d, a, blocks, threads, lds, st);
@@ -655,9 +670,11 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
 template <typename R, typename S>
 hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_rk_t<R, S, 8, 8, 8>(d, a, st);
-    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_rk_t<R, S, 5, 1, 5>(d, a, st);
+    if (d.npx == 7 && d.npy == 7 && d.npz == 7) return launch_rk_t<R, S, 7, 7, 7>(d, a, st);
     if (d.npx == 6 && d.npy == 6 && d.npz == 6) return launch_rk_t<R, S, 6, 6, 6>(d, a, st);
+    if (d.npx == 5 && d.npy == 5 && d.npz == 5) return launch_rk_t<R, S, 5, 5, 5>(d, a, st);
     if (d.npx == 4 && d.npy == 4 && d.npz == 4) return launch_rk_t<R, S, 4, 4, 4>(d, a, st);
+    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_rk_t<R, S, 5, 1, 5>(d, a, st);
     return launch_rk_t<R, S, 0, 0, 0>(d, a, st);
 }
 

@@ -8,9 +8,17 @@ from test_gpu_dg_split import relerr, setup
 
 pytestmark = pytest.mark.gpu
 
+_E3_N3 = [c for c in EULER_CASES if c["name"] == "e3_n3"][0]
+_E3_VISC = [c for c in EULER_CASES if c["name"] == "e3_visc_n3"][0]
+# the remaining compile-time element shapes of the stage kernel (5^3, 6^3 and
+# 7^3 nodes), inviscid and viscous, beside 4^3 (N = 3), 8^3 (N = 7), 5x1x5
+# (the 2D N = 4 deck) and the runtime-shape kernel (the anisotropic deck)
+ORDER_CASES = ([dict(_E3_N3, name=f"e3_n{n}", order=(n, n, n), dt=0.004) for n in (4, 5, 6)]
+               + [dict(_E3_VISC, name=f"e3_visc_n{n}", order=(n, n, n), dt=0.004) for n in (4, 6)])
+
 CASES = [c for c in EULER_CASES
          if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_n7", "e3_aniso_bubbles", "e3_trilinear_n3",
-                          "e3_visc_n3", "cfg0_2d_visc")]
+                          "e3_visc_n3", "cfg0_2d_visc")] + ORDER_CASES
 
 
 @pytest.mark.parametrize("name,affine", [("cfg0_2d_n4", True), ("e3_n3", True), ("e3_wall_n3", False),
@@ -65,6 +73,39 @@ def test_rk3_fp32_perturbation_error(oracle, tmp_path):
         l2, linf = relerr(a, b)
         print(f"  fp32 vs fp64 {name}: rel L2 {l2:.2e} rel max {linf:.2e}", flush=True)
         assert l2 <= 1e-5, (name, l2)
+    dg.close()
+    dg2.close()
+
+
+@pytest.mark.parametrize("case", ORDER_CASES, ids=[c["name"] for c in ORDER_CASES])
+def test_rk3_fp32_orders_against_fp64(oracle, tmp_path, case):
+    """The FP32-perturbation form of every compile-time element shape against
+    FP64 on the same deck.
+
+    Theta' = (rho theta)' carries the pressure perturbation: it starts at zero
+    (the bubble is in pressure balance) and after 10 steps is ~1e-7 of
+    Theta_ref = (P0/R) (p_ref/P0)^(1/gamma), so it is measured against
+    Theta_ref, where FP32 rounding of the full state would be ~6e-8; rho' and
+    m against themselves."""
+    print(f"[gpu] rk3 fp32 vs fp64 {case['name']}", flush=True)
+    mesh, om, prm, dg, state = setup(oracle, tmp_path, case)
+    dg.rk3_steps(10, 64)
+    q64 = dg.get_conserved()
+    p_ref = dg.background()[0]
+    mesh2, om2, prm2, dg2, state2 = setup(oracle, tmp_path / "b", case)
+    dg2.rk3_steps(10, 32)
+    q32 = dg2.get_conserved()
+    P0, R, gamma = 101325.0, 1004.67 - 715.5, 1004.67 / 715.5
+    theta_ref = (P0 / R) * (p_ref / P0) ** (1.0 / gamma)
+    for nm, a, b in zip(("rho'", "m", "Theta'"), q32, q64):
+        l2, linf = relerr(a, b)
+        print(f"  fp32 vs fp64 {nm}: rel L2 {l2:.2e} rel max {linf:.2e}", flush=True)
+        if nm == "Theta'":
+            l2 = np.linalg.norm(a - b) / np.linalg.norm(theta_ref)
+            print(f"  Theta' error / |Theta_ref|: {l2:.2e}", flush=True)
+            assert l2 <= 1e-9, (nm, l2)
+        else:
+            assert l2 <= 1e-5, (nm, l2)
     dg.close()
     dg2.close()
 
