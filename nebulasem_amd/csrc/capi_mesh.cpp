@@ -119,6 +119,15 @@ bool view_array(const nsem_mesh* m, const std::string& name, ArrayView& a) {
         if (d < 0 || d > 2) return false;
         const Basis1D& b = g.basis[d];
         a.d = name[0] == 'd' ? b.dpsi : b.psi; a.kind = 1;
+    } else if (name.size() == 7 && (name.compare(0, 6, "psiRef") == 0 || name.compare(0, 6, "psiCor") == 0)) {
+        // both halves, lower then upper, n*n each
+        int d = name.back() - '0';
+        if (d < 0 || d > 2) return false;
+        const Basis1D& b = g.basis[d];
+        const std::vector<double>* h = name[3] == 'R' ? b.ref : b.cor;
+        a.d = h[0];
+        a.d.insert(a.d.end(), h[1].begin(), h[1].end());
+        a.kind = 1;
     } else return false;
     a.count = a.kind == 0 ? a.u.size() : a.d.size();
     a.data = a.kind == 0 ? (const void*)a.u.data() : (const void*)a.d.data();

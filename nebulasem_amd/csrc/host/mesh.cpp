@@ -545,6 +545,71 @@ struct MeshWork {
         } while (std::next_permutation(order, order + 4));
         for (int i = 0; i < 4; i++) vp.push_back(fm[1][best[i]]);
     }
+    // Union of two polygons that share at least one edge (mesh.cpp:890-980):
+    // both are turned to the same orientation and rotated to their first
+    // vertex the other does not hold; the union walks f1 up to the first
+    // shared vertex, then f2's unshared run, then f1 from the last shared
+    // vertex, and starts again at a corner (not an edge midpoint).
+    bool merge_facets(const IntVec& a, const IntVec& b, IntVec& f) const {
+        IntVec f1 = a, f2 = b;
+        {
+            V3 N1, N2;
+            unit_normal(a, N1);
+            unit_normal(b, N2);
+            if (dot(N1, N2) < 0) std::reverse(f2.begin() + 1, f2.end());
+        }
+        auto first_unshared = [](const IntVec& x, const IntVec& y) {
+            for (size_t i = 0; i < x.size(); i++)
+                if (std::find(y.begin(), y.end(), x[i]) == y.end()) return int(i);
+            return -1;
+        };
+        const int v1 = first_unshared(f1, f2), v2 = first_unshared(f2, f1);
+        if (v1 < 0) return false;
+        std::rotate(f1.begin(), f1.begin() + v1, f1.end());
+        const bool contained = v2 < 0;
+        if (!contained) std::rotate(f2.begin(), f2.begin() + v2, f2.end());
+        size_t sa[2] = {0, 0}, sb[2] = {0, 0};
+        int count = 0;
+        for (size_t i = 0; i < f1.size(); i++)
+            for (size_t j = 0; j < f2.size(); j++)
+                if (f1[i] == f2[j]) {
+                    const int slot = count ? 1 : 0;
+                    sa[slot] = i;
+                    sb[slot] = j;
+                    count++;
+                }
+        if (count < 2) return false;
+        f.clear();
+        for (size_t i = 0; i <= sa[0]; i++) f.push_back(f1[i]);
+        if (contained) {
+            for (size_t i = sb[0] + 1; i < sb[1]; i++) f.push_back(f2[i]);
+        } else {
+            for (size_t i = sb[0] + 1; i < f2.size(); i++) f.push_back(f2[i]);
+            for (size_t i = 0; i < sb[1]; i++) f.push_back(f2[i]);
+        }
+        for (size_t i = sa[1]; i < f1.size(); i++) f.push_back(f1[i]);
+        for (size_t turn = 0; turn < f.size() && point_in_line(V[f[0]], V[f.back()], V[f[1]]); turn++)
+            std::rotate(f.begin(), f.begin() + 1, f.end());
+        return true;
+    }
+    // The merged outline of a group of coplanar faces (mesh.cpp:984-1016):
+    // faces that share no edge with the outline yet are retried after the
+    // others.  False when a pass merges nothing (the group is not connected).
+    bool merge_group(const IntVec& faces, IntVec& fn) const {
+        fn = F[faces[0]];
+        IntVec rest(faces.begin() + 1, faces.end());
+        while (!rest.empty()) {
+            IntVec left;
+            for (Int fi : rest) {
+                IntVec fm;
+                if (merge_facets(fn, F[fi], fm)) fn = fm;
+                else left.push_back(fi);
+            }
+            if (left.size() == rest.size()) return false;
+            rest.swap(left);
+        }
+        return true;
+    }
 };
 
 }  // namespace
@@ -606,9 +671,12 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
             for (size_t j = 1; j < c.size(); j++)
                 if (w.coplanar(F[c[0]], F[c[j]])) { ci.push_back(c[j]); indices.push_back(j); }
             erase_indices(c, indices);
-            if (ci.size() != 1) return set_err(err, "non-conforming (AMR) cells are not supported yet");
+            // a side split into several faces (a 2:1 non-conforming side)
+            // is represented by the outline of its faces
+            IntVec outline;
+            if (!w.merge_group(ci, outline)) return set_err(err, "could not merge the faces of a cell side");
             cng.push_back(ci);
-            fng.push_back(F[ci[0]]);
+            fng.push_back(outline);
             groupId[i] = -1;
         }
         const IntVec f0 = fng[0];
@@ -694,6 +762,13 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err) {
                 for (Int fc : cng[i]) { c.push_back(fc); b.push_back(idx); }
             }
         m.faceid.push_back(b);
+        // mortar flags (mesh.cpp:433-444): a face that shares its side of
+        // this cell with others is a mortar face; 2 when this cell owns it
+        if (c.size() > 6)
+            for (size_t j = 0; j < c.size(); j++) {
+                const Int f = c[j];
+                if (std::count(b.begin(), b.end(), b[j]) > 1) m.FMC[f] = (m.FOC[f] == Int(cidx)) ? 2 : 1;
+            }
     }
 
     tm.mark("fixHexCells");
@@ -870,6 +945,96 @@ void lgl_nodes(int N, double* x, double* w) {
     for (int i = 0; i < ph; i++) { x[i] = -x[p - i]; w[i] = +w[p - i]; }
 }
 
+// Lagrange basis of the n nodes x at the ns points xs: out[j*ns+s] = l_j(xs[s]).
+static void lagrange_at(int n, const double* x, int ns, const double* xs, double* out) {
+    for (int s = 0; s < ns; s++)
+        for (int j = 0; j < n; j++) {
+            double prod = 1;
+            for (int k = 0; k < n; k++)
+                if (k != j) prod *= ((xs[s] - x[k]) / (x[j] - x[k]));
+            out[j * ns + s] = prod;
+        }
+}
+
+// Gauss-Jordan inverse of an n x n matrix after one bubble pass that moves
+// larger first-column entries up (tensor.cpp:170-222; the same operation
+// order, so the mortar matrices agree bit for bit).
+static void gauss_jordan_inverse(const std::vector<double>& a_in, std::vector<double>& x, int n) {
+    std::vector<double> a = a_in;
+    x.assign(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; i++) x[i * n + i] = 1.0;
+    for (int i = n - 1; i > 0; i--)
+        if (a[(i - 1) * n] < a[i * n])
+            for (int k = 0; k < n; k++) {
+                std::swap(a[i * n + k], a[(i - 1) * n + k]);
+                std::swap(x[i * n + k], x[(i - 1) * n + k]);
+            }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            const double t = a[j * n + i] / a[i * n + i];
+            for (int k = 0; k < n; k++) {
+                a[j * n + k] -= a[i * n + k] * t;
+                x[j * n + k] -= x[i * n + k] * t;
+            }
+        }
+    for (int i = 0; i < n; i++) {
+        const double t = a[i * n + i];
+        for (int j = 0; j < n; j++) {
+            a[i * n + j] = a[i * n + j] / t;
+            x[i * n + j] = x[i * n + j] / t;
+        }
+    }
+}
+
+// 2:1 mortar matrices (dg.cpp:516-583).  With an (n+1)-point LGL rule on
+// the parent interval, M = (phi_j, phi_k), S_c = (phi_j, phi_k(half c)) and
+// its transpose G_c; ref[c] = (M^-1 S_c)^T, cor[c] = (M^-1 G_c)^T.
+static void build_mortar(Basis1D& b) {
+    const int n = b.n, ne = n + 1;
+    std::vector<double> xe(ne), we(ne), xr(2 * ne, 0.0), xc(2 * ne, 0.0);
+    lgl_nodes(ne, xe.data(), we.data());
+    if (n != 1)
+        for (int j = 0; j < ne; j++) {
+            xr[j] = -0.5 + xe[j] / 2;
+            xr[j + ne] = 0.5 + xe[j] / 2;
+            xc[j] = 2 * xe[j] + 1;
+            xc[j + ne] = 2 * xe[j] - 1;
+        }
+    std::vector<double> pr(2 * ne * ne, 0.0), pe(size_t(n) * ne);
+    for (int c = 0; c < 2; c++) lagrange_at(n, b.x.data(), ne, &xr[c * ne], &pr[c * ne * ne]);
+    lagrange_at(n, b.x.data(), ne, xe.data(), pe.data());
+    std::vector<double> M(n * n, 0.0), S(2 * n * n, 0.0), G(2 * n * n, 0.0);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++)
+            for (int q = 0; q < ne; q++) {
+                M[j * n + k] += (we[q] / 2) * pe[j * ne + q] * pe[k * ne + q];
+                for (int c = 0; c < 2; c++) {
+                    const double vs = (we[q] / 2) * pe[j * ne + q] * pr[c * ne * ne + k * ne + q];
+                    S[c * n * n + j * n + k] += vs;
+                    G[c * n * n + k * n + j] += vs;
+                }
+            }
+    std::vector<double> Mi;
+    gauss_jordan_inverse(M, Mi, n);
+    auto product_transposed = [&](const double* B, std::vector<double>& out) {
+        std::vector<double> t(n * n);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double sum = 0.0;
+                for (int k = 0; k < n; k++) sum += Mi[i * n + k] * B[k * n + j];
+                t[i * n + j] = sum;
+            }
+        out.assign(n * n, 0.0);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) out[i * n + j] = t[j * n + i];
+    };
+    for (int c = 0; c < 2; c++) {
+        product_transposed(&S[c * n * n], b.ref[c]);
+        product_transposed(&G[c * n * n], b.cor[c]);
+    }
+}
+
 void build_basis(int n, Basis1D& b) {
     b.n = n;
     b.x.assign(n, 0); b.w.assign(n, 0);
@@ -896,6 +1061,7 @@ void build_basis(int n, Basis1D& b) {
             }
             b.dpsi[s * n + i] = acc;
         }
+    build_mortar(b);
 }
 
 namespace {
@@ -1012,23 +1178,33 @@ bool build_geometry(const LoadedMesh& m, const int order[3], int nonortho, Geome
             const IntVec& fids = m.faceid[ci];
             if (c.size() < 4 || fids.size() != c.size())
                 return set_err(err, "a cell lost its faces (every boundary face unnamed?)");
-            Int id0 = fids[0], id1 = id0 ^ 1;
-            int n0 = 0, n1 = 0;
-            for (size_t i = 0; i < c.size(); i++) { n0 += fids[i] == id0; n1 += fids[i] == id1; }
-            if (n0 != 1 || n1 != 1) return set_err(err, "non-conforming cells are not supported yet");
+        }
+        // the first side of each cell and its opposite, as outlines when
+        // split (dg.cpp:180-201: merged face by face in cell order)
+        std::vector<IntVec> side0(m.nbcs), side1(m.nbcs);
+        for (Int ci = 0; ci < m.nbcs; ci++) {
+            const IntVec& c = m.cells[ci];
+            const IntVec& fids = m.faceid[ci];
+            const Int id0 = fids[0], id1 = id0 ^ 1;
+            for (size_t i = 0; i < c.size(); i++) {
+                IntVec* side = fids[i] == id0 ? &side0[ci] : fids[i] == id1 ? &side1[ci] : nullptr;
+                if (!side) continue;
+                if (side->empty()) { *side = m.facets[c[i]]; continue; }
+                IntVec fm;
+                if (!w.merge_facets(*side, m.facets[c[i]], fm))
+                    return set_err(err, "could not merge the faces of a cell side");
+                *side = fm;
+            }
+            if (side0[ci].empty() || side1[ci].empty()) return set_err(err, "a cell has no opposite sides");
         }
         parallel_for(m.nbcs, [&](uint64_t cb, uint64_t ce) {
         for (Int ci = (Int)cb; ci < (Int)ce; ci++) {
             const IntVec& c = m.cells[ci];
             const IntVec& fids = m.faceid[ci];
-            Int id0 = fids[0], id1 = id0 ^ 1;
-            IntVec f1, f2;
-            for (size_t i = 0; i < c.size(); i++) {
-                if (fids[i] == id0) f1 = m.facets[c[i]];
-                else if (fids[i] == id1) f2 = m.facets[c[i]];
-            }
+            const Int id0 = fids[0];
+            (void)c;
             IntVec vpi;
-            w.hex_corners(f1, f2, vpi);
+            w.hex_corners(side0[ci], side1[ci], vpi);
             V3 vp1[8], vp[8];
             for (int i = 0; i < 8; i++) vp1[i] = m.vertices[vpi[i]];
             if (id0 == 2) {

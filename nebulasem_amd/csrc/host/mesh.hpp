@@ -102,6 +102,10 @@ bool load_mesh(const Grid& g, LoadedMesh& m, std::string* err);
 struct Basis1D {
     Int n = 1;
     std::vector<double> x, w, psi, dpsi;
+    // 2:1 mortar matrices, n*n each, for the lower (0) and upper (1) half
+    // of the interval: ref interpolates a parent's nodes to a half's nodes,
+    // cor projects a half's values back (dg.cpp:516-583)
+    std::vector<double> ref[2], cor[2];
 };
 void lgl_nodes(int n, double* x, double* w);
 void build_basis(int n, Basis1D& b);

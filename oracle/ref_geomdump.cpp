@@ -140,6 +140,14 @@ int main(int argc, char* argv[]) {
                 std::snprintf(nm, sizeof nm, "wgl%d", d); put_f64(nm, w);
                 std::snprintf(nm, sizeof nm, "psi%d", d); put_f64(nm, p);
                 std::snprintf(nm, sizeof nm, "dpsi%d", d); put_f64(nm, dp);
+                // 2:1 mortar matrices, lower then upper half
+                std::vector<double> rf, cr;
+                for (int h = 0; h < 2; h++) {
+                    rf.insert(rf.end(), DG::psiRef[d * 2 + h], DG::psiRef[d * 2 + h] + n * n);
+                    cr.insert(cr.end(), DG::psiCor[d * 2 + h], DG::psiCor[d * 2 + h] + n * n);
+                }
+                std::snprintf(nm, sizeof nm, "psiRef%d", d); put_f64(nm, rf);
+                std::snprintf(nm, sizeof nm, "psiCor%d", d); put_f64(nm, cr);
             }
         }
     }

@@ -61,7 +61,28 @@ euler
 }
 """ % (max(case.get("check_steps", [1])), case.get("dt", 0.005), o[0], o[1], o[2],
        case.get("viscosity", 1.568e-5), case.get("write_format", "BINARY"),
-       "NO" if case.get("viscosity") is None else "YES")
+       "NO" if case.get("viscosity") is None else "YES") + refinement_controls(case)
+
+
+def refinement_controls(case):
+    """The prepare tool's refinement block (field.cpp:474-482) for AMR decks."""
+    r = case.get("refine")
+    if not r:
+        return ""
+    return """prepare
+{
+    fields    4 { U T p rho }
+}
+refinement
+{
+    direction       %d %d %d
+    field           %s
+    field_min       %r
+    field_max       %r
+    max_level       %d
+    limit           100000
+}
+""" % (tuple(r["direction"]) + (r["field"], r["field_min"], r["field_max"], r["max_level"]))
 
 
 def field_file(ncomp, gens, bcs):
@@ -173,6 +194,23 @@ EULER_CASES = [
     dict(name="e3_wall_n3", ndiv=(5, 4, 6), size=(1000, 800, 600), order=(3, 3, 3),
          spacing="wall 3{5 4 6} 3{2 3 0.5}", patches=PATCHES_3D, dt=0.002, check_steps=[2],
          T=["gaussian 0 0.5 500 400 300 200 200 200"]),
+]
+
+# 2:1 non-conforming meshes: the base block refined by the reference's
+# prepare tool (`prepare controls -refine`, once per pass, field.cpp:625-955)
+# around the bubble, two levels; the refined grid file is a committed
+# fixture (tests/golden/amr/<name>_grid_0.bin) and the initial fields are
+# the deck's generators evaluated on it, as the reference's euler does (it
+# reads <field>0.txt before the prepare tool's refined .bin, field.cpp:567).
+AMR_CASES = [
+    dict(name="amr3_n2", ndiv=(4, 4, 2), size=(1000, 1000, 500), order=(2, 2, 2),
+         patches=PATCHES_3D, dt=0.002, check_steps=[1, 2], keep_fields=True,
+         T=["gaussian 0 1.0 500 500 250 200 200 200"],
+         refine=dict(direction=(1, 1, 1), field="T", field_min=0.05, field_max=0.4, max_level=2, passes=2)),
+    dict(name="amr3_n3_visc", ndiv=(4, 3, 2), size=(1000, 750, 500), order=(3, 3, 3),
+         patches=PATCHES_3D, dt=0.002, check_steps=[1, 2], viscosity=1.5,
+         T=["gaussian 0 1.0 400 375 250 200 200 200"],
+         refine=dict(direction=(1, 1, 1), field="T", field_min=0.05, field_max=0.4, max_level=2, passes=2)),
 ]
 
 GEOM_CASES = [dict(c) for c in EULER_CASES] + [

@@ -27,11 +27,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from _cases import EULER_CASES, GEOM_CASES, POTENTIAL_CASES, write_euler_case, write_potential_case  # noqa: E402
+from _cases import (AMR_CASES, EULER_CASES, GEOM_CASES, POTENTIAL_CASES, write_euler_case,  # noqa: E402
+                    write_potential_case)
 from _fieldio import read_field_bin  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 FIELDS = os.path.join(HERE, "fields")
+AMR = os.path.join(HERE, "amr")
 GEOM_SKIP = {"header", "bnames"}
 
 
@@ -61,8 +63,9 @@ def run(cmd, cwd):
     return r.stdout
 
 
-def geometry_hashes(case, d, keep_arrays):
-    run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+def geometry_hashes(case, d, keep_arrays, make_grid=True):
+    if make_grid:
+        run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
     run([os.path.join(REF, "geomdump"), "controls", "geom.bin"], d)
     dump = read_dump(os.path.join(d, "geom.bin"))
     out = {"sha256": {}, "sizes": {}}
@@ -76,6 +79,35 @@ def geometry_hashes(case, d, keep_arrays):
     out["bnames"] = bytes(dump["bnames"].astype(np.uint8)).decode().split()
     arrays = {k: v for k, v in dump.items() if k in keep_arrays}
     return out, arrays
+
+
+def euler_record(case, d, arrays):
+    """Run the reference euler in d and hash its output fields."""
+    log = run([os.path.join(REF, "euler"), "controls"], d)
+    rec = {"steps": {}}
+    for s in case["check_steps"]:
+        rec["steps"][str(s)] = {}
+        for name in ("rho", "U", "T", "p"):
+            v = read_field_bin(os.path.join(d, f"{name}{s}.bin")).reshape(-1)
+            rec["steps"][str(s)][name] = {"sha256": sha(v), "l2": float(np.linalg.norm(v)),
+                                          "max": float(np.max(np.abs(v))), "n": int(v.size)}
+            if case.get("keep_fields"):
+                arrays[f"euler_{case['name']}_{name}{s}"] = v
+    rec["log_tail"] = log.strip().splitlines()[-3:]
+    return rec
+
+
+def refine(case, d):
+    """Base grid from the mesh tool, then the prepare tool's refinement
+    passes; the refined grid is kept as a fixture."""
+    run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+    out = []
+    for _ in range(case["refine"]["passes"]):
+        log = run([os.path.join(REF, "prepare"), "controls", "-refine", "-start", "0"], d)
+        out += [ln.strip() for ln in log.splitlines() if ln.startswith(("Level", "Refining "))]
+    os.makedirs(AMR, exist_ok=True)
+    shutil.copy(os.path.join(d, "grid_0.bin"), os.path.join(AMR, f"{case['name']}_grid_0.bin"))
+    return out
 
 
 def main():
@@ -127,6 +159,16 @@ def main():
                 run([os.path.join(REF, "mesh"), "spec", "-o", "grid_t.txt"], dt)
                 shutil.copy(os.path.join(dt, "grid_t.txt"), os.path.join(FIELDS, f"{case['name']}_grid_0.txt"))
             print("euler", case["name"], "ok")
+        for case in AMR_CASES:
+            d = os.path.join(tmp, "a_" + case["name"])
+            write_euler_case(d, case)
+            passes = refine(case, d)
+            fx, arr = geometry_hashes(case, d, keep_arrays=(), make_grid=False)
+            fx["refine_log"] = passes
+            fixtures["geometry"][case["name"]] = fx
+            rec = euler_record(case, d, arrays)
+            fixtures["euler"][case["name"]] = rec
+            print("amr", case["name"], passes, rec["log_tail"][-1])
         for case in POTENTIAL_CASES:
             d = os.path.join(tmp, "p_" + case["name"])
             write_potential_case(d, case)

@@ -6,11 +6,33 @@ import os
 import numpy as np
 import pytest
 
-from _cases import GEOM_CASES, write_euler_case
+from _cases import AMR_CASES, GEOM_CASES, write_euler_case
 from _golden import arrays, fixtures, sha
 from nebulasem_amd import Mesh
 
 FX = fixtures()["geometry"]
+AMR_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "amr")
+
+
+@pytest.mark.parametrize("case", AMR_CASES, ids=[c["name"] for c in AMR_CASES])
+def test_amr_geometry_bit_exact(case):
+    """A 2:1 non-conforming mesh (two levels refined by the reference's
+    prepare tool, grid file committed): LoadMesh's fix-ups with split cell
+    sides (merged outlines, mortar flags FMC), the DG node geometry and face
+    maps, and the 2:1 mortar matrices psiRef/psiCor agree bit for bit."""
+    mesh = Mesh.load(os.path.join(AMR_DIR, f"{case['name']}_grid_0.bin"), order=case["order"])
+    fx = FX[case["name"]]
+    assert mesh.boundaries == fx["bnames"]
+    bad = []
+    for name, h in fx["sha256"].items():
+        a = mesh.array(name)
+        if a.size != fx["sizes"][name] or sha(a) != h:
+            bad.append(name)
+    assert not bad, f"arrays differing from the reference: {bad}"
+    # mortar faces are present; the prepare tool numbers a coarse cell
+    # before its refined neighbours, so the coarse cell owns them (FMC 2)
+    fmc = mesh.array("FMC")
+    assert (fmc == 2).sum() > 0 and set(np.unique(fmc)) <= {0, 2}
 
 
 @pytest.mark.parametrize("case", GEOM_CASES, ids=[c["name"] for c in GEOM_CASES])
