@@ -1,11 +1,74 @@
 // C-ABI: reading the reference's field files (initial values + BCs).
+#include <cmath>
 #include <cstring>
 #include <string>
 
 #include "host/fields.hpp"
+#include "host/mesh.hpp"
 #include "mesh_access.hpp"
 #include "nebulasem_amd.h"
 #include "status.hpp"
+
+namespace {
+
+double dot_r(const double* p, const double* q) {  // right-nested, tensor.h:117
+    double r = p[2] * q[2] + 0.0;
+    r = p[1] * q[1] + r;
+    return p[0] * q[0] + r;
+}
+
+// The tangential part of p on a face of normal n, rescaled to |p|
+// (tensor.h:486-494).
+void reflect(const double* p, const double* n, double* r) {
+    const double mn = std::sqrt(dot_r(n, n));
+    const double en[3] = {n[0] / mn, n[1] / mn, n[2] / mn};
+    const double A[6] = {1 - en[0] * en[0], 1 - en[1] * en[1], 1 - en[2] * en[2],
+                         0 - en[0] * en[1], 0 - en[1] * en[2], 0 - en[0] * en[2]};
+    r[0] = A[0] * p[0] + A[3] * p[1] + A[5] * p[2];
+    r[1] = A[3] * p[0] + A[1] * p[1] + A[4] * p[2];
+    r[2] = A[5] * p[0] + A[4] * p[1] + A[2] * p[2];
+    const double mr = std::sqrt(dot_r(r, r));
+    if (std::fabs(mr) <= 1e-7) return;
+    const double s = std::sqrt(dot_r(p, p)) / mr;
+    for (int c = 0; c < 3; c++) r[c] *= s;
+}
+
+// Ghost nodes from the boundary conditions just read, as a field read does
+// (MeshField::read_, field.h:1557-1564, then applyExplicitBCs 2585-2727):
+// NEUMANN: owner + value |x_ghost - x_owner|; SYMMETRY: a vector reflected
+// about the face, a scalar copied; DIRICHLET: the value.
+void apply_explicit_bcs(const nsem::Geometry& g, const nsem::LoadedMesh& lm, int ncomp,
+                        const std::vector<nsem::BoundaryCondition>& bcs, double* v) {
+    for (const auto& bc : bcs) {
+        auto it = lm.boundaries.find(bc.patch);
+        if (it == lm.boundaries.end()) continue;
+        for (nsem::Int face : it->second)
+            for (nsem::Int n = 0; n < g.npf; n++) {
+                const uint64_t k = uint64_t(face) * g.npf + n;
+                const uint64_t c1 = g.FO[k], c2 = g.FN[k];
+                if (c2 >= g.nall_field) continue;
+                double* dst = v + c2 * ncomp;
+                const double* src = v + c1 * ncomp;
+                if (bc.type == "NEUMANN") {
+                    double dv[3];
+                    for (int c = 0; c < 3; c++) dv[c] = g.cC[c2][c] - g.cC[c1][c];
+                    const double md = std::sqrt(dot_r(dv, dv));
+                    for (int c = 0; c < ncomp; c++) dst[c] = src[c] + bc.value[c] * md;
+                } else if (bc.type == "SYMMETRY") {
+                    if (ncomp == 3) {
+                        const double nv[3] = {g.fN[k][0], g.fN[k][1], g.fN[k][2]};
+                        reflect(src, nv, dst);
+                    } else {
+                        for (int c = 0; c < ncomp; c++) dst[c] = src[c];
+                    }
+                } else if (bc.type == "DIRICHLET") {
+                    for (int c = 0; c < ncomp; c++) dst[c] = bc.value[c];
+                }
+            }
+    }
+}
+
+}  // namespace
 
 extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp,
                                const double gravity[3], double* values, int32_t* bc_kind,
@@ -26,7 +89,6 @@ extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t nco
     nsem::FieldData fd;
     std::string err;
     if (!nsem::read_field_file(path, ctx, ncomp, fd, &err)) return nsem_fail(NSEM_EIO, err);
-    std::memcpy(values, fd.values.data(), sizeof(double) * fd.values.size());
     const std::vector<std::string>& names = nsem_mesh_boundary_names(m);
     for (size_t b = 0; b < names.size(); b++) {
         bc_kind[b] = -1;
@@ -40,6 +102,8 @@ extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t nco
             for (int c = 0; c < ncomp; c++) bc_value[b * ncomp + c] = bc.value[c];
         }
     }
+    apply_explicit_bcs(g, nsem_mesh_loaded(m), ncomp, fd.bcs, fd.values.data());
+    std::memcpy(values, fd.values.data(), sizeof(double) * fd.values.size());
     return NSEM_OK;
 }
 

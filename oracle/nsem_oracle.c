@@ -55,6 +55,11 @@ typedef struct {
     const uint32_t *allFaces, *fi0, *fi1;
     int32_t nbnd;                  /* boundary patches (name order)       */
     const uint32_t *boff, *bfaces;
+    /* 2:1 non-conforming faces; FMC may be null (conforming mesh)        */
+    const uint32_t *FMC, *FOC, *FNC;             /* nfacets                */
+    const uint32_t *cell_off, *cell_faces, *faceid; /* cell side lists    */
+    const double *gFC;                           /* nfacets*3 centres      */
+    const double *psiRef[3], *psiCor[3];         /* [2][n*n] per direction */
 } om_mesh;
 
 enum { OM_NEUMANN = 0, OM_SYMMETRY = 1, OM_DIRICHLET = 2 };
@@ -179,23 +184,171 @@ static void fill_bcs(const om_mesh* m, double* f, int nc) {
  * (rows=1: vector flux of a scalar equation, rows=3: tensor flux), with
  * Rusanov face flux using q (rows components) and lambda per facet node.
  * r has naf*rows entries.  field.h:3416-3464. */
+/* ---- 2:1 non-conforming faces (field.h:2018-2248) ----------------------
+ * A mortar face fi (FMC 1 or 2) is one of the faces a coarse cell cn has on
+ * one side; its other cell is a refined one.  FMC 2: cn is the face's owner,
+ * FMC 1: its neighbour.  The coarse side's values at fi's nodes are the
+ * coarse side's nodes interpolated to the half of the side fi covers
+ * (psiRef, scatter); a face field is brought back to the coarse side's node
+ * positions by psiCor (gather).  The half is found by comparing fi's centre
+ * with the side's centre (mean of its faces' centres) along the side's two
+ * in-plane node directions. */
+typedef struct { uint32_t cn; int fid, h0, h1; } om_mortar;
+
+static uint64_t node_of(const om_mesh* m, uint32_t c, int i, int j, int k) {
+    return (uint64_t)c * m->np + (uint64_t)i * m->npy * m->npz + (uint64_t)j * m->npz + k;
+}
+
+static int mortar_of(const om_mesh* m, uint32_t fi, om_mortar* mt) {
+    const uint32_t fm = m->FMC ? m->FMC[fi] : 0;
+    if (fm < 1) return 0;
+    const uint32_t cn = fm == 1 ? m->FNC[fi] : m->FOC[fi];
+    int fid = -1;
+    for (uint32_t j = m->cell_off[cn]; j < m->cell_off[cn + 1]; j++)
+        if (m->cell_faces[j] == fi) { fid = (int)m->faceid[j]; break; }
+    double ccn[3] = {0, 0, 0};
+    int nch = 0;
+    for (uint32_t j = m->cell_off[cn]; j < m->cell_off[cn + 1]; j++)
+        if ((int)m->faceid[j] == fid) {
+            for (int c = 0; c < 3; c++) ccn[c] += m->gFC[3 * (uint64_t)m->cell_faces[j] + c];
+            nch++;
+        }
+    for (int c = 0; c < 3; c++) ccn[c] /= (double)nch;
+    const double* cco = &m->gFC[3 * (uint64_t)fi];
+    const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
+    uint64_t v0, va, vb;
+    if (fid == 0 || fid == 1) {
+        const int k = fid == 0 ? 0 : NPZ - 1;
+        v0 = node_of(m, cn, 0, 0, k); va = node_of(m, cn, NPX - 1, 0, k); vb = node_of(m, cn, 0, NPY - 1, k);
+    } else if (fid == 2 || fid == 3) {
+        const int j = fid == 2 ? 0 : NPY - 1;
+        v0 = node_of(m, cn, 0, j, 0); va = node_of(m, cn, NPX - 1, j, 0); vb = node_of(m, cn, 0, j, NPZ - 1);
+    } else {
+        const int i = fid == 4 ? 0 : NPX - 1;
+        v0 = node_of(m, cn, i, 0, 0); va = node_of(m, cn, i, NPY - 1, 0); vb = node_of(m, cn, i, 0, NPZ - 1);
+    }
+    const double *p0 = &m->cC[3 * v0], *pa = &m->cC[3 * va], *pb = &m->cC[3 * vb];
+    double dn[3], dO[3], ea[3], eb[3];
+    for (int c = 0; c < 3; c++) {
+        dn[c] = ccn[c] - p0[c]; dO[c] = cco[c] - p0[c];
+        ea[c] = pa[c] - p0[c]; eb[c] = pb[c] - p0[c];
+    }
+    mt->cn = cn;
+    mt->fid = fid;
+    mt->h0 = dot3(dn, ea) >= dot3(dO, ea) ? 0 : 1;
+    mt->h1 = dot3(dn, eb) >= dot3(dO, eb) ? 0 : 1;
+    return (int)fm;
+}
+
+/* The side's in-plane directions (d0, d1) and the face-node index of node
+ * (a, b) on it, for side id fid. */
+static void side_dirs(const om_mesh* m, int fid, int* d0, int* d1) {
+    if (fid == 0 || fid == 1) { *d0 = 0; *d1 = 1; }
+    else if (fid == 2 || fid == 3) { *d0 = 0; *d1 = 2; }
+    else { *d0 = 1; *d1 = 2; }
+    (void)m;
+}
+static int npd(const om_mesh* m, int d) { return d == 0 ? m->npx : d == 1 ? m->npy : m->npz; }
+static uint64_t side_node(const om_mesh* m, const om_mortar* mt, int a, int b) {
+    const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
+    if (mt->fid == 0 || mt->fid == 1) return node_of(m, mt->cn, a, b, mt->fid == 0 ? 0 : NPZ - 1);
+    if (mt->fid == 2 || mt->fid == 3) return node_of(m, mt->cn, a, mt->fid == 2 ? 0 : NPY - 1, b);
+    return node_of(m, mt->cn, mt->fid == 4 ? 0 : NPX - 1, a, b);
+}
+
+/* Owner- and neighbour-side values of a node field (w doubles per node) at
+ * every face node; the coarse side of a mortar face interpolated
+ * (scatter_non_conforming, field.h:2134-2248). */
+static void scatter_nc(const om_mesh* m, const double* f, int w, double* fO, double* fN) {
+    const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    for (uint64_t k = 0; k < nfn; k++) {
+        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        for (int c = 0; c < w; c++) {
+            fO[k * w + c] = c1 < m->naf ? f[(uint64_t)c1 * w + c] : 0;
+            fN[k * w + c] = c2 < m->naf ? f[(uint64_t)c2 * w + c] : 0;
+        }
+    }
+    if (!m->FMC || m->np <= 1) return;  /* FV: no mortars (NPMAT = 0) */
+    for (uint32_t fi = 0; fi < m->nfacets; fi++) {
+        om_mortar mt;
+        const int fm = mortar_of(m, fi, &mt);
+        if (!fm) continue;
+        double* side = fm == 1 ? fN : fO;
+        int d0, d1;
+        side_dirs(m, mt.fid, &d0, &d1);
+        const int n0 = npd(m, d0), n1 = npd(m, d1);
+        const double* R0 = m->psiRef[d0] + (uint64_t)mt.h0 * n0 * n0;
+        const double* R1 = m->psiRef[d1] + (uint64_t)mt.h1 * n1 * n1;
+        for (int ao = 0; ao < n0; ao++)
+            for (int bo = 0; bo < n1; bo++) {
+                const uint64_t out = (uint64_t)fi * m->npf + ao * n1 + bo;
+                for (int c = 0; c < w; c++) {
+                    double sum = 0.0;
+                    for (int an = 0; an < n0; an++)
+                        for (int bn = 0; bn < n1; bn++)
+                            sum += f[side_node(m, &mt, an, bn) * w + c] * R0[an * n0 + ao] * R1[bn * n1 + bo];
+                    side[out * w + c] = sum;
+                }
+            }
+    }
+}
+
+/* Owner- and neighbour-side copies of a face field (w doubles per face
+ * node); the coarse side of a mortar face projected back to the coarse
+ * side's node positions (gather_non_conforming, field.h:2018-2132). */
+static void gather_nc(const om_mesh* m, const double* fF, int w, double* fO, double* fN) {
+    const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    memcpy(fO, fF, sizeof(double) * nfn * w);
+    memcpy(fN, fF, sizeof(double) * nfn * w);
+    if (!m->FMC || m->np <= 1) return;  /* FV: no mortars (NPMAT = 0) */
+    for (uint32_t fi = 0; fi < m->nfacets; fi++) {
+        om_mortar mt;
+        const int fm = mortar_of(m, fi, &mt);
+        if (!fm) continue;
+        double* side = fm == 1 ? fN : fO;
+        int d0, d1;
+        side_dirs(m, mt.fid, &d0, &d1);
+        const int n0 = npd(m, d0), n1 = npd(m, d1);
+        const double* C0 = m->psiCor[d0] + (uint64_t)mt.h0 * n0 * n0;
+        const double* C1 = m->psiCor[d1] + (uint64_t)mt.h1 * n1 * n1;
+        for (int an = 0; an < n0; an++)
+            for (int bn = 0; bn < n1; bn++) {
+                const uint64_t out = (uint64_t)fi * m->npf + an * n1 + bn;
+                for (int c = 0; c < w; c++) {
+                    double sum = 0.0;
+                    for (int ao = 0; ao < n0; ao++)
+                        for (int bo = 0; bo < n1; bo++)
+                            sum += fF[((uint64_t)fi * m->npf + ao * n1 + bo) * w + c] * C0[ao * n0 + an] *
+                                   C1[bo * n1 + bn];
+                    side[out * w + c] = sum;
+                }
+            }
+    }
+}
+
 static void divf_dg(const om_mesh* m, int rows, const double* F, const double* q,
                     const double* lam, double* r) {
     const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
-    double* fF = (double*)calloc(nfn * rows * 3, sizeof(double));
+    const int w = rows * 3;
+    double* fF = (double*)calloc(nfn * w, sizeof(double));
+    double* FOs = (double*)malloc(sizeof(double) * nfn * w);
+    double* FNs = (double*)malloc(sizeof(double) * nfn * w);
+    double* qO = (double*)malloc(sizeof(double) * nfn * rows);
+    double* qN = (double*)malloc(sizeof(double) * nfn * rows);
+    scatter_nc(m, F, w, FOs, FNs);
+    scatter_nc(m, q, rows, qO, qN);
     /* rusanov = cds(F) - unit(fN) (x) lambda (qN - qO) */
     for (uint64_t k = 0; k < nfn; k++) {
-        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        uint32_t c2 = m->FN[k];
         if (c2 >= m->naf) continue;
         const double fi = m->fI[k];
         const double* nv = &m->fN[3 * k];
         double mn = sqrt(dot3(nv, nv));
         double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
         for (int rr = 0; rr < rows; rr++) {
-            double dq = lam[k] * (q[(uint64_t)c2 * rows + rr] - q[(uint64_t)c1 * rows + rr]);
+            double dq = lam[k] * (qN[k * rows + rr] - qO[k * rows + rr]);
             for (int c = 0; c < 3; c++) {
-                double v = F[((uint64_t)c1 * rows + rr) * 3 + c] * fi +
-                           F[((uint64_t)c2 * rows + rr) * 3 + c] * (1 - fi);
+                double v = FOs[k * w + rr * 3 + c] * fi + FNs[k * w + rr * 3 + c] * (1 - fi);
                 if (rows == 1) v = v - en[c] * dq;
                 fF[(k * rows + rr) * 3 + c] = v;
             }
@@ -203,12 +356,16 @@ static void divf_dg(const om_mesh* m, int rows, const double* F, const double* q
         if (rows == 3) { /* tensor flux: row r gets en_r * (lambda dq_c) in column c */
             for (int rr = 0; rr < 3; rr++)
                 for (int c = 0; c < 3; c++) {
-                    double dq = lam[k] * (q[(uint64_t)c2 * 3 + c] - q[(uint64_t)c1 * 3 + c]);
+                    double dq = lam[k] * (qN[k * 3 + c] - qO[k * 3 + c]);
                     fF[(k * 3 + rr) * 3 + c] -= en[rr] * dq;
                 }
         }
     }
-    /* surface integral, cell by cell (div_flux<false>) */
+    /* surface integral, cell by cell (div_flux<false>), the coarse side of a
+     * mortar face taking the gathered flux */
+    double* fFO = FOs;
+    double* fFN = FNs;
+    gather_nc(m, fF, w, fFO, fFN);
     memset(r, 0, sizeof(double) * m->naf * rows);
     for (uint32_t i = 0; i < m->ncells; i++)
         for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++)
@@ -219,13 +376,17 @@ static void divf_dg(const om_mesh* m, int rows, const double* F, const double* q
                 int own = (c1 >= (uint64_t)i * m->np && c1 < (uint64_t)(i + 1) * m->np);
                 if (!own && c2 >= m->naf) continue;
                 for (int rr = 0; rr < rows; rr++) {
-                    const double* v = &fF[(k * rows + rr) * 3];
+                    const double* v = own ? &fFO[(k * rows + rr) * 3] : &fFN[(k * rows + rr) * 3];
                     double d = rows == 1 ? dot3(v, nv) : v[0] * nv[0] + v[1] * nv[1] + v[2] * nv[2];
                     if (own) r[(uint64_t)c1 * rows + rr] += d;
                     else r[(uint64_t)c2 * rows + rr] -= d;
                 }
             }
     free(fF);
+    free(FOs);
+    free(FNs);
+    free(qO);
+    free(qN);
     /* volume integral (weak form) */
     const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
     for (uint32_t ci = 0; ci < m->nbcs; ci++)
@@ -261,14 +422,19 @@ static void divf_dg(const om_mesh* m, int rows, const double* F, const double* q
     fill_bcs(m, r, rows);
 }
 
-/* cds of a node scalar at every facet node */
+/* cds of a node scalar at every facet node (field.h:2880-2893) */
 static void cds_scalar(const om_mesh* m, const double* f, double* out) {
     const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
+    double* fO = (double*)malloc(sizeof(double) * nfn);
+    double* fN = (double*)malloc(sizeof(double) * nfn);
+    scatter_nc(m, f, 1, fO, fN);
     for (uint64_t k = 0; k < nfn; k++) {
-        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        uint32_t c2 = m->FN[k];
         if (c2 >= m->naf) { out[k] = 0; continue; }
-        out[k] = f[c1] * m->fI[k] + f[c2] * (1 - m->fI[k]);
+        out[k] = fO[k] * m->fI[k] + fN[k] * (1 - m->fI[k]);
     }
+    free(fO);
+    free(fN);
 }
 
 /* Strong-form DG gradient of a node field with `rows` components,
@@ -283,12 +449,17 @@ static void gradf_strong(const om_mesh* m, int rows, const double* p, const int3
                          const double* val, double* G) {
     const uint64_t nfn = (uint64_t)m->nfacets * m->npf;
     double* fF = (double*)calloc(nfn * rows, sizeof(double));
+    double* fO = (double*)malloc(sizeof(double) * nfn * rows);
+    double* fNs = (double*)malloc(sizeof(double) * nfn * rows);
+    scatter_nc(m, p, rows, fO, fNs);
     for (uint64_t k = 0; k < nfn; k++) {
-        uint32_t c1 = m->FO[k], c2 = m->FN[k];
+        uint32_t c2 = m->FN[k];
         if (c2 >= m->naf) continue;
         for (int b = 0; b < rows; b++)
-            fF[k * rows + b] = p[(uint64_t)c1 * rows + b] * m->fI[k] + p[(uint64_t)c2 * rows + b] * (1 - m->fI[k]);
+            fF[k * rows + b] = fO[k * rows + b] * m->fI[k] + fNs[k * rows + b] * (1 - m->fI[k]);
     }
+    /* grad_flux<true> reads the gathered face values (field.h:3050-3082) */
+    gather_nc(m, fF, rows, fO, fNs);
     memset(G, 0, sizeof(double) * m->naf * 3 * rows);
     for (uint32_t i = 0; i < m->ncells; i++)
         for (uint32_t f = m->fi0[i]; f < m->fi1[i]; f++)
@@ -298,17 +469,19 @@ static void gradf_strong(const om_mesh* m, int rows, const double* p, const int3
                 const double* nv = &m->fN[3 * k];
                 if (c1 >= (uint64_t)i * m->np && c1 < (uint64_t)(i + 1) * m->np) {
                     for (int b = 0; b < rows; b++) {
-                        const double dp = fF[k * rows + b] - p[(uint64_t)c1 * rows + b];
+                        const double dp = fO[k * rows + b] - p[(uint64_t)c1 * rows + b];
                         for (int a = 0; a < 3; a++) G[((uint64_t)c1 * 3 + a) * rows + b] += nv[a] * dp;
                     }
                 } else if (c2 < m->naf) {
                     for (int b = 0; b < rows; b++) {
-                        const double dp = fF[k * rows + b] - p[(uint64_t)c2 * rows + b];
+                        const double dp = fNs[k * rows + b] - p[(uint64_t)c2 * rows + b];
                         for (int a = 0; a < 3; a++) G[((uint64_t)c2 * 3 + a) * rows + b] -= nv[a] * dp;
                     }
                 }
             }
     free(fF);
+    free(fO);
+    free(fNs);
     const int NPX = m->npx, NPY = m->npy, NPZ = m->npz;
     for (uint32_t ci = 0; ci < m->nbcs; ci++)
         for (int ii = 0; ii < NPX; ii++)

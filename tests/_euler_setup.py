@@ -1,6 +1,7 @@
 """Shared set-up of an Euler case: product mesh + product field reader,
 initial state as the reference's euler app builds it."""
 import os
+import shutil
 
 import numpy as np
 
@@ -10,13 +11,28 @@ from nebulasem_amd import Mesh, read_field
 GRAVITY = (0.0, 0.0, -9.80606)
 
 
+AMR_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "amr")
+
+
 def load_case(tmpdir, case):
+    """The deck's mesh (the grid generator, or for an AMR deck the committed
+    refined grid) and its initial fields."""
     d = os.path.join(str(tmpdir), case["name"])
     write_euler_case(d, case)
-    mesh = Mesh.from_spec(os.path.join(d, "spec"), order=case["order"])
+    if case.get("refine"):
+        # the refined grid and the prepare tool's refined temperature, which
+        # the reference's euler reads before the deck's T0.txt
+        mesh = Mesh.load(os.path.join(AMR_DIR, f"{case['name']}_grid_0.bin"), order=case["order"])
+        shutil.copy(os.path.join(AMR_DIR, f"{case['name']}_T0.bin"), os.path.join(d, "T0.bin"))
+        os.remove(os.path.join(d, "T0.txt"))
+    else:
+        mesh = Mesh.from_spec(os.path.join(d, "spec"), order=case["order"])
     f = {}
     for name, nc in (("T", 1), ("U", 3), ("p", 1), ("rho", 1)):
-        f[name] = read_field(mesh, os.path.join(d, name + "0.txt"), nc, GRAVITY)
+        path = os.path.join(d, name + "0.bin")
+        if not os.path.exists(path):
+            path = os.path.join(d, name + "0.txt")
+        f[name] = read_field(mesh, path, nc, GRAVITY)
     return mesh, f
 
 

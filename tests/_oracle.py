@@ -69,7 +69,9 @@ class OmMesh(C.Structure):
                 ("FO", PU), ("FN", PU), ("fN", PD), ("fI", PD), ("fD", PD), ("cV", PD),
                 ("cC", PD), ("Jinv", PD), ("psi", PD * 3), ("dpsi", PD * 3),
                 ("allFaces", PU), ("fi0", PU), ("fi1", PU), ("nbnd", C.c_int32),
-                ("boff", PU), ("bfaces", PU)]
+                ("boff", PU), ("bfaces", PU),
+                ("FMC", PU), ("FOC", PU), ("FNC", PU), ("cell_off", PU), ("cell_faces", PU),
+                ("faceid", PU), ("gFC", PD), ("psiRef", PD * 3), ("psiCor", PD * 3)]
 
 
 class OmEuler(C.Structure):
@@ -114,6 +116,19 @@ class OracleMesh:
         s.allFaces, s.fi0, s.fi1 = _p(k["allFaces"], PU), _p(k["faceIndices0"], PU), _p(k["faceIndices1"], PU)
         s.nbnd = len(mesh.boundaries)
         s.boff, s.bfaces = _p(k["boff"], PU), _p(k["bfaces"], PU)
+        # 2:1 non-conforming faces: the mortar projections run only where a
+        # face is flagged (FMC 1 or 2)
+        for name in ("FMC", "FOC", "FNC", "cell_off", "cell_faces", "faceid", "gFC"):
+            k[name] = np.ascontiguousarray(mesh.array(name))
+        if inf.np > 1 and k["FMC"].any():
+            s.FMC, s.FOC, s.FNC = _p(k["FMC"], PU), _p(k["FOC"], PU), _p(k["FNC"], PU)
+            s.cell_off, s.cell_faces, s.faceid = _p(k["cell_off"], PU), _p(k["cell_faces"], PU), _p(k["faceid"], PU)
+            s.gFC = _p(k["gFC"], PD)
+            for d in range(3):
+                k[f"psiRef{d}"] = np.ascontiguousarray(mesh.array(f"psiRef{d}"))
+                k[f"psiCor{d}"] = np.ascontiguousarray(mesh.array(f"psiCor{d}"))
+                s.psiRef[d] = _p(k[f"psiRef{d}"], PD)
+                s.psiCor[d] = _p(k[f"psiCor{d}"], PD)
         self.s = s
         self.info = inf
 

@@ -107,6 +107,10 @@ def refine(case, d):
         out += [ln.strip() for ln in log.splitlines() if ln.startswith(("Level", "Refining "))]
     os.makedirs(AMR, exist_ok=True)
     shutil.copy(os.path.join(d, "grid_0.bin"), os.path.join(AMR, f"{case['name']}_grid_0.bin"))
+    # the prepare tool's refined temperature (refineField), which euler
+    # reads in preference to the deck's T0.txt (field.h:1579-1584); the
+    # other fields are uniform zeros, the same either way
+    shutil.copy(os.path.join(d, "T0.bin"), os.path.join(AMR, f"{case['name']}_T0.bin"))
     return out
 
 

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from _cases import EULER_CASES, POTENTIAL_CASES, write_potential_case
+from _cases import AMR_CASES, EULER_CASES, POTENTIAL_CASES, write_potential_case
 from _euler_setup import bc_table, load_case
 from _golden import arrays, fixtures, sha
 from nebulasem_amd import Mesh, read_field
@@ -26,7 +26,7 @@ def run_oracle_euler(oracle, tmp_path, case):
     return mesh, om, prm, p_ref, rho_ref, rho, U, T, p
 
 
-@pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
+@pytest.mark.parametrize("case", EULER_CASES + AMR_CASES, ids=[c["name"] for c in EULER_CASES + AMR_CASES])
 def test_oracle_euler_matches_reference(oracle, tmp_path, case):
     mesh, om, prm, p_ref, rho_ref, rho, U, T, p = run_oracle_euler(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field
