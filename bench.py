@@ -124,6 +124,63 @@ def setup_dg(ndiv, size, order, dt, seed, tmp):
     return mesh, dg, (rho, U, T, p)
 
 
+def amr_bench(steps=20, dt=0.002):
+    """3D AMR dGSEM Euler: a 32x32x16 hex block over 10x10x5 km, N=5, refined
+    twice (2:1) around a 1 K bubble (centre 5000 5000 2000 m, radius
+    1500 m) by the reference's prepare tool (the grid is committed,
+    xz-compressed: 47,380 elements on three levels); initial temperature the
+    deck's Gaussian evaluated on the refined nodes.  FP32-perturbation and
+    FP64 SSP-RK3, mortar pre-pass included in every stage."""
+    import lzma
+    import shutil
+    from nebulasem_amd import DGEuler, Mesh, euler_params, read_field
+    src = os.path.join(ROOT, "tests", "golden", "amr", "amr3_32x32x16_n5_grid_0.bin.xz")
+    with tempfile.TemporaryDirectory() as tmp:
+        grid = os.path.join(tmp, "grid_0.bin")
+        with lzma.open(src) as fi, open(grid, "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+        t = time.time()
+        mesh = Mesh.load(grid, order=(5, 5, 5))
+        log(f"amr mesh: {mesh.info.nbcs_field} nodes, {time.time() - t:.1f}s")
+        names = mesh.boundaries
+        g = (0.0, 0.0, -9.80606)
+        field_file(os.path.join(tmp, "T0.txt"), 1, ["gaussian 0 1.0 5000 5000 2000 1500 1500 1500"], names,
+                   "NEUMANN")
+        field_file(os.path.join(tmp, "U0.txt"), 3, ["uniform 0 0 0"], names, "SYMMETRY")
+        field_file(os.path.join(tmp, "p0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        field_file(os.path.join(tmp, "rho0.txt"), 1, ["uniform 0"], names, "NEUMANN")
+        f = {k: read_field(mesh, os.path.join(tmp, k + "0.txt"), nc, g)
+             for k, nc in (("T", 1), ("U", 3), ("p", 1), ("rho", 1))}
+    nb = mesh.info.nbcs_field
+    fmc = mesh.array("FMC")
+    dg = DGEuler.from_fields(mesh, euler_params(dt), f)
+    T = np.ascontiguousarray(f["T"][0][:nb, 0])
+    p = np.ascontiguousarray(f["p"][0][:nb, 0])
+    U = np.ascontiguousarray(f["U"][0][:nb])
+    rho = dg.initial_density(T, p)
+    del f
+    out = {"workload": "3D AMR dGSEM Euler, 32x32x16 base hex over 10x10x5 km refined twice (2:1) around "
+                       "a 1 K bubble, N=5, SSP-RK3 dt=%g" % dt,
+           "elements": nb // 216, "nodes": nb, "dof": NQ * nb, "mortar_faces": int((fmc > 0).sum()),
+           "steps": steps, "stage_kernel": "rk_stage_kernel<R,S,0,0,0,DIN,AFF,false,true> + mortar pre-pass"}
+    q = {}
+    for prec in (32, 64):
+        dg.set_state(rho, U, T, p)
+        dg.rk3_steps(2, prec)
+        dg.synchronize()
+        dg.set_state(rho, U, T, p)
+        sec = dg.time_rk3(steps, prec)
+        q[prec] = dg.get_conserved()
+        out[f"fp{prec}"] = {"value": NQ * nb * 3 * steps / sec, "unit": "DOF*stage/s",
+                            "ms_per_step": sec / steps * 1e3}
+    out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["fp32_vs_fp64_rel_l2"] = {nm: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+                                  for nm, a, b in zip(("rho_p", "m", "Theta_p"), q[32], q[64])}
+    out["adjoint"] = "not carried on 2:1 non-conforming meshes by this build"
+    dg.close()
+    return out
+
+
 def adjoint_bench(nsteps=50, checkpoint=10, dt=0.001):
     """Discrete adjoint of the 2D bubble: 64x64 elements (1 thick in y) over
     1000 x 1000 m, N = 5, Gaussian dtheta = 0.5 K (radius 250 m, centre
@@ -378,6 +435,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fv", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--fv-n", type=int, default=256)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only time the plain copy kernel over MIB MiB (counter calibration)")
@@ -454,6 +512,7 @@ def main():
                 "stream_copy_GBps": stream_bw / 1e9,
                 "frac_of_stream_copy": achieved / stream_bw}
     adj = None if args.no_adjoint else adjoint_bench()
+    amr = None if args.no_amr else amr_bench()
     fv = None
     if not args.no_fv:
         fv = fv_bench(args.fv_n)
@@ -490,6 +549,7 @@ def main():
         "roofline": roofline,
         "pcg": fv,
         "adjoint": adj,
+        "amr": amr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -22,10 +22,25 @@ constexpr int kMaxN = 16;  // LGL points per direction supported
 
 // meta bits: [0:3) other face group, [3] self is owner, [4] face present,
 // [5] other is a ghost (boundary), [8:24) boundary patch index
+// meta bits: 0-2 the other side's face group, 3 this element owns the face,
+// 4 present, 5 boundary (ghost), 6 mortar side; bits 8+: the boundary
+// patch, or for a mortar side its slot in DgDev::msurf.
 struct EFace {
     uint32_t facet;
     uint32_t other;
     uint32_t meta;
+};
+
+// A 2:1 non-conforming face: a refined element's face lying on a split side
+// of a coarse element (field.h:2018-2248).
+struct MortarFace {
+    uint32_t coarse;    // the coarse element
+    int32_t side;       // its side (face group) holding the face
+    int32_t h0, h1;     // the half of the side the face covers, along the side's two node directions
+    int32_t fine_owner; // 1: the refined element owns the face (FMC 1), 0: the coarse one (FMC 2)
+    int32_t n0, n1;     // node extents of the side
+    int32_t pad;
+    double nf[3];       // the face's area normal, pointing out of the owner
 };
 
 enum BcKind : int32_t { BC_NEUMANN = 0, BC_SYMMETRY = 1, BC_DIRICHLET = 2 };
@@ -68,6 +83,25 @@ struct DgDev {
     int affine;
     const double* emet;
     const double* wnode;
+    // 2:1 non-conforming faces (nmort of them, sides of up to npfm nodes):
+    // per face node, the refined side's node (mort_fine), the entry of that
+    // node in msurf (mort_fms), the face-node weight w_a w_b / 4 (mort_w) and
+    // the background density interpolated from the coarse side minus the
+    // refined side's (mort_bg); psiRef / psiCor [2][n*n] per direction.  A
+    // pre-pass of every stage writes the face flux per unit weight (mphi)
+    // and the surface terms of every mortar side (msurf [slot][npfm][5]);
+    // cslot [ncslot][8] = element, side, children, slot, child faces (4).
+    int nmort, npfm, ncslot;
+    const MortarFace* mort;
+    const uint64_t* mort_fine;
+    const uint32_t* mort_fms;
+    const double* mort_w;
+    const double* mort_bg;
+    const double* psiRef[3];
+    const double* psiCor[3];
+    const uint32_t* cslot;
+    double* mphi;
+    double* msurf;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

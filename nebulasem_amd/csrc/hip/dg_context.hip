@@ -87,8 +87,12 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             b++;
         }
     }
-    // element face table
+    // element face table; a side holding a 2:1 mortar face (FMC 1 or 2) --
+    // the refined element's face, or a coarse element's split side -- gets
+    // a slot in the mortar surface-term buffer
     std::vector<EFace> ef((size_t)g.nbcs * 6);
+    std::vector<int64_t> side_slot((size_t)g.nbcs * 6, -1);
+    uint32_t nslots = 0;
     for (Int ci = 0; ci < g.nbcs; ci++) {
         const IntVec& c = lm.cells[ci];
         for (size_t mm = 0; mm < c.size(); mm++) {
@@ -96,7 +100,18 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             Int grp = lm.faceid[ci][mm];
             if (grp > 5) return nsem_fail(NSEM_EUNSUPPORTED, "bad face group");
             EFace& e = ef[(size_t)ci * 6 + grp];
-            if ((e.meta >> 4) & 1) return nsem_fail(NSEM_EUNSUPPORTED, "non-conforming faces are not supported yet");
+            if (lm.FMC[fi] > 0) {
+                if (side_slot[(size_t)ci * 6 + grp] < 0) {
+                    if (nslots >= (1u << 24)) return nsem_fail(NSEM_EUNSUPPORTED, "too many mortar sides");
+                    side_slot[(size_t)ci * 6 + grp] = nslots;
+                    e.facet = fi;
+                    e.other = ci;
+                    e.meta = 16u | 32u | 64u | (nslots << 8);
+                    nslots++;
+                }
+                continue;
+            }
+            if ((e.meta >> 4) & 1) return nsem_fail(NSEM_EUNSUPPORTED, "a cell side with several conforming faces");
             bool own = lm.FOC[fi] == ci;
             Int other = own ? lm.FNC[fi] : lm.FOC[fi];
             Int og = grp ^ 1;
@@ -126,6 +141,124 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             e.other = other;
             e.meta = og | (own ? 8u : 0u) | 16u | (ghost ? 32u : 0u) |
                      ((uint32_t)(ghost ? patch_of[fi] : 0) << 8);
+        }
+    }
+    // 2:1 mortar faces (field.h:2157-2183): the coarse element and its side,
+    // the half of that side the face covers (its centre against the side's
+    // centre along the side's node directions), the refined side's node and
+    // surface-term entry per face node
+    const int NPF = g.npf;
+    const int npfm = std::max(g.npx * g.npy, std::max(g.npx * g.npz, g.npy * g.npz));
+    std::vector<MortarFace> mfaces;
+    std::vector<Int> mface_of;
+    std::vector<uint64_t> mfine;
+    std::vector<uint32_t> mfms;
+    std::vector<double> mw;
+    for (Int fi = 0; fi < g.nfacets; fi++) {
+        const Int fm = lm.FMC[fi];
+        if (fm < 1) continue;
+        if (g.np <= 1) return nsem_fail(NSEM_EUNSUPPORTED, "mortar faces on a finite-volume mesh");
+        MortarFace mf{};
+        const Int cn = fm == 1 ? lm.FNC[fi] : lm.FOC[fi];
+        const Int cf = fm == 1 ? lm.FOC[fi] : lm.FNC[fi];
+        if (cn >= g.nbcs || cf >= g.nbcs) return nsem_fail(NSEM_EINVAL, "mortar face on the boundary");
+        auto side_of = [&](Int cell, Int face) -> int {
+            const IntVec& c = lm.cells[cell];
+            for (size_t t = 0; t < c.size(); t++) if (c[t] == face) return (int)lm.faceid[cell][t];
+            return -1;
+        };
+        const int fid = side_of(cn, fi), gf = side_of(cf, fi);
+        if (fid < 0 || gf < 0) return nsem_fail(NSEM_EINVAL, "inconsistent mortar connectivity");
+        double ccn[3] = {0, 0, 0};
+        int nch = 0;
+        {
+            const IntVec& c = lm.cells[cn];
+            for (size_t t = 0; t < c.size(); t++)
+                if ((int)lm.faceid[cn][t] == fid) {
+                    for (int q = 0; q < 3; q++) ccn[q] += lm.fc[c[t]][q];
+                    nch++;
+                }
+            for (int q = 0; q < 3; q++) ccn[q] /= (double)nch;
+        }
+        const int n0 = fid < 2 ? g.npx : fid < 4 ? g.npx : g.npy;
+        const int n1 = fid < 2 ? g.npy : g.npz;
+        auto node = [&](Int cell, int side, int a, int b) {
+            const int v = (side & 1) ? (side < 2 ? g.npz - 1 : side < 4 ? g.npy - 1 : g.npx - 1) : 0;
+            int i, j, k;
+            if (side < 2) { i = a; j = b; k = v; }
+            else if (side < 4) { i = a; j = v; k = b; }
+            else { i = v; j = a; k = b; }
+            return (uint64_t)cell * g.np + (uint64_t)(i * g.npy * g.npz + j * g.npz + k);
+        };
+        auto dotr = [](const double* p, const double* q) {
+            double r = p[2] * q[2] + 0.0;
+            r = p[1] * q[1] + r;
+            return p[0] * q[0] + r;
+        };
+        {
+            const V3& p0 = g.cC[node(cn, fid, 0, 0)];
+            const V3& pa = g.cC[node(cn, fid, n0 - 1, 0)];
+            const V3& pb = g.cC[node(cn, fid, 0, n1 - 1)];
+            double dn[3], dO[3], ea[3], eb[3];
+            for (int q = 0; q < 3; q++) {
+                dn[q] = ccn[q] - p0[q];
+                dO[q] = lm.fc[fi][q] - p0[q];
+                ea[q] = pa[q] - p0[q];
+                eb[q] = pb[q] - p0[q];
+            }
+            mf.h0 = dotr(dn, ea) >= dotr(dO, ea) ? 0 : 1;
+            mf.h1 = dotr(dn, eb) >= dotr(dO, eb) ? 0 : 1;
+        }
+        mf.coarse = (uint32_t)cn;
+        mf.side = fid;
+        mf.fine_owner = fm == 1 ? 1 : 0;
+        mf.n0 = n0;
+        mf.n1 = n1;
+        for (int q = 0; q < 3; q++) mf.nf[q] = lm.fn[fi][q];
+        const int64_t fslot = side_slot[(size_t)cf * 6 + gf];
+        if (fslot < 0) return nsem_fail(NSEM_EINVAL, "mortar face without a slot");
+        const int dA = fid < 4 ? 0 : 1, dB = fid < 2 ? 1 : 2;
+        const int nbx = gf < 2 ? g.npy : g.npz;
+        for (int t = 0; t < npfm; t++) {
+            uint64_t fnode = (uint64_t)cf * g.np;
+            uint32_t fms = 0;
+            double wk = 0;
+            if (t < n0 * n1) {
+                const uint64_t k = (uint64_t)fi * NPF + t;
+                fnode = fm == 2 ? g.FN[k] : g.FO[k];
+                if (fnode / g.np != (uint64_t)cf) return nsem_fail(NSEM_EINVAL, "mortar face node outside its element");
+                const int q = (int)(fnode - (uint64_t)cf * g.np);
+                const int i = q / (g.npy * g.npz), j = (q / g.npz) % g.npy, kk = q % g.npz;
+                int fa, fb;
+                if (gf < 2) { fa = i; fb = j; }
+                else if (gf < 4) { fa = i; fb = kk; }
+                else { fa = j; fb = kk; }
+                fms = (uint32_t)(fslot * npfm + fa * nbx + fb);
+                const int a = t / n1, b = t - a * n1;
+                wk = g.basis[dA].w[a] * g.basis[dB].w[b] / 4;
+            }
+            mfine.push_back(fnode);
+            mfms.push_back(fms);
+            mw.push_back(wk);
+        }
+        mfaces.push_back(mf);
+        mface_of.push_back(fi);
+    }
+    // split coarse sides: element, side, children, slot, child faces
+    std::vector<uint32_t> cslot;
+    {
+        std::vector<std::vector<uint32_t>> kids((size_t)g.nbcs * 6);
+        for (size_t f = 0; f < mfaces.size(); f++)
+            kids[(size_t)mfaces[f].coarse * 6 + mfaces[f].side].push_back((uint32_t)f);
+        for (size_t cs = 0; cs < kids.size(); cs++) {
+            if (kids[cs].empty()) continue;
+            if (kids[cs].size() > 4) return nsem_fail(NSEM_EUNSUPPORTED, "a split side with more than 4 faces");
+            const int64_t sl = side_slot[cs];
+            if (sl < 0) return nsem_fail(NSEM_EINVAL, "split side without a slot");
+            uint32_t rec[8] = {(uint32_t)(cs / 6), (uint32_t)(cs % 6), (uint32_t)kids[cs].size(), (uint32_t)sl,
+                               0, 0, 0, 0};
+            for (size_t q = 0; q < kids[cs].size(); q++) rec[4 + q] = kids[cs][q];
+            cslot.insert(cslot.end(), rec, rec + 8);
         }
     }
     // metric, volumes, background (euler.cpp:104-131)
@@ -187,6 +320,30 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     });
     bool all_affine = true;
     for (uint8_t a : aff) all_affine = all_affine && a;
+    // background density interpolated from the coarse side minus the refined
+    // side's, per mortar face node
+    std::vector<double> mbg(mfine.size(), 0.0);
+    for (size_t f = 0; f < mfaces.size(); f++) {
+        const MortarFace& mf = mfaces[f];
+        const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+        const double* R0 = g.basis[d0].ref[mf.h0].data();
+        const double* R1 = g.basis[d1].ref[mf.h1].data();
+        for (int t = 0; t < mf.n0 * mf.n1; t++) {
+            const int ao = t / mf.n1, bo = t - ao * mf.n1;
+            double sum = 0;
+            for (int an = 0; an < mf.n0; an++)
+                for (int bn = 0; bn < mf.n1; bn++) {
+                    const int v = (mf.side & 1) ? (mf.side < 2 ? NPZ - 1 : mf.side < 4 ? NPY - 1 : NPX - 1) : 0;
+                    int i, j, k;
+                    if (mf.side < 2) { i = an; j = bn; k = v; }
+                    else if (mf.side < 4) { i = an; j = v; k = bn; }
+                    else { i = v; j = an; k = bn; }
+                    const uint64_t nc = (uint64_t)mf.coarse * NPn + (uint64_t)(i * NPY * NPZ + j * NPZ + k);
+                    sum += R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo] * rref[nc];
+                }
+            mbg[f * npfm + t] = sum - rref[mfine[f * npfm + t]];
+        }
+    }
     std::vector<double> gfn(3 * (size_t)g.nfacets);
     for (Int f = 0; f < g.nfacets; f++)
         for (int c = 0; c < 3; c++) gfn[3 * (size_t)f + c] = lm.fn[f][c];
@@ -202,6 +359,24 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         if ((e = upload(d->D[dir], g.basis[dir].dpsi, s)) != hipSuccess ||
             (e = upload(d->w[dir], g.basis[dir].w, s)) != hipSuccess)
             return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+    }
+    if (!mfaces.empty()) {
+        std::vector<double> zero(1, 0.0);
+        if ((e = upload(d->mort, mfaces, s)) != hipSuccess || (e = upload(d->mort_fine, mfine, s)) != hipSuccess ||
+            (e = upload(d->mort_fms, mfms, s)) != hipSuccess || (e = upload(d->mort_w, mw, s)) != hipSuccess ||
+            (e = upload(d->mort_bg, mbg, s)) != hipSuccess ||
+            (e = upload(d->cslot, cslot.empty() ? std::vector<uint32_t>(8, 0) : cslot, s)) != hipSuccess)
+            return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+        for (int dir = 0; dir < 3; dir++) {
+            std::vector<double> rf = g.basis[dir].ref[0], cr = g.basis[dir].cor[0];
+            rf.insert(rf.end(), g.basis[dir].ref[1].begin(), g.basis[dir].ref[1].end());
+            cr.insert(cr.end(), g.basis[dir].cor[1].begin(), g.basis[dir].cor[1].end());
+            if ((e = upload(d->psiRef[dir], rf, s)) != hipSuccess || (e = upload(d->psiCor[dir], cr, s)) != hipSuccess)
+                return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+        }
+        HIP_TRY(d->mphi.alloc(mfaces.size() * npfm * 5 * sizeof(double)));
+        HIP_TRY(d->msurf.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(d->msurf.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
     }
     HIP_TRY(d->bc.alloc(sizeof(BcTable)));
     HIP_TRY(hipMemcpyAsync(d->bc.p, &bt, sizeof(BcTable), hipMemcpyHostToDevice, s));
@@ -244,6 +419,21 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.affine = all_affine ? 1 : 0;
     dv.emet = d->emet.as<double>();
     dv.wnode = d->wnode.as<double>();
+    dv.nmort = (int)mfaces.size();
+    dv.npfm = npfm;
+    dv.ncslot = (int)(cslot.size() / 8);
+    dv.mort = d->mort.as<MortarFace>();
+    dv.mort_fine = d->mort_fine.as<uint64_t>();
+    dv.mort_fms = d->mort_fms.as<uint32_t>();
+    dv.mort_w = d->mort_w.as<double>();
+    dv.mort_bg = d->mort_bg.as<double>();
+    for (int dir = 0; dir < 3; dir++) {
+        dv.psiRef[dir] = d->psiRef[dir].as<double>();
+        dv.psiCor[dir] = d->psiCor[dir].as<double>();
+    }
+    dv.cslot = d->cslot.as<uint32_t>();
+    dv.mphi = d->mphi.as<double>();
+    dv.msurf = d->msurf.as<double>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -368,6 +558,10 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
 
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    if (d->dev.nmort)
+        return nsem_fail(NSEM_EUNSUPPORTED,
+                         "the reference-order split step does not take 2:1 non-conforming meshes; "
+                         "use the method-of-lines steps");
     for (int32_t s = 0; s < nsteps; s++) {
         DevBuf* A = d->st[d->cur];
         DevBuf* B = d->st[1 - d->cur];
@@ -433,6 +627,17 @@ int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double,
 int nsem_mol_supported(const nsem_dg* d) {
     if (d->prm.diffusion)
         return nsem_fail(NSEM_EUNSUPPORTED, "the derivative entry points do not carry artificial viscosity");
+    if (d->dev.nmort)
+        return nsem_fail(NSEM_EUNSUPPORTED, "the derivative entry points do not take 2:1 non-conforming meshes");
+    return NSEM_OK;
+}
+
+// The method-of-lines steps on a 2:1 non-conforming mesh carry the mortar
+// projections of the inviscid flux only.
+static int rk3_supported(const nsem_dg* d) {
+    if (d->dev.nmort && d->prm.diffusion)
+        return nsem_fail(NSEM_EUNSUPPORTED,
+                         "artificial viscosity on 2:1 non-conforming meshes is not supported by the stage kernels");
     return NSEM_OK;
 }
 
@@ -440,6 +645,7 @@ extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    if (int rs = rk3_supported(d)) return rs;
     int rc;
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
@@ -458,6 +664,7 @@ int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
 
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
     if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    if (int rs = rk3_supported(d)) return rs;
     int rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);

@@ -182,7 +182,11 @@ constexpr int kPrimRows = 10;
 // node's LGL weight product) instead of being read per node.
 // VISC: artificial viscosity: the flux rows carry -mu grad U and -mu/Pr grad
 // theta' (mu = rho visc) from a.grad, on both sides of every face.
-template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN, bool AFF, bool VISC>
+// MORT: the mesh has 2:1 mortar sides, whose surface terms come from the
+// stage's mortar pre-pass (a separate instantiation, so that conforming
+// meshes run the kernel without the test).
+template <typename R, typename S, int NXc, int NYc, int NZc, bool O32, bool DIN, bool AFF, bool VISC,
+          bool MORT = false>
 __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -345,6 +349,8 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             }
             const FaceTask<R>& F = ft;
             if (!((F.meta >> 4) & 1)) continue;
+            if constexpr (MORT)
+                if ((F.meta >> 6) & 1) continue;  // mortar side: surface term from the pre-pass
             const bool own = (F.meta >> 3) & 1;
             const bool ghost = (F.meta >> 5) & 1;
             const int fnode = F.fnode;
@@ -403,6 +409,15 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
         if (!((meta >> 4) & 1)) continue;
         int fa, fb;
         face_ab(g, i, j, k, fa, fb);
+        if constexpr (MORT) {
+            if ((meta >> 6) & 1) {
+                // 2:1 mortar side: its surface term, sign included, from the
+                // stage's mortar pre-pass
+                const double* ms = d.msurf + ((size_t)(meta >> 8) * d.npfm + fa * (g < 2 ? NY : NZ) + fb) * 5;
+                for (int v = 0; v < 5; v++) res[v] += (R)ms[v];
+                continue;
+            }
+        }
         const int t = face_task_index(g, fa, fb, NX, NY, NZ);
         if ((meta >> 3) & 1)
             for (int v = 0; v < 5; v++) res[v] -= FL[v * NFN + t];
@@ -591,9 +606,9 @@ template hipError_t launch_mol_gradient<double, double>(const DgDev&, const doub
 template hipError_t launch_mol_gradient<float, float>(const DgDev&, const double* const[5], const float* const[5],
                                                       float*, hipStream_t);
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC>
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC, bool MORT = false>
 __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z, false, DIN, AFF, VISC>(d, a);
+    rk_stage_body<R, S, X, Y, Z, false, DIN, AFF, VISC, MORT>(d, a);
 }
 
 // Compile-time element shape (X x Y x Z nodes), node offsets in 32 bits,
@@ -636,6 +651,18 @@ static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks
                              hipStream_t st) {
     // the tangent-linear instantiation has no viscous variant (the
     // derivative entry points refuse diffusion)
+    if constexpr (!is_dual<R>::value && X == 0) {
+        // 2:1 non-conforming meshes (inviscid; the runtime-shape kernel)
+        if (d.nmort) {
+            if (d.affine)
+                hipLaunchKernelGGL((rk_stage_kernel<R, S, 0, 0, 0, DIN, true, false, true>), dim3(blocks),
+                                   dim3(threads), lds, st, d, a);
+            else
+                hipLaunchKernelGGL((rk_stage_kernel<R, S, 0, 0, 0, DIN, false, false, true>), dim3(blocks),
+                                   dim3(threads), lds, st, d, a);
+            return;
+        }
+    }
     if constexpr (!is_dual<R>::value) {
         if (a.grad) {
             if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, true>(d, a, blocks, threads, lds, st);
@@ -667,8 +694,159 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// 2:1 non-conforming faces of the method-of-lines stage (field.h:2018-2248,
+// 2927-2943, 3085-3117).  On a mortar face the coarse side's node values
+// reach the refined face's nodes through psiRef (scatter): the normal
+// fluxes, the state, |U| and c are interpolated, not recomputed from an
+// interpolated state.  The Rusanov flux at the refined face's nodes then
+// goes to the refined element as it is and to the coarse element's side
+// nodes through psiCor (gather).  Both contributions are linear in the face
+// normal, so the flux is formed per unit face-node weight (phi) and
+// weighted where it is used.
+//
+// mortar_flux_kernel: one work-group per mortar face, one thread per face
+// node: phi, and the refined side's surface term.
+template <typename R, typename S, bool DIN>
+__global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    if (t >= mf.n0 * mf.n1) return;
+    const int NX = d.npx, NY = d.npy, NZ = d.npz;
+    const int ao = t / mf.n1, bo = t - ao * mf.n1;
+    const size_t ft = (size_t)f * d.npfm + t;
+    const R T0 = (R)d.T0, gam = (R)d.gamma, gR = (R)(d.gamma * d.R);
+    const R nv[3] = {(R)mf.nf[0], (R)mf.nf[1], (R)mf.nf[2]};
+    auto normal_fluxes = [&](const Prim<R>& P, R* fn) {
+        const R mn = P.m[0] * nv[0] + P.m[1] * nv[1] + P.m[2] * nv[2];
+        const R un = P.u[0] * nv[0] + P.u[1] * nv[1] + P.u[2] * nv[2];
+        fn[0] = mn;
+        for (int r = 0; r < 3; r++) fn[1 + r] = P.m[r] * un + P.pp * nv[r];
+        fn[4] = mn * T0 + mn * P.thp;
+    };
+    auto load_prim = [&](uint64_t n, Prim<R>& P) {
+        P.rp = load_state<R, S, false, DIN>(a, 0, n);
+        P.m[0] = load_state<R, S, false, DIN>(a, 1, n);
+        P.m[1] = load_state<R, S, false, DIN>(a, 2, n);
+        P.m[2] = load_state<R, S, false, DIN>(a, 3, n);
+        P.tp = load_state<R, S, false, DIN>(a, 4, n);
+        make_prim(P, (R)d.rho_ref[n], (R)d.p_ref[n], T0, gam, gR);
+    };
+    // refined side
+    Prim<R> Pf{};
+    load_prim(d.mort_fine[ft], Pf);
+    R ff[5];
+    normal_fluxes(Pf, ff);
+    // coarse side, interpolated to this node
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    R fc[5] = {0, 0, 0, 0, 0}, rc = 0, mc[3] = {0, 0, 0}, tc = 0, uc = 0, cc = 0;
+    for (int an = 0; an < mf.n0; an++)
+        for (int bn = 0; bn < mf.n1; bn++) {
+            const R w = (R)(R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo]);
+            Prim<R> P{};
+            load_prim(face_node(mf.coarse, mf.side, an, bn, NX, NY, NZ), P);
+            R fn[5];
+            normal_fluxes(P, fn);
+            for (int v = 0; v < 5; v++) fc[v] += w * fn[v];
+            rc += w * P.rp;
+            for (int c = 0; c < 3; c++) mc[c] += w * P.m[c];
+            tc += w * P.tp;
+            uc += w * P.umag;
+            cc += w * P.c;
+        }
+    // Rusanov at the interior weight 1/2, owner O and neighbour N; the
+    // background density and Theta enter the jumps through mort_bg
+    // (coarse minus refined, Theta_ref = rho_ref T0)
+    const bool fo = mf.fine_owner;
+    const R bg = (R)d.mort_bg[ft];
+    const R drho = fo ? (rc - Pf.rp) + bg : (Pf.rp - rc) - bg;   // rho_N - rho_O
+    const R dth = fo ? (tc - Pf.tp) + T0 * bg : (Pf.tp - tc) - T0 * bg;
+    R dm[3];
+    for (int c = 0; c < 3; c++) dm[c] = fo ? mc[c] - Pf.m[c] : Pf.m[c] - mc[c];
+    const R lam = (((Pf.umag + uc) * R(0.5)) + ((Pf.c + cc) * R(0.5))) * R(0.5);
+    const R mn = r_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const R imn = r_div(R(1), mn);
+    const R dmn = lam * (dm[0] * nv[0] + dm[1] * nv[1] + dm[2] * nv[2]);
+    R phi[5];
+    phi[0] = (ff[0] + fc[0]) * R(0.5) - lam * drho * mn;
+    for (int r = 0; r < 3; r++) phi[1 + r] = (ff[1 + r] + fc[1 + r]) * R(0.5) - (nv[r] * imn) * dmn;
+    phi[4] = (ff[4] + fc[4]) * R(0.5) - lam * dth * mn;
+    // the refined element: -w phi when it owns the face, +w phi otherwise
+    const double wgt = d.mort_w[ft];
+    double* ms = d.msurf + (size_t)d.mort_fms[ft] * 5;
+    for (int v = 0; v < 5; v++) {
+        d.mphi[ft * 5 + v] = (double)phi[v];
+        ms[v] = (fo ? -wgt : wgt) * (double)phi[v];
+    }
+}
+
+// mortar_gather_kernel: one work-group per split coarse side, one thread per
+// side node: the sum over the side's faces of w(node) psiCor(phi), with the
+// coarse element's sign.
+__global__ void __launch_bounds__(256) mortar_gather_kernel(DgDev d) {
+    const uint32_t* cs = d.cslot + (size_t)blockIdx.x * 8;
+    const int side = (int)cs[1], nch = (int)cs[2];
+    const uint32_t slot = cs[3];
+    const MortarFace m0 = d.mort[cs[4]];
+    const int t = threadIdx.x;
+    if (t >= m0.n0 * m0.n1) return;
+    const int an = t / m0.n1, bn = t - an * m0.n1;
+    const int d0 = side < 4 ? 0 : 1, d1 = side < 2 ? 1 : 2;
+    double sum[5] = {0, 0, 0, 0, 0};
+    for (int ch = 0; ch < nch; ch++) {
+        const uint32_t f = cs[4 + ch];
+        const MortarFace mf = d.mort[f];
+        const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+        const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+        const double* ph = d.mphi + (size_t)f * d.npfm * 5;
+        double g[5] = {0, 0, 0, 0, 0};
+        for (int ao = 0; ao < mf.n0; ao++)
+            for (int bo = 0; bo < mf.n1; bo++) {
+                const double w = C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn];
+                for (int v = 0; v < 5; v++) g[v] += w * ph[(ao * mf.n1 + bo) * 5 + v];
+            }
+        // the weight of this node on the refined face (the reference's
+        // fN[k] at the same face-node index)
+        const double wk = d.mort_w[(size_t)f * d.npfm + t];
+        const double sgn = mf.fine_owner ? wk : -wk;   // the coarse element owns the face: subtract
+        for (int v = 0; v < 5; v++) sum[v] += sgn * g[v];
+    }
+    double* ms = d.msurf + ((size_t)slot * d.npfm + t) * 5;
+    for (int v = 0; v < 5; v++) ms[v] = sum[v];
+}
+
+template <typename R, typename S>
+hipError_t launch_rk_shape(const DgDev& d, const RkArgs<S>& a, hipStream_t st);
+
+template <typename R, typename S>
+static hipError_t launch_mortar(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
+    if (d.npfm > 256) return hipErrorInvalidValue;
+    const int threads = (d.npfm + 63) / 64 * 64;
+    if (a.din[0])
+        hipLaunchKernelGGL((mortar_flux_kernel<R, S, true>), dim3(d.nmort), dim3(threads), 0, st, d, a);
+    else
+        hipLaunchKernelGGL((mortar_flux_kernel<R, S, false>), dim3(d.nmort), dim3(threads), 0, st, d, a);
+    if (d.ncslot) hipLaunchKernelGGL(mortar_gather_kernel, dim3(d.ncslot), dim3(threads), 0, st, d);
+    return hipGetLastError();
+}
+
 template <typename R, typename S>
 hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
+    if constexpr (!is_dual<R>::value) {
+        if (d.nmort) {
+            const hipError_t e = launch_mortar<R, S>(d, a, st);
+            if (e != hipSuccess) return e;
+            return launch_rk_t<R, S, 0, 0, 0>(d, a, st);
+        }
+    }
+    return launch_rk_shape<R, S>(d, a, st);
+}
+
+template <typename R, typename S>
+hipError_t launch_rk_shape(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_rk_t<R, S, 8, 8, 8>(d, a, st);
     if (d.npx == 7 && d.npy == 7 && d.npz == 7) return launch_rk_t<R, S, 7, 7, 7>(d, a, st);
     if (d.npx == 6 && d.npy == 6 && d.npz == 6) return launch_rk_t<R, S, 6, 6, 6>(d, a, st);
