@@ -162,7 +162,7 @@ def amr_bench(steps=20, dt=0.002):
     out = {"workload": "3D AMR dGSEM Euler, 32x32x16 base hex over 10x10x5 km refined twice (2:1) around "
                        "a 1 K bubble, N=5, SSP-RK3 dt=%g" % dt,
            "elements": nb // 216, "nodes": nb, "dof": NQ * nb, "mortar_faces": int((fmc > 0).sum()),
-           "steps": steps, "stage_kernel": "rk_stage_kernel<R,S,0,0,0,DIN,AFF,false,true> + mortar pre-pass"}
+           "steps": steps, "stage_kernel": "rk_stage_kernel_fx<R,S,6,6,6,W,DIN,AFF,false,true> + mortar pre-pass"}
     q = {}
     for prec in (32, 64):
         dg.set_state(rho, U, T, p)

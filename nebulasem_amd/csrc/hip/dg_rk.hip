@@ -626,24 +626,25 @@ constexpr int rk_waves_per_simd() {
     return VISC ? 2 : (np > 128 ? 4 : np > 64 ? 3 : 2);
 }
 
-template <typename R, typename S, int X, int Y, int Z, int W, bool DIN, bool AFF, bool VISC>
+template <typename R, typename S, int X, int Y, int Z, int W, bool DIN, bool AFF, bool VISC, bool MORT = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(W)))
 rk_stage_kernel_fx(DgDev d, RkArgs<S> a) {
-    rk_stage_body<R, S, X, Y, Z, true, DIN, AFF, VISC>(d, a);
+    rk_stage_body<R, S, X, Y, Z, true, DIN, AFF, VISC, MORT>(d, a);
 }
 
-template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC>
+template <typename R, typename S, int X, int Y, int Z, bool DIN, bool AFF, bool VISC, bool MORT = false>
 static void launch_rk_kernel(const DgDev& d, const RkArgs<S>& a, unsigned blocks, int threads, size_t lds,
                              hipStream_t st) {
     constexpr int W = rk_waves_per_simd<R, X, Y, Z, VISC>();
     if constexpr (X > 0 && !is_dual<R>::value) {
         if (d.nb * sizeof(double) < (1ull << 32)) {
-            hipLaunchKernelGGL((rk_stage_kernel_fx<R, S, X, Y, Z, W, DIN, AFF, VISC>), dim3(blocks), dim3(threads),
-                               lds, st, d, a);
+            hipLaunchKernelGGL((rk_stage_kernel_fx<R, S, X, Y, Z, W, DIN, AFF, VISC, MORT>), dim3(blocks),
+                               dim3(threads), lds, st, d, a);
             return;
         }
     }
-    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN, AFF, VISC>), dim3(blocks), dim3(threads), lds, st, d, a);
+    hipLaunchKernelGGL((rk_stage_kernel<R, S, X, Y, Z, DIN, AFF, VISC, MORT>), dim3(blocks), dim3(threads), lds, st,
+                       d, a);
 }
 
 template <typename R, typename S, int X, int Y, int Z, bool DIN>
@@ -651,15 +652,11 @@ static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks
                              hipStream_t st) {
     // the tangent-linear instantiation has no viscous variant (the
     // derivative entry points refuse diffusion)
-    if constexpr (!is_dual<R>::value && X == 0) {
-        // 2:1 non-conforming meshes (inviscid; the runtime-shape kernel)
+    if constexpr (!is_dual<R>::value) {
+        // 2:1 non-conforming meshes (inviscid)
         if (d.nmort) {
-            if (d.affine)
-                hipLaunchKernelGGL((rk_stage_kernel<R, S, 0, 0, 0, DIN, true, false, true>), dim3(blocks),
-                                   dim3(threads), lds, st, d, a);
-            else
-                hipLaunchKernelGGL((rk_stage_kernel<R, S, 0, 0, 0, DIN, false, false, true>), dim3(blocks),
-                                   dim3(threads), lds, st, d, a);
+            if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, false, true>(d, a, blocks, threads, lds, st);
+            else launch_rk_kernel<R, S, X, Y, Z, DIN, false, false, true>(d, a, blocks, threads, lds, st);
             return;
         }
     }
@@ -706,13 +703,15 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
 // weighted where it is used.
 //
 // mortar_flux_kernel: one work-group per mortar face, one thread per face
-// node: phi, and the refined side's surface term.
+// node: the coarse side's node values (normal fluxes, state, |U|, c) once
+// per node into LDS, then per refined face node their interpolation, phi,
+// and the refined side's surface term.
 template <typename R, typename S, bool DIN>
 __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) {
     const uint32_t f = blockIdx.x;
     const int t = threadIdx.x;
     const MortarFace mf = d.mort[f];
-    if (t >= mf.n0 * mf.n1) return;
+    const bool act = t < mf.n0 * mf.n1;
     const int NX = d.npx, NY = d.npy, NZ = d.npz;
     const int ao = t / mf.n1, bo = t - ao * mf.n1;
     const size_t ft = (size_t)f * d.npfm + t;
@@ -733,6 +732,25 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
         P.tp = load_state<R, S, false, DIN>(a, 4, n);
         make_prim(P, (R)d.rho_ref[n], (R)d.p_ref[n], T0, gam, gR);
     };
+    // coarse side node (ao, bo) of this thread: its 12 values into LDS
+    // [12][npfm]: normal fluxes (5), rho', m (3), Theta', |U|, c
+    extern __shared__ unsigned char mortar_lds[];
+    R* L = reinterpret_cast<R*>(mortar_lds);
+    const int np = d.npfm;
+    if (act) {
+        Prim<R> P{};
+        load_prim(face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ), P);
+        R fn[5];
+        normal_fluxes(P, fn);
+        for (int v = 0; v < 5; v++) L[v * np + t] = fn[v];
+        L[5 * np + t] = P.rp;
+        for (int c = 0; c < 3; c++) L[(6 + c) * np + t] = P.m[c];
+        L[9 * np + t] = P.tp;
+        L[10 * np + t] = P.umag;
+        L[11 * np + t] = P.c;
+    }
+    __syncthreads();
+    if (!act) return;
     // refined side
     Prim<R> Pf{};
     load_prim(d.mort_fine[ft], Pf);
@@ -746,16 +764,13 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
     for (int an = 0; an < mf.n0; an++)
         for (int bn = 0; bn < mf.n1; bn++) {
             const R w = (R)(R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo]);
-            Prim<R> P{};
-            load_prim(face_node(mf.coarse, mf.side, an, bn, NX, NY, NZ), P);
-            R fn[5];
-            normal_fluxes(P, fn);
-            for (int v = 0; v < 5; v++) fc[v] += w * fn[v];
-            rc += w * P.rp;
-            for (int c = 0; c < 3; c++) mc[c] += w * P.m[c];
-            tc += w * P.tp;
-            uc += w * P.umag;
-            cc += w * P.c;
+            const int s2 = an * mf.n1 + bn;
+            for (int v = 0; v < 5; v++) fc[v] += w * L[v * np + s2];
+            rc += w * L[5 * np + s2];
+            for (int c = 0; c < 3; c++) mc[c] += w * L[(6 + c) * np + s2];
+            tc += w * L[9 * np + s2];
+            uc += w * L[10 * np + s2];
+            cc += w * L[11 * np + s2];
         }
     // Rusanov at the interior weight 1/2, owner O and neighbour N; the
     // background density and Theta enter the jumps through mort_bg
@@ -825,10 +840,11 @@ template <typename R, typename S>
 static hipError_t launch_mortar(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
     if (d.npfm > 256) return hipErrorInvalidValue;
     const int threads = (d.npfm + 63) / 64 * 64;
+    const size_t lds = (size_t)12 * d.npfm * sizeof(R);
     if (a.din[0])
-        hipLaunchKernelGGL((mortar_flux_kernel<R, S, true>), dim3(d.nmort), dim3(threads), 0, st, d, a);
+        hipLaunchKernelGGL((mortar_flux_kernel<R, S, true>), dim3(d.nmort), dim3(threads), lds, st, d, a);
     else
-        hipLaunchKernelGGL((mortar_flux_kernel<R, S, false>), dim3(d.nmort), dim3(threads), 0, st, d, a);
+        hipLaunchKernelGGL((mortar_flux_kernel<R, S, false>), dim3(d.nmort), dim3(threads), lds, st, d, a);
     if (d.ncslot) hipLaunchKernelGGL(mortar_gather_kernel, dim3(d.ncslot), dim3(threads), 0, st, d);
     return hipGetLastError();
 }
@@ -839,7 +855,6 @@ hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
         if (d.nmort) {
             const hipError_t e = launch_mortar<R, S>(d, a, st);
             if (e != hipSuccess) return e;
-            return launch_rk_t<R, S, 0, 0, 0>(d, a, st);
         }
     }
     return launch_rk_shape<R, S>(d, a, st);
