@@ -374,7 +374,9 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             if ((e = upload(d->psiRef[dir], rf, s)) != hipSuccess || (e = upload(d->psiCor[dir], cr, s)) != hipSuccess)
                 return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
         }
-        HIP_TRY(d->mphi.alloc(mfaces.size() * npfm * 5 * sizeof(double)));
+        // per face node: 5 values of the method-of-lines pre-pass, 16 of the
+        // split step's (flux vectors and lambda)
+        HIP_TRY(d->mphi.alloc(mfaces.size() * npfm * 16 * sizeof(double)));
         HIP_TRY(d->msurf.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d->msurf.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
     }
@@ -558,10 +560,9 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
 
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    if (d->dev.nmort)
+    if (d->dev.nmort && d->prm.diffusion)
         return nsem_fail(NSEM_EUNSUPPORTED,
-                         "the reference-order split step does not take 2:1 non-conforming meshes; "
-                         "use the method-of-lines steps");
+                         "artificial viscosity on 2:1 non-conforming meshes is not supported by the split step");
     for (int32_t s = 0; s < nsteps; s++) {
         DevBuf* A = d->st[d->cur];
         DevBuf* B = d->st[1 - d->cur];

@@ -112,6 +112,10 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
         const int og = ef.meta & 7;
         int a, b;
         face_ab(g, i, j, k, a, b);
+        if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed term from the pre-pass
+            surf += d.msurf[((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5];
+            continue;
+        }
         const double wgt = face_weight(d, own ? g : og, a, b);
         const double nx = d.gFN[3 * (size_t)ef.facet] * wgt;
         const double ny = d.gFN[3 * (size_t)ef.facet + 1] * wgt;
@@ -328,6 +332,11 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
             const int og = ef.meta & 7;
             int a, b;
             face_ab(g, i, j, k, a, b);
+            if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed terms from the pre-pass
+                const double* ms = d.msurf + ((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5;
+                for (int r = 0; r < 4; r++) res[r] += ms[1 + r];
+                continue;
+            }
             const double wgt = face_weight(d, own ? g : og, a, b);
             const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
                                   d.gFN[3 * (size_t)ef.facet + 2] * wgt};
@@ -434,6 +443,214 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
 }
 
 // ---------------------------------------------------------------------------
+// 2:1 mortar faces of the split step, in the reference's form
+// (scatter_non_conforming / gather_non_conforming, field.h:2018-2248, inside
+// cds, rusanov and div_flux): the coarse side's node fields -- flux vectors,
+// the Rusanov state, |U| and c -- interpolated to each refined face node
+// with psiRef, the Rusanov flux vector formed there, and gathered to the
+// coarse side's node positions with psiCor before the dot with the face
+// normal.  One work-group per mortar face (coarse node fields staged in
+// LDS), one thread per face node; then one work-group per split coarse side.
+// The terms are stored signed (+ on the owner side, as div_flux adds) in
+// msurf: entry 0 for the density pass, 1-3 momentum, 4 theta.  mphi holds
+// per face node the flux vectors (density 0-2 in the first pass; momentum
+// rows 0-8 and theta 9-11 in the second) and lambda (12).
+constexpr int kSplitPhi = 16;
+
+__device__ __forceinline__ void mortar_node_ids(const DgDev& d, const MortarFace& mf, int t, int& ao, int& bo) {
+    ao = t / mf.n1;
+    bo = t - ao * mf.n1;
+    (void)d;
+}
+
+// density pass: F = U rho, q = rho; lambda = (cds |U| + cds c) / 2
+__global__ void __launch_bounds__(256) split_mortar_rho_kernel(DgDev d, SplitState s) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    const bool act = t < mf.n0 * mf.n1;
+    int ao, bo;
+    mortar_node_ids(d, mf, t, ao, bo);
+    const int np = d.npfm;
+    const double gRT = d.gamma * d.R;
+    auto vals = [&](uint64_t m, double* v) {
+        const double rho = s.rho[m], u0 = s.ux[m], u1 = s.uy[m], u2 = s.uz[m];
+        const double th = s.T[m] + d.T0;
+        v[0] = u0 * rho; v[1] = u1 * rho; v[2] = u2 * rho;
+        v[3] = rho;
+        v[4] = sqrt(dot3r(u0, u1, u2, u0, u1, u2));
+        v[5] = sqrt(gRT * th);
+    };
+    extern __shared__ double L[];  // [6][npfm]
+    if (act) {
+        double v[6];
+        vals(face_node(mf.coarse, mf.side, ao, bo, d.npx, d.npy, d.npz), v);
+        for (int q = 0; q < 6; q++) L[q * np + t] = v[q];
+    }
+    __syncthreads();
+    if (!act) return;
+    const size_t ft = (size_t)f * np + t;
+    double vf[6], vc[6] = {0, 0, 0, 0, 0, 0};
+    vals(d.mort_fine[ft], vf);
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    for (int q = 0; q < 6; q++) {
+        double sum = 0.0;
+        for (int an = 0; an < mf.n0; an++)
+            for (int bn = 0; bn < mf.n1; bn++)
+                sum += L[q * np + an * mf.n1 + bn] * R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo];
+        vc[q] = sum;
+    }
+    const bool fo = mf.fine_owner;
+    const double* vO = fo ? vf : vc;
+    const double* vN = fo ? vc : vf;
+    const double fI = 0.5;
+    const double lam = ((vO[4] * fI + vN[4] * (1 - fI)) + (vO[5] * fI + vN[5] * (1 - fI))) / 2;
+    const double wk = d.mort_w[ft];
+    const double nv[3] = {mf.nf[0] * wk, mf.nf[1] * wk, mf.nf[2] * wk};
+    const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+    const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+    const double dq = lam * (vN[3] - vO[3]);
+    double fF[3];
+    for (int c = 0; c < 3; c++) fF[c] = vO[c] * fI + vN[c] * (1 - fI) - en[c] * dq;
+    double* ph = d.mphi + ft * kSplitPhi;
+    for (int c = 0; c < 3; c++) ph[c] = fF[c];
+    ph[12] = lam;
+    const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
+    d.msurf[(size_t)d.mort_fms[ft] * 5] = fo ? flux : -flux;
+}
+
+// momentum and theta pass: F_U = (rho^n U) (x) U + p' I, q_U = rho^{n+1} U;
+// F_theta = (rho^n U) theta, q_theta = rho^{n+1} theta; lambda of the first
+// pass
+__global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitState s, const double* rho1) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    const bool act = t < mf.n0 * mf.n1;
+    int ao, bo;
+    mortar_node_ids(d, mf, t, ao, bo);
+    const int np = d.npfm;
+    const double R = d.R, gam = d.gamma;
+    // v: F_U rows (9), F_theta (3), q_U (3), q_theta (1)
+    auto vals = [&](uint64_t m, double* v) {
+        const double rho0 = s.rho[m], rho = rho1[m];
+        const double u[3] = {s.ux[m], s.uy[m], s.uz[m]};
+        const double th = s.T[m] + d.T0;
+        const double pp = d.P0 * pow((rho * th * R) / d.P0, gam) - d.p_ref[m];
+        const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
+        for (int row = 0; row < 3; row++)
+            for (int c = 0; c < 3; c++) v[3 * row + c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
+        for (int c = 0; c < 3; c++) v[9 + c] = fc[c] * th;
+        for (int c = 0; c < 3; c++) v[12 + c] = rho * u[c];
+        v[15] = rho * th;
+    };
+    extern __shared__ double L[];  // [16][npfm]
+    if (act) {
+        double v[16];
+        vals(face_node(mf.coarse, mf.side, ao, bo, d.npx, d.npy, d.npz), v);
+        for (int q = 0; q < 16; q++) L[q * np + t] = v[q];
+    }
+    __syncthreads();
+    if (!act) return;
+    const size_t ft = (size_t)f * np + t;
+    double vf[16], vc[16];
+    vals(d.mort_fine[ft], vf);
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    for (int q = 0; q < 16; q++) {
+        double sum = 0.0;
+        for (int an = 0; an < mf.n0; an++)
+            for (int bn = 0; bn < mf.n1; bn++)
+                sum += L[q * np + an * mf.n1 + bn] * R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo];
+        vc[q] = sum;
+    }
+    const bool fo = mf.fine_owner;
+    const double* vO = fo ? vf : vc;
+    const double* vN = fo ? vc : vf;
+    const double fI = 0.5;
+    double* ph = d.mphi + ft * kSplitPhi;
+    const double lam = ph[12];
+    const double wk = d.mort_w[ft];
+    const double nv[3] = {mf.nf[0] * wk, mf.nf[1] * wk, mf.nf[2] * wk};
+    const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+    const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+    double* ms = d.msurf + (size_t)d.mort_fms[ft] * 5;
+    for (int row = 0; row < 3; row++) {
+        double fF[3];
+        for (int c = 0; c < 3; c++) {
+            const double dq = lam * (vN[12 + c] - vO[12 + c]);
+            fF[c] = vO[3 * row + c] * fI + vN[3 * row + c] * (1 - fI) - en[row] * dq;
+            ph[3 * row + c] = fF[c];
+        }
+        const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
+        ms[1 + row] = fo ? flux : -flux;
+    }
+    {
+        const double dq = lam * (vN[15] - vO[15]);
+        double fF[3];
+        for (int c = 0; c < 3; c++) {
+            fF[c] = vO[9 + c] * fI + vN[9 + c] * (1 - fI) - en[c] * dq;
+            ph[9 + c] = fF[c];
+        }
+        const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
+        ms[4] = fo ? flux : -flux;
+    }
+}
+
+// gather of either pass to the split coarse sides: per coarse side node,
+// over the side's faces in order, the flux vectors projected with psiCor and
+// dotted with the face's normal at that node; UT selects the second pass
+template <bool UT>
+__global__ void __launch_bounds__(256) split_mortar_gather_kernel(DgDev d) {
+    const uint32_t* cs = d.cslot + (size_t)blockIdx.x * 8;
+    const int side = (int)cs[1], nch = (int)cs[2];
+    const uint32_t slot = cs[3];
+    const MortarFace m0 = d.mort[cs[4]];
+    const int t = threadIdx.x;
+    if (t >= m0.n0 * m0.n1) return;
+    const int an = t / m0.n1, bn = t - an * m0.n1;
+    const int d0 = side < 4 ? 0 : 1, d1 = side < 2 ? 1 : 2;
+    constexpr int NV = UT ? 12 : 3;
+    double out[4] = {0, 0, 0, 0};
+    for (int ch = 0; ch < nch; ch++) {
+        const uint32_t f = cs[4 + ch];
+        const MortarFace mf = d.mort[f];
+        const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+        const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+        const double* ph = d.mphi + (size_t)f * d.npfm * kSplitPhi;
+        double G[NV];
+        for (int q = 0; q < NV; q++) {
+            double sum = 0.0;
+            for (int ao = 0; ao < mf.n0; ao++)
+                for (int bo = 0; bo < mf.n1; bo++)
+                    sum += ph[(ao * mf.n1 + bo) * kSplitPhi + q] * C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn];
+            G[q] = sum;
+        }
+        const double wk = d.mort_w[(size_t)f * d.npfm + t];
+        const double nv[3] = {mf.nf[0] * wk, mf.nf[1] * wk, mf.nf[2] * wk};
+        const bool coarse_owns = !mf.fine_owner;
+        if constexpr (UT) {
+            for (int row = 0; row < 3; row++) {
+                const double flux = G[3 * row] * nv[0] + G[3 * row + 1] * nv[1] + G[3 * row + 2] * nv[2];
+                out[row] = coarse_owns ? out[row] + flux : out[row] - flux;
+            }
+            const double flux = dot3r(G[9], G[10], G[11], nv[0], nv[1], nv[2]);
+            out[3] = coarse_owns ? out[3] + flux : out[3] - flux;
+        } else {
+            const double flux = dot3r(G[0], G[1], G[2], nv[0], nv[1], nv[2]);
+            out[0] = coarse_owns ? out[0] + flux : out[0] - flux;
+        }
+    }
+    double* ms = d.msurf + ((size_t)slot * d.npfm + t) * 5;
+    if constexpr (UT)
+        for (int r = 0; r < 4; r++) ms[1 + r] = out[r];
+    else
+        ms[0] = out[0];
+}
+
 template <int X, int Y, int Z>
 static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
                                  double* grad, hipStream_t st) {
@@ -452,8 +669,22 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
         hipLaunchKernelGGL((split_grad_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_g, st, d, s, grad);
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
+    const int mthreads = (d.npfm + 63) / 64 * 64;
+    if (d.nmort) {
+        if (d.diffusion || d.npfm > 256) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(split_mortar_rho_kernel, dim3(d.nmort), dim3(mthreads),
+                           (size_t)6 * d.npfm * sizeof(double), st, d, s);
+        if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<false>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
     hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
     if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (d.nmort) {
+        hipLaunchKernelGGL(split_mortar_ut_kernel, dim3(d.nmort), dim3(mthreads),
+                           (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
+        if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<true>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+    }
     if (d.diffusion)
         hipLaunchKernelGGL((split_ut_kernel<X, Y, Z, true>), dim3(blocks), dim3(threads), lds_b, st, d, s, rho1, o);
     else

@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from _cases import EULER_CASES
+from _cases import AMR_CASES, EULER_CASES
 from _euler_setup import bc_table, load_case
 from nebulasem_amd import DGEuler, euler_params
 
@@ -50,7 +50,11 @@ def setup(oracle, tmp_path, case):
     return mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p)
 
 
-@pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
+# plus the inviscid 2:1 non-conforming decks (mortar pre-pass of each pass)
+SPLIT_CASES = EULER_CASES + [c for c in AMR_CASES if c.get("viscosity") is None]
+
+
+@pytest.mark.parametrize("case", SPLIT_CASES, ids=[c["name"] for c in SPLIT_CASES])
 def test_split_steps_match_oracle(oracle, tmp_path, case):
     print(f"[gpu] split steps {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
