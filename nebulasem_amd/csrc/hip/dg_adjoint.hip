@@ -222,7 +222,8 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
             face_task(t, NX, NY, NZ, g, fa, fb, fnode);
             const EFace ef = d.eface[(size_t)e * 6 + g];
             double contrib[5] = {0, 0, 0, 0, 0};
-            if ((ef.meta >> 4) & 1) {
+            // 2:1 mortar sides: their share comes from mortar_vjp_kernel (madj)
+            if (((ef.meta >> 4) & 1) && !((ef.meta >> 6) & 1)) {
                 const bool own = (ef.meta >> 3) & 1;
                 const bool ghost = (ef.meta >> 5) & 1;
                 const int og = ef.meta & 7;
@@ -271,10 +272,145 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         const int t = face_task_index(g, fa, fb, NX, NY, NZ);
         for (int v = 0; v < 5; v++) qbar[v] += FL[v * NFN + t];
     }
+    if (d.nmort)
+        for (int v = 0; v < 5; v++) qbar[v] += d.madj[(size_t)v * nb + n];
     for (int v = 0; v < 5; v++) {
         if (a.t[0]) a.t[v][n] = qbar[v];
         if (a.ad_out[0]) a.ad_out[v][n] = a.c1 * (wn[v] + a.dt * qbar[v]);
         if (a.acc_out[0]) a.acc_out[v][n] = a.acc_in[v][n] + a.c2 * a.dt * qbar[v];
+    }
+}
+
+// Vector-Jacobian product of the mortar faces' share of the residual (the
+// method-of-lines pre-pass of dg_rk.hip: mortar_flux_kernel and
+// mortar_gather_kernel).  There the refined node gains sf w_t phi_t and each
+// coarse side node t' gains sc w_t' sum_t psiCor(t, t') phi_t (sf = -1 when
+// the refined element owns the face, else +1; sc = -sf), phi_t the Rusanov
+// flux per unit weight from the refined node's values and the coarse side's
+// values interpolated with psiRef.  With w^ = w / cV the cotangent of phi_t
+// is  sf w_t w^(refined) + sum_t' psiCor(t, t') sc w_t' w^(coarse t');  phi's
+// partial derivatives (with respect to the refined node's state and to the
+// interpolated values) are taken by forward-mode dual numbers through the
+// same functions as the forward pass; the interpolated values' cotangents go
+// back to the coarse nodes through psiRef transposed, and through the node
+// values' derivatives to the coarse states.  Contributions are added into
+// d.madj [5][nb] (a node can sit on several mortar faces), which the VJP
+// kernel adds to its own.  One work-group per mortar face, one thread per
+// face node (also, in the last phase, per coarse side node).
+__global__ void __launch_bounds__(256) mortar_vjp_kernel(DgDev d, VjpArgs a) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    const bool act = t < mf.n0 * mf.n1;
+    const int NX = d.npx, NY = d.npy, NZ = d.npz;
+    const int ao = t / mf.n1, bo = t - ao * mf.n1;
+    const int np = d.npfm;
+    const size_t ft = (size_t)f * np + t;
+    const uint64_t nb = d.nb;
+    const double T0 = d.T0, gam = d.gamma, gR = d.gamma * d.R;
+    const double nv[3] = {mf.nf[0], mf.nf[1], mf.nf[2]};
+    const D1 nvD[3] = {D1(nv[0]), D1(nv[1]), D1(nv[2])};
+    const bool fo = mf.fine_owner != 0;
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    auto Q = [&](uint64_t n, int v) { return a.q[v][n] + (a.din[v] ? a.din[v][n] : 0.0); };
+    auto values = [&](uint64_t n, double* v) {
+        Prim<double> P;
+        P.rp = Q(n, 0); P.m[0] = Q(n, 1); P.m[1] = Q(n, 2); P.m[2] = Q(n, 3); P.tp = Q(n, 4);
+        make_prim(P, d.rho_ref[n], d.p_ref[n], T0, gam, gR);
+        mortar_node_values(P, nv, T0, v);
+    };
+    // the node values' derivatives with respect to the node's state k
+    auto values_d = [&](uint64_t n, int k, D1* v) {
+        Prim<D1> P;
+        P.rp = seeded(Q(n, 0), k == 0);
+        P.m[0] = seeded(Q(n, 1), k == 1); P.m[1] = seeded(Q(n, 2), k == 2); P.m[2] = seeded(Q(n, 3), k == 3);
+        P.tp = seeded(Q(n, 4), k == 4);
+        make_prim(P, D1(d.rho_ref[n]), D1(d.p_ref[n]), D1(T0), D1(gam), D1(gR));
+        mortar_node_values(P, nvD, D1(T0), v);
+    };
+    extern __shared__ double mortar_adj_lds[];
+    double* Lv = mortar_adj_lds;   // [12][np] coarse node values
+    double* Lw = Lv + 12 * np;     // [5][np]  sc w_t' w^(coarse node t')
+    double* Lp = Lw + 5 * np;      // [12][np] cotangents of the interpolated values
+    uint64_t nc = 0;
+    if (act) {
+        nc = face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ);
+        double v[12];
+        values(nc, v);
+        for (int q = 0; q < 12; q++) Lv[q * np + t] = v[q];
+        const double sc = fo ? d.mort_w[ft] : -d.mort_w[ft];
+        const double icv = 1.0 / d.cV[nc];
+        for (int v5 = 0; v5 < 5; v5++) Lw[v5 * np + t] = sc * (a.w[v5][nc] * icv);
+    }
+    __syncthreads();
+    if (act) {
+        const uint64_t nf = d.mort_fine[ft];
+        double vf[12], vc[12];
+        values(nf, vf);
+        for (int q = 0; q < 12; q++) vc[q] = 0.0;
+        for (int an = 0; an < mf.n0; an++)
+            for (int bn = 0; bn < mf.n1; bn++) {
+                const double w = R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo];
+                for (int q = 0; q < 12; q++) vc[q] += w * Lv[q * np + an * mf.n1 + bn];
+            }
+        // cotangent of phi_t
+        const double sf = fo ? -d.mort_w[ft] : d.mort_w[ft];
+        const double icvf = 1.0 / d.cV[nf];
+        double phib[5];
+        for (int v5 = 0; v5 < 5; v5++) phib[v5] = sf * (a.w[v5][nf] * icvf);
+        for (int an = 0; an < mf.n0; an++)
+            for (int bn = 0; bn < mf.n1; bn++) {
+                const double c = C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn];
+                for (int v5 = 0; v5 < 5; v5++) phib[v5] += c * Lw[v5 * np + an * mf.n1 + bn];
+            }
+        const D1 bg(d.mort_bg[ft]);
+        // the refined node's state
+#pragma nounroll
+        for (int k = 0; k < 5; k++) {
+            D1 vfD[12], vcD[12], phi[5];
+            values_d(nf, k, vfD);
+            for (int q = 0; q < 12; q++) vcD[q] = D1(vc[q]);
+            mortar_rusanov(vfD, vcD, bg, nvD, fo, D1(T0), phi);
+            double acc = 0;
+            for (int v5 = 0; v5 < 5; v5++) acc += phib[v5] * phi[v5].d[0];
+            atomicAdd(d.madj + (size_t)k * nb + nf, acc);
+        }
+        // the interpolated values
+#pragma nounroll
+        for (int j = 0; j < 12; j++) {
+            D1 vfD[12], vcD[12], phi[5];
+            for (int q = 0; q < 12; q++) {
+                vfD[q] = D1(vf[q]);
+                vcD[q] = seeded(vc[q], q == j);
+            }
+            mortar_rusanov(vfD, vcD, bg, nvD, fo, D1(T0), phi);
+            double acc = 0;
+            for (int v5 = 0; v5 < 5; v5++) acc += phib[v5] * phi[v5].d[0];
+            Lp[j * np + t] = acc;
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+    // coarse side node (ao, bo): psiRef transposed, then the node values'
+    // derivatives
+    double gbar[12];
+    for (int q = 0; q < 12; q++) gbar[q] = 0.0;
+    for (int a2 = 0; a2 < mf.n0; a2++)
+        for (int b2 = 0; b2 < mf.n1; b2++) {
+            const double r = R0[ao * mf.n0 + a2] * R1[bo * mf.n1 + b2];
+            for (int q = 0; q < 12; q++) gbar[q] += r * Lp[q * np + a2 * mf.n1 + b2];
+        }
+#pragma nounroll
+    for (int k = 0; k < 5; k++) {
+        D1 v[12];
+        values_d(nc, k, v);
+        double acc = 0;
+        for (int q = 0; q < 12; q++) acc += gbar[q] * v[q].d[0];
+        atomicAdd(d.madj + (size_t)k * nb + nc, acc);
     }
 }
 
@@ -303,6 +439,15 @@ size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb) {
 }
 
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
+    if (d.nmort) {
+        if (d.npfm > 256) return hipErrorInvalidValue;
+        hipError_t e = hipMemsetAsync(d.madj, 0, 5 * d.nb * sizeof(double), st);
+        if (e != hipSuccess) return e;
+        const int threads = (d.npfm + 63) / 64 * 64;
+        hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), (size_t)29 * d.npfm * sizeof(double),
+                           st, d, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_vjp_t<8, 8, 8>(d, a, st);
     if (d.npx == 6 && d.npy == 1 && d.npz == 6) return launch_vjp_t<6, 1, 6>(d, a, st);
     if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_vjp_t<5, 1, 5>(d, a, st);

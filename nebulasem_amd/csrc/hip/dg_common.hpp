@@ -102,6 +102,11 @@ struct DgDev {
     const uint32_t* cslot;
     double* mphi;
     double* msurf;
+    // tangents of mphi / msurf (tangent-linear stages) and the mortar faces'
+    // share of a vector-Jacobian product [5][nb] (adjoint)
+    double* mphi_t;
+    double* msurf_t;
+    double* madj;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

@@ -379,6 +379,10 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         HIP_TRY(d->mphi.alloc(mfaces.size() * npfm * 16 * sizeof(double)));
         HIP_TRY(d->msurf.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d->msurf.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
+        HIP_TRY(d->mphi_t.alloc(mfaces.size() * npfm * 5 * sizeof(double)));
+        HIP_TRY(d->msurf_t.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(d->msurf_t.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
+        HIP_TRY(d->madj.alloc(5 * nb * sizeof(double)));
     }
     HIP_TRY(d->bc.alloc(sizeof(BcTable)));
     HIP_TRY(hipMemcpyAsync(d->bc.p, &bt, sizeof(BcTable), hipMemcpyHostToDevice, s));
@@ -436,6 +440,9 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.cslot = d->cslot.as<uint32_t>();
     dv.mphi = d->mphi.as<double>();
     dv.msurf = d->msurf.as<double>();
+    dv.mphi_t = d->mphi_t.as<double>();
+    dv.msurf_t = d->msurf_t.as<double>();
+    dv.madj = d->madj.as<double>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -628,8 +635,6 @@ int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double,
 int nsem_mol_supported(const nsem_dg* d) {
     if (d->prm.diffusion)
         return nsem_fail(NSEM_EUNSUPPORTED, "the derivative entry points do not carry artificial viscosity");
-    if (d->dev.nmort)
-        return nsem_fail(NSEM_EUNSUPPORTED, "the derivative entry points do not take 2:1 non-conforming meshes");
     return NSEM_OK;
 }
 

@@ -239,4 +239,42 @@ __device__ __forceinline__ void rusanov(const Prim<R>& O, const Prim<R>& N, cons
     flux[4] = (mnO * T0 + mnO * O.thp) * fI + (mnN * T0 + mnN * N.thp) * fJ - lam * (N.tp - O.tp) * mn;
 }
 
+// The 12 node values a mortar face interpolates from the coarse side (and
+// takes at the refined node): the normal fluxes through nv (5), rho', m (3),
+// Theta', |U|, c.
+template <typename R>
+__device__ __forceinline__ void mortar_node_values(const Prim<R>& P, const R nv[3], R T0, R* v) {
+    const R mn = P.m[0] * nv[0] + P.m[1] * nv[1] + P.m[2] * nv[2];
+    const R un = P.u[0] * nv[0] + P.u[1] * nv[1] + P.u[2] * nv[2];
+    v[0] = mn;
+    for (int r = 0; r < 3; r++) v[1 + r] = P.m[r] * un + P.pp * nv[r];
+    v[4] = mn * T0 + mn * P.thp;
+    v[5] = P.rp;
+    for (int c = 0; c < 3; c++) v[6 + c] = P.m[c];
+    v[9] = P.tp;
+    v[10] = P.umag;
+    v[11] = P.c;
+}
+
+// Rusanov flux per unit face-node weight (phi) from the refined node's
+// values vf and the coarse side's interpolated values vc, at the interior
+// weight 1/2; owner O, neighbour N (fo: the refined element owns the face).
+// The background density and Theta enter the jumps through bg (coarse
+// interpolation minus the refined node's; Theta_ref = rho_ref T0).
+template <typename R>
+__device__ __forceinline__ void mortar_rusanov(const R* vf, const R* vc, R bg, const R nv[3], bool fo, R T0,
+                                               R* phi) {
+    const R drho = fo ? (vc[5] - vf[5]) + bg : (vf[5] - vc[5]) - bg;   // rho_N - rho_O
+    const R dth = fo ? (vc[9] - vf[9]) + T0 * bg : (vf[9] - vc[9]) - T0 * bg;
+    R dm[3];
+    for (int c = 0; c < 3; c++) dm[c] = fo ? vc[6 + c] - vf[6 + c] : vf[6 + c] - vc[6 + c];
+    const R lam = (((vf[10] + vc[10]) * R(0.5)) + ((vf[11] + vc[11]) * R(0.5))) * R(0.5);
+    const R mn = r_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    const R imn = r_div(R(1), mn);
+    const R dmn = lam * (dm[0] * nv[0] + dm[1] * nv[1] + dm[2] * nv[2]);
+    phi[0] = (vf[0] + vc[0]) * R(0.5) - lam * drho * mn;
+    for (int r = 0; r < 3; r++) phi[1 + r] = (vf[1 + r] + vc[1 + r]) * R(0.5) - (nv[r] * imn) * dmn;
+    phi[4] = (vf[4] + vc[4]) * R(0.5) - lam * dth * mn;
+}
+
 }  // namespace nsem

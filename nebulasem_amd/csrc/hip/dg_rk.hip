@@ -413,8 +413,12 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             if ((meta >> 6) & 1) {
                 // 2:1 mortar side: its surface term, sign included, from the
                 // stage's mortar pre-pass
-                const double* ms = d.msurf + ((size_t)(meta >> 8) * d.npfm + fa * (g < 2 ? NY : NZ) + fb) * 5;
-                for (int v = 0; v < 5; v++) res[v] += (R)ms[v];
+                const size_t mi = ((size_t)(meta >> 8) * d.npfm + fa * (g < 2 ? NY : NZ) + fb) * 5;
+                for (int v = 0; v < 5; v++) {
+                    R x = (R)d.msurf[mi + v];
+                    if constexpr (is_dual<R>::value) x.d[0] = d.msurf_t[mi + v];
+                    res[v] += x;
+                }
                 continue;
             }
         }
@@ -652,7 +656,7 @@ static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks
                              hipStream_t st) {
     // the tangent-linear instantiation has no viscous variant (the
     // derivative entry points refuse diffusion)
-    if constexpr (!is_dual<R>::value) {
+    {
         // 2:1 non-conforming meshes (inviscid)
         if (d.nmort) {
             if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, false, true>(d, a, blocks, threads, lds, st);
@@ -703,9 +707,10 @@ static hipError_t launch_rk_t(const DgDev& d, const RkArgs<S>& a_in, hipStream_t
 // weighted where it is used.
 //
 // mortar_flux_kernel: one work-group per mortar face, one thread per face
-// node: the coarse side's node values (normal fluxes, state, |U|, c) once
-// per node into LDS, then per refined face node their interpolation, phi,
-// and the refined side's surface term.
+// node: the coarse side's node values once per node into LDS, then per
+// refined face node their interpolation, phi, and the refined side's
+// surface term.  On dual numbers (tangent-linear stages) the tangents go to
+// mphi_t / msurf_t.
 template <typename R, typename S, bool DIN>
 __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) {
     const uint32_t f = blockIdx.x;
@@ -717,13 +722,6 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
     const size_t ft = (size_t)f * d.npfm + t;
     const R T0 = (R)d.T0, gam = (R)d.gamma, gR = (R)(d.gamma * d.R);
     const R nv[3] = {(R)mf.nf[0], (R)mf.nf[1], (R)mf.nf[2]};
-    auto normal_fluxes = [&](const Prim<R>& P, R* fn) {
-        const R mn = P.m[0] * nv[0] + P.m[1] * nv[1] + P.m[2] * nv[2];
-        const R un = P.u[0] * nv[0] + P.u[1] * nv[1] + P.u[2] * nv[2];
-        fn[0] = mn;
-        for (int r = 0; r < 3; r++) fn[1 + r] = P.m[r] * un + P.pp * nv[r];
-        fn[4] = mn * T0 + mn * P.thp;
-    };
     auto load_prim = [&](uint64_t n, Prim<R>& P) {
         P.rp = load_state<R, S, false, DIN>(a, 0, n);
         P.m[0] = load_state<R, S, false, DIN>(a, 1, n);
@@ -732,76 +730,51 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
         P.tp = load_state<R, S, false, DIN>(a, 4, n);
         make_prim(P, (R)d.rho_ref[n], (R)d.p_ref[n], T0, gam, gR);
     };
-    // coarse side node (ao, bo) of this thread: its 12 values into LDS
-    // [12][npfm]: normal fluxes (5), rho', m (3), Theta', |U|, c
     extern __shared__ unsigned char mortar_lds[];
-    R* L = reinterpret_cast<R*>(mortar_lds);
+    R* L = reinterpret_cast<R*>(mortar_lds);   // [12][npfm]
     const int np = d.npfm;
     if (act) {
         Prim<R> P{};
         load_prim(face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ), P);
-        R fn[5];
-        normal_fluxes(P, fn);
-        for (int v = 0; v < 5; v++) L[v * np + t] = fn[v];
-        L[5 * np + t] = P.rp;
-        for (int c = 0; c < 3; c++) L[(6 + c) * np + t] = P.m[c];
-        L[9 * np + t] = P.tp;
-        L[10 * np + t] = P.umag;
-        L[11 * np + t] = P.c;
+        R v[12];
+        mortar_node_values(P, nv, T0, v);
+        for (int q = 0; q < 12; q++) L[q * np + t] = v[q];
     }
     __syncthreads();
     if (!act) return;
-    // refined side
     Prim<R> Pf{};
     load_prim(d.mort_fine[ft], Pf);
-    R ff[5];
-    normal_fluxes(Pf, ff);
-    // coarse side, interpolated to this node
+    R vf[12], vc[12];
+    mortar_node_values(Pf, nv, T0, vf);
+    for (int q = 0; q < 12; q++) vc[q] = R(0);
     const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
     const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
     const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
-    R fc[5] = {0, 0, 0, 0, 0}, rc = 0, mc[3] = {0, 0, 0}, tc = 0, uc = 0, cc = 0;
     for (int an = 0; an < mf.n0; an++)
         for (int bn = 0; bn < mf.n1; bn++) {
             const R w = (R)(R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo]);
             const int s2 = an * mf.n1 + bn;
-            for (int v = 0; v < 5; v++) fc[v] += w * L[v * np + s2];
-            rc += w * L[5 * np + s2];
-            for (int c = 0; c < 3; c++) mc[c] += w * L[(6 + c) * np + s2];
-            tc += w * L[9 * np + s2];
-            uc += w * L[10 * np + s2];
-            cc += w * L[11 * np + s2];
+            for (int q = 0; q < 12; q++) vc[q] += w * L[q * np + s2];
         }
-    // Rusanov at the interior weight 1/2, owner O and neighbour N; the
-    // background density and Theta enter the jumps through mort_bg
-    // (coarse minus refined, Theta_ref = rho_ref T0)
-    const bool fo = mf.fine_owner;
-    const R bg = (R)d.mort_bg[ft];
-    const R drho = fo ? (rc - Pf.rp) + bg : (Pf.rp - rc) - bg;   // rho_N - rho_O
-    const R dth = fo ? (tc - Pf.tp) + T0 * bg : (Pf.tp - tc) - T0 * bg;
-    R dm[3];
-    for (int c = 0; c < 3; c++) dm[c] = fo ? mc[c] - Pf.m[c] : Pf.m[c] - mc[c];
-    const R lam = (((Pf.umag + uc) * R(0.5)) + ((Pf.c + cc) * R(0.5))) * R(0.5);
-    const R mn = r_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-    const R imn = r_div(R(1), mn);
-    const R dmn = lam * (dm[0] * nv[0] + dm[1] * nv[1] + dm[2] * nv[2]);
     R phi[5];
-    phi[0] = (ff[0] + fc[0]) * R(0.5) - lam * drho * mn;
-    for (int r = 0; r < 3; r++) phi[1 + r] = (ff[1 + r] + fc[1 + r]) * R(0.5) - (nv[r] * imn) * dmn;
-    phi[4] = (ff[4] + fc[4]) * R(0.5) - lam * dth * mn;
+    mortar_rusanov(vf, vc, (R)d.mort_bg[ft], nv, mf.fine_owner != 0, T0, phi);
     // the refined element: -w phi when it owns the face, +w phi otherwise
-    const double wgt = d.mort_w[ft];
+    const double wgt = d.mort_w[ft], sw = mf.fine_owner ? -wgt : wgt;
     double* ms = d.msurf + (size_t)d.mort_fms[ft] * 5;
     for (int v = 0; v < 5; v++) {
-        d.mphi[ft * 5 + v] = (double)phi[v];
-        ms[v] = (fo ? -wgt : wgt) * (double)phi[v];
+        d.mphi[ft * 5 + v] = value_of(phi[v]);
+        ms[v] = sw * value_of(phi[v]);
+        if constexpr (is_dual<R>::value) {
+            d.mphi_t[ft * 5 + v] = phi[v].d[0];
+            d.msurf_t[(size_t)d.mort_fms[ft] * 5 + v] = sw * phi[v].d[0];
+        }
     }
 }
 
 // mortar_gather_kernel: one work-group per split coarse side, one thread per
 // side node: the sum over the side's faces of w(node) psiCor(phi), with the
 // coarse element's sign.
-__global__ void __launch_bounds__(256) mortar_gather_kernel(DgDev d) {
+__global__ void __launch_bounds__(256) mortar_gather_kernel(DgDev d, const double* mphi, double* msurf) {
     const uint32_t* cs = d.cslot + (size_t)blockIdx.x * 8;
     const int side = (int)cs[1], nch = (int)cs[2];
     const uint32_t slot = cs[3];
@@ -816,7 +789,7 @@ __global__ void __launch_bounds__(256) mortar_gather_kernel(DgDev d) {
         const MortarFace mf = d.mort[f];
         const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
         const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
-        const double* ph = d.mphi + (size_t)f * d.npfm * 5;
+        const double* ph = mphi + (size_t)f * d.npfm * 5;
         double g[5] = {0, 0, 0, 0, 0};
         for (int ao = 0; ao < mf.n0; ao++)
             for (int bo = 0; bo < mf.n1; bo++) {
@@ -829,7 +802,7 @@ __global__ void __launch_bounds__(256) mortar_gather_kernel(DgDev d) {
         const double sgn = mf.fine_owner ? wk : -wk;   // the coarse element owns the face: subtract
         for (int v = 0; v < 5; v++) sum[v] += sgn * g[v];
     }
-    double* ms = d.msurf + ((size_t)slot * d.npfm + t) * 5;
+    double* ms = msurf + ((size_t)slot * d.npfm + t) * 5;
     for (int v = 0; v < 5; v++) ms[v] = sum[v];
 }
 
@@ -845,17 +818,21 @@ static hipError_t launch_mortar(const DgDev& d, const RkArgs<S>& a, hipStream_t 
         hipLaunchKernelGGL((mortar_flux_kernel<R, S, true>), dim3(d.nmort), dim3(threads), lds, st, d, a);
     else
         hipLaunchKernelGGL((mortar_flux_kernel<R, S, false>), dim3(d.nmort), dim3(threads), lds, st, d, a);
-    if (d.ncslot) hipLaunchKernelGGL(mortar_gather_kernel, dim3(d.ncslot), dim3(threads), 0, st, d);
+    if (d.ncslot) {
+        hipLaunchKernelGGL(mortar_gather_kernel, dim3(d.ncslot), dim3(threads), 0, st, d, (const double*)d.mphi,
+                           d.msurf);
+        if constexpr (is_dual<R>::value)
+            hipLaunchKernelGGL(mortar_gather_kernel, dim3(d.ncslot), dim3(threads), 0, st, d,
+                               (const double*)d.mphi_t, d.msurf_t);
+    }
     return hipGetLastError();
 }
 
 template <typename R, typename S>
 hipError_t launch_rk_stage(const DgDev& d, const RkArgs<S>& a, hipStream_t st) {
-    if constexpr (!is_dual<R>::value) {
-        if (d.nmort) {
-            const hipError_t e = launch_mortar<R, S>(d, a, st);
-            if (e != hipSuccess) return e;
-        }
+    if (d.nmort) {
+        const hipError_t e = launch_mortar<R, S>(d, a, st);
+        if (e != hipSuccess) return e;
     }
     return launch_rk_shape<R, S>(d, a, st);
 }

@@ -10,12 +10,15 @@ O(h^2) and rounding O(eps/h) both well below that)."""
 import numpy as np
 import pytest
 
-from _cases import EULER_CASES
+from _cases import AMR_CASES, EULER_CASES
 from test_gpu_dg_split import setup
 
 pytestmark = pytest.mark.gpu
 
-CASES = [c for c in EULER_CASES if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_wall_n3")]
+# plus the inviscid 2:1 non-conforming decks (the mortar pre-pass in the
+# tangent-linear stages, its transpose in the vector-Jacobian product)
+CASES = ([c for c in EULER_CASES if c["name"] in ("cfg0_2d_n4", "e3_n3", "e3_wall_n3")]
+         + [c for c in AMR_CASES if c.get("viscosity") is None])
 
 
 def conserved(dg):

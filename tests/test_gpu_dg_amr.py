@@ -67,17 +67,11 @@ def test_amr_rk3_fp32_against_fp64(oracle, tmp_path, case):
 
 
 def test_amr_unsupported_paths_say_so(tmp_path):
-    """The derivative entry points and artificial viscosity do not carry the
-    mortar projections yet: they refuse a non-conforming mesh instead of
-    treating its split sides as conforming.  (The split step and the
-    method-of-lines steps carry them; their parity tests are in
-    test_gpu_dg_split.py and above.)"""
-    case = INVISCID[0]
-    mesh, f = load_case(tmp_path, case)
-    dg = DGEuler.from_fields(mesh, euler_params(case["dt"]), f)
-    with pytest.raises(NsemError, match="non-conforming"):
-        dg.residual_vjp(np.zeros((5, mesh.info.nbcs_field)))
-    dg.close()
+    """Artificial viscosity does not carry the mortar projections yet: the
+    steps refuse a viscous problem on a non-conforming mesh instead of
+    treating its split sides as conforming.  (The split step, the
+    method-of-lines steps and the derivative entry points carry them; their
+    tests are in test_gpu_dg_split.py, above and test_gpu_dg_adjoint.py.)"""
     visc = [c for c in AMR_CASES if c.get("viscosity") is not None][0]
     mesh, f = load_case(tmp_path / "v", visc)
     dg = DGEuler.from_fields(mesh, euler_params(visc["dt"], diffusion=1, viscosity=visc["viscosity"]), f)
