@@ -449,6 +449,7 @@ hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_vjp_t<8, 8, 8>(d, a, st);
+    if (d.npx == 6 && d.npy == 6 && d.npz == 6) return launch_vjp_t<6, 6, 6>(d, a, st);
     if (d.npx == 6 && d.npy == 1 && d.npz == 6) return launch_vjp_t<6, 1, 6>(d, a, st);
     if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_vjp_t<5, 1, 5>(d, a, st);
     return launch_vjp_t<0, 0, 0>(d, a, st);
