@@ -107,6 +107,8 @@ struct DgDev {
     double* mphi_t;
     double* msurf_t;
     double* madj;
+    // the split step's strong-gradient lift of the mortar sides [slot][npfm][12]
+    double* mgrad;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

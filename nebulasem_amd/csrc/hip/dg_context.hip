@@ -383,6 +383,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         HIP_TRY(d->msurf_t.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d->msurf_t.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
         HIP_TRY(d->madj.alloc(5 * nb * sizeof(double)));
+        if (prm->diffusion) HIP_TRY(d->mgrad.alloc((size_t)nslots * npfm * 12 * sizeof(double)));
     }
     HIP_TRY(d->bc.alloc(sizeof(BcTable)));
     HIP_TRY(hipMemcpyAsync(d->bc.p, &bt, sizeof(BcTable), hipMemcpyHostToDevice, s));
@@ -443,6 +444,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.mphi_t = d->mphi_t.as<double>();
     dv.msurf_t = d->msurf_t.as<double>();
     dv.madj = d->madj.as<double>();
+    dv.mgrad = d->mgrad.as<double>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -567,9 +569,6 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
 
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    if (d->dev.nmort && d->prm.diffusion)
-        return nsem_fail(NSEM_EUNSUPPORTED,
-                         "artificial viscosity on 2:1 non-conforming meshes is not supported by the split step");
     for (int32_t s = 0; s < nsteps; s++) {
         DevBuf* A = d->st[d->cur];
         DevBuf* B = d->st[1 - d->cur];

@@ -42,7 +42,7 @@ struct nsem_dg {
     DevBuf JcV, cV, JcV32, cV32, p_ref, rho_ref, eface, gFN, D[3], w[3], bc, emet, wnode;
     // 2:1 non-conforming faces (empty on conforming meshes)
     DevBuf mort, mort_fine, mort_fms, mort_w, mort_bg, psiRef[3], psiCor[3], cslot, mphi, msurf, mphi_t, msurf_t,
-        madj;
+        madj, mgrad;
     // double-buffered state: rho ux uy uz T p
     DevBuf st[2][6];
     int cur = 0;

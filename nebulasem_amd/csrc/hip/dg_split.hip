@@ -199,6 +199,12 @@ __global__ void __launch_bounds__(512) split_grad_kernel(DgDev d, SplitState s, 
         const int og = ef.meta & 7;
         int a, b;
         face_ab(g, i, j, k, a, b);
+        if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed lift from the pre-pass
+            const double* mg = d.mgrad + ((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 12;
+            for (int c = 0; c < 9; c++) gu[c] += mg[c];
+            for (int c = 0; c < 3; c++) gt[c] += mg[9 + c];
+            continue;
+        }
         const double wgt = face_weight(d, own ? g : og, a, b);
         const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
                               d.gFN[3 * (size_t)ef.facet + 2] * wgt};
@@ -524,6 +530,7 @@ __global__ void __launch_bounds__(256) split_mortar_rho_kernel(DgDev d, SplitSta
 // momentum and theta pass: F_U = (rho^n U) (x) U + p' I, q_U = rho^{n+1} U;
 // F_theta = (rho^n U) theta, q_theta = rho^{n+1} theta; lambda of the first
 // pass
+template <bool VISC>
 __global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitState s, const double* rho1) {
     const uint32_t f = blockIdx.x;
     const int t = threadIdx.x;
@@ -543,6 +550,11 @@ __global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitStat
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) v[3 * row + c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
         for (int c = 0; c < 3; c++) v[9 + c] = fc[c] * th;
+        if constexpr (VISC) {   // - mu grad U, - mu/Pr grad theta; mu = rho^n visc
+            const double mu = rho0 * d.visc, imu = mu * d.iPr;
+            for (int c = 0; c < 9; c++) v[c] = v[c] - mu * d.grad[(size_t)c * d.nb + m];
+            for (int c = 0; c < 3; c++) v[9 + c] = v[9 + c] - imu * d.grad[(size_t)(9 + c) * d.nb + m];
+        }
         for (int c = 0; c < 3; c++) v[12 + c] = rho * u[c];
         v[15] = rho * th;
     };
@@ -598,6 +610,113 @@ __global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitStat
         const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
         ms[4] = fo ? flux : -flux;
     }
+}
+
+// strong-gradient pass (diffusion): cds of U and theta at the refined face
+// nodes (scatter), and the lift fN (x) (face value - node value) of the
+// refined node, signed as grad_flux<true> adds it (field.h:3050-3082); the
+// face values are kept in mphi 12-15 for the gather
+__global__ void __launch_bounds__(256) split_mortar_grad_kernel(DgDev d, SplitState s) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    const bool act = t < mf.n0 * mf.n1;
+    int ao, bo;
+    mortar_node_ids(d, mf, t, ao, bo);
+    const int np = d.npfm;
+    auto vals = [&](uint64_t m, double* v) {
+        v[0] = s.ux[m]; v[1] = s.uy[m]; v[2] = s.uz[m];
+        v[3] = s.T[m] + d.T0;
+    };
+    extern __shared__ double L[];  // [4][npfm]
+    if (act) {
+        double v[4];
+        vals(face_node(mf.coarse, mf.side, ao, bo, d.npx, d.npy, d.npz), v);
+        for (int q = 0; q < 4; q++) L[q * np + t] = v[q];
+    }
+    __syncthreads();
+    if (!act) return;
+    const size_t ft = (size_t)f * np + t;
+    double vf[4], vc[4];
+    vals(d.mort_fine[ft], vf);
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    for (int q = 0; q < 4; q++) {
+        double sum = 0.0;
+        for (int an = 0; an < mf.n0; an++)
+            for (int bn = 0; bn < mf.n1; bn++)
+                sum += L[q * np + an * mf.n1 + bn] * R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo];
+        vc[q] = sum;
+    }
+    const bool fo = mf.fine_owner;
+    const double* vO = fo ? vf : vc;
+    const double* vN = fo ? vc : vf;
+    const double fI = 0.5;
+    double* ph = d.mphi + ft * kSplitPhi;
+    double fF[4];
+    for (int q = 0; q < 4; q++) {
+        fF[q] = vO[q] * fI + vN[q] * (1 - fI);
+        ph[12 + q] = fF[q];
+    }
+    const double wk = d.mort_w[ft];
+    const double nv[3] = {mf.nf[0] * wk, mf.nf[1] * wk, mf.nf[2] * wk};
+    double* mg = d.mgrad + (size_t)d.mort_fms[ft] * 12;
+    for (int aa = 0; aa < 3; aa++) {
+        for (int c = 0; c < 3; c++) {
+            const double x = nv[aa] * (fF[c] - vf[c]);
+            mg[3 * aa + c] = fo ? x : -x;
+        }
+        const double x = nv[aa] * (fF[3] - vf[3]);
+        mg[9 + aa] = fo ? x : -x;
+    }
+}
+
+// gather of the gradient pass: per coarse side node, over the side's faces,
+// the face values projected with psiCor minus the node's own value, times
+// the face normal at that node, signed for the coarse element
+__global__ void __launch_bounds__(256) split_mortar_grad_gather_kernel(DgDev d, SplitState s) {
+    const uint32_t* cs = d.cslot + (size_t)blockIdx.x * 8;
+    const uint32_t elem = cs[0];
+    const int side = (int)cs[1], nch = (int)cs[2];
+    const uint32_t slot = cs[3];
+    const MortarFace m0 = d.mort[cs[4]];
+    const int t = threadIdx.x;
+    if (t >= m0.n0 * m0.n1) return;
+    const int an = t / m0.n1, bn = t - an * m0.n1;
+    const int d0 = side < 4 ? 0 : 1, d1 = side < 2 ? 1 : 2;
+    const uint64_t nc = face_node(elem, side, an, bn, d.npx, d.npy, d.npz);
+    const double xc[4] = {s.ux[nc], s.uy[nc], s.uz[nc], s.T[nc] + d.T0};
+    double out[12];
+    for (int q = 0; q < 12; q++) out[q] = 0.0;
+    for (int ch = 0; ch < nch; ch++) {
+        const uint32_t f = cs[4 + ch];
+        const MortarFace mf = d.mort[f];
+        const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+        const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+        const double* ph = d.mphi + (size_t)f * d.npfm * kSplitPhi;
+        double G[4];
+        for (int q = 0; q < 4; q++) {
+            double sum = 0.0;
+            for (int ao = 0; ao < mf.n0; ao++)
+                for (int bo = 0; bo < mf.n1; bo++)
+                    sum += ph[(ao * mf.n1 + bo) * kSplitPhi + 12 + q] * C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn];
+            G[q] = sum;
+        }
+        const double wk = d.mort_w[(size_t)f * d.npfm + t];
+        const double nv[3] = {mf.nf[0] * wk, mf.nf[1] * wk, mf.nf[2] * wk};
+        const bool coarse_owns = !mf.fine_owner;
+        for (int aa = 0; aa < 3; aa++) {
+            for (int c = 0; c < 3; c++) {
+                const double x = nv[aa] * (G[c] - xc[c]);
+                out[3 * aa + c] = coarse_owns ? out[3 * aa + c] + x : out[3 * aa + c] - x;
+            }
+            const double x = nv[aa] * (G[3] - xc[3]);
+            out[9 + aa] = coarse_owns ? out[9 + aa] + x : out[9 + aa] - x;
+        }
+    }
+    double* mg = d.mgrad + ((size_t)slot * d.npfm + t) * 12;
+    for (int q = 0; q < 12; q++) mg[q] = out[q];
 }
 
 // gather of either pass to the split coarse sides: per coarse side node,
@@ -664,14 +783,22 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
     size_t lds_b = (size_t)epb * 15 * NP * sizeof(double);
     size_t lds_g = (size_t)epb * 4 * NP * sizeof(double);
     hipError_t err;
+    const int mthreads = (d.npfm + 63) / 64 * 64;
+    if (d.nmort && d.npfm > 256) return hipErrorInvalidValue;
     if (d.diffusion) {
         if (!grad || grad != d.grad) return hipErrorInvalidValue;
+        if (d.nmort) {
+            if (!d.mgrad) return hipErrorInvalidValue;
+            hipLaunchKernelGGL(split_mortar_grad_kernel, dim3(d.nmort), dim3(mthreads),
+                               (size_t)4 * d.npfm * sizeof(double), st, d, s);
+            if (d.ncslot)
+                hipLaunchKernelGGL(split_mortar_grad_gather_kernel, dim3(d.ncslot), dim3(mthreads), 0, st, d, s);
+            if ((err = hipGetLastError()) != hipSuccess) return err;
+        }
         hipLaunchKernelGGL((split_grad_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_g, st, d, s, grad);
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
-    const int mthreads = (d.npfm + 63) / 64 * 64;
     if (d.nmort) {
-        if (d.diffusion || d.npfm > 256) return hipErrorInvalidValue;
         hipLaunchKernelGGL(split_mortar_rho_kernel, dim3(d.nmort), dim3(mthreads),
                            (size_t)6 * d.npfm * sizeof(double), st, d, s);
         if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<false>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
@@ -680,8 +807,12 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
     hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (d.nmort) {
-        hipLaunchKernelGGL(split_mortar_ut_kernel, dim3(d.nmort), dim3(mthreads),
-                           (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
+        if (d.diffusion)
+            hipLaunchKernelGGL(split_mortar_ut_kernel<true>, dim3(d.nmort), dim3(mthreads),
+                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
+        else
+            hipLaunchKernelGGL(split_mortar_ut_kernel<false>, dim3(d.nmort), dim3(mthreads),
+                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
         if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<true>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }

@@ -50,8 +50,9 @@ def setup(oracle, tmp_path, case):
     return mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p)
 
 
-# plus the inviscid 2:1 non-conforming decks (mortar pre-pass of each pass)
-SPLIT_CASES = EULER_CASES + [c for c in AMR_CASES if c.get("viscosity") is None]
+# plus the 2:1 non-conforming decks, inviscid and viscous (mortar pre-pass
+# of each pass, the strong gradient's included)
+SPLIT_CASES = EULER_CASES + AMR_CASES
 
 
 @pytest.mark.parametrize("case", SPLIT_CASES, ids=[c["name"] for c in SPLIT_CASES])
