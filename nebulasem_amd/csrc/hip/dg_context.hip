@@ -637,20 +637,12 @@ int nsem_mol_supported(const nsem_dg* d) {
     return NSEM_OK;
 }
 
-// The method-of-lines steps on a 2:1 non-conforming mesh carry the mortar
-// projections of the inviscid flux only.
-static int rk3_supported(const nsem_dg* d) {
-    if (d->dev.nmort && d->prm.diffusion)
-        return nsem_fail(NSEM_EUNSUPPORTED,
-                         "artificial viscosity on 2:1 non-conforming meshes is not supported by the stage kernels");
-    return NSEM_OK;
-}
+
 
 extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    if (int rs = rk3_supported(d)) return rs;
     int rc;
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
     else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
@@ -669,7 +661,6 @@ int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
 
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
     if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
-    if (int rs = rk3_supported(d)) return rs;
     int rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);

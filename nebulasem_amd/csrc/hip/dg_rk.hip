@@ -506,6 +506,12 @@ __global__ void __launch_bounds__(512) mol_grad_kernel(DgDev d, MolGradIn<S> in,
         const int og = ef.meta & 7;
         int a, b;
         face_ab(g, i, j, k, a, b);
+        if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed lift from the pre-pass
+            const double* mg = d.mgrad + ((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 12;
+            for (int c = 0; c < 9; c++) gu[c] += (R)mg[c];
+            for (int c = 0; c < 3; c++) gt[c] += (R)mg[9 + c];
+            continue;
+        }
         const double wgt = face_weight(d, own ? g : og, a, b);
         const R nv[3] = {(R)(d.gFN[3 * (size_t)ef.facet] * wgt), (R)(d.gFN[3 * (size_t)ef.facet + 1] * wgt),
                          (R)(d.gFN[3 * (size_t)ef.facet + 2] * wgt)};
@@ -575,6 +581,105 @@ __global__ void __launch_bounds__(512) mol_grad_kernel(DgDev d, MolGradIn<S> in,
     for (int c = 0; c < 3; c++) grad[(size_t)(9 + c) * d.nb + n] = gt[c] * icv;
 }
 
+// 2:1 mortar faces of the method-of-lines gradient (as the split step's:
+// cds of U and theta' at the refined face nodes with the coarse side
+// interpolated by psiRef, the refined node's lift fN (x) (face - node), the
+// face values gathered by psiCor for the coarse side's lift), in R, stored
+// in mgrad as double; mphi 12-15 hold the face values.
+template <typename R, typename S, bool DIN>
+__global__ void __launch_bounds__(256) mol_mortar_grad_kernel(DgDev d, MolGradIn<S> in) {
+    const uint32_t f = blockIdx.x;
+    const int t = threadIdx.x;
+    const MortarFace mf = d.mort[f];
+    const bool act = t < mf.n0 * mf.n1;
+    const int ao = t / mf.n1, bo = t - ao * mf.n1;
+    const int np = d.npfm;
+    const R T0 = (R)d.T0;
+    auto vals = [&](uint64_t m, R* v) {
+        R Qv[5];
+        for (int q = 0; q < 5; q++) Qv[q] = (R)(in.q[q][m] + (DIN ? (double)in.din[q][m] : 0.0));
+        const R ir = r_div(R(1), (R)d.rho_ref[m] + Qv[0]);
+        v[0] = Qv[1] * ir; v[1] = Qv[2] * ir; v[2] = Qv[3] * ir;
+        v[3] = (Qv[4] - T0 * Qv[0]) * ir;
+    };
+    extern __shared__ unsigned char mortar_grad_lds[];
+    R* L = reinterpret_cast<R*>(mortar_grad_lds);   // [4][npfm]
+    if (act) {
+        R v[4];
+        vals(face_node(mf.coarse, mf.side, ao, bo, d.npx, d.npy, d.npz), v);
+        for (int q = 0; q < 4; q++) L[q * np + t] = v[q];
+    }
+    __syncthreads();
+    if (!act) return;
+    const size_t ft = (size_t)f * np + t;
+    R vf[4], vc[4] = {0, 0, 0, 0};
+    vals(d.mort_fine[ft], vf);
+    const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
+    const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+    const double* R1 = d.psiRef[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+    for (int an = 0; an < mf.n0; an++)
+        for (int bn = 0; bn < mf.n1; bn++) {
+            const R w = (R)(R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo]);
+            for (int q = 0; q < 4; q++) vc[q] += w * L[q * np + an * mf.n1 + bn];
+        }
+    R fF[4];
+    for (int q = 0; q < 4; q++) {
+        fF[q] = (vf[q] + vc[q]) * R(0.5);
+        d.mphi[ft * 16 + 12 + q] = (double)fF[q];
+    }
+    const double wk = d.mort_w[ft];
+    const R nv[3] = {(R)(mf.nf[0] * wk), (R)(mf.nf[1] * wk), (R)(mf.nf[2] * wk)};
+    const R sg = mf.fine_owner ? R(1) : R(-1);
+    double* mg = d.mgrad + (size_t)d.mort_fms[ft] * 12;
+    for (int aa = 0; aa < 3; aa++) {
+        for (int c = 0; c < 3; c++) mg[3 * aa + c] = (double)(sg * (nv[aa] * (fF[c] - vf[c])));
+        mg[9 + aa] = (double)(sg * (nv[aa] * (fF[3] - vf[3])));
+    }
+}
+
+template <typename R, typename S, bool DIN>
+__global__ void __launch_bounds__(256) mol_mortar_grad_gather_kernel(DgDev d, MolGradIn<S> in) {
+    const uint32_t* cs = d.cslot + (size_t)blockIdx.x * 8;
+    const uint32_t elem = cs[0];
+    const int side = (int)cs[1], nch = (int)cs[2];
+    const uint32_t slot = cs[3];
+    const MortarFace m0 = d.mort[cs[4]];
+    const int t = threadIdx.x;
+    if (t >= m0.n0 * m0.n1) return;
+    const int an = t / m0.n1, bn = t - an * m0.n1;
+    const int d0 = side < 4 ? 0 : 1, d1 = side < 2 ? 1 : 2;
+    const uint64_t nc = face_node(elem, side, an, bn, d.npx, d.npy, d.npz);
+    const R T0 = (R)d.T0;
+    R Qv[5];
+    for (int q = 0; q < 5; q++) Qv[q] = (R)(in.q[q][nc] + (DIN ? (double)in.din[q][nc] : 0.0));
+    const R ir = r_div(R(1), (R)d.rho_ref[nc] + Qv[0]);
+    const R xc[4] = {Qv[1] * ir, Qv[2] * ir, Qv[3] * ir, (Qv[4] - T0 * Qv[0]) * ir};
+    R out[12];
+    for (int q = 0; q < 12; q++) out[q] = R(0);
+    for (int ch = 0; ch < nch; ch++) {
+        const uint32_t f = cs[4 + ch];
+        const MortarFace mf = d.mort[f];
+        const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
+        const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
+        const double* ph = d.mphi + (size_t)f * d.npfm * 16;
+        R G[4] = {0, 0, 0, 0};
+        for (int ao = 0; ao < mf.n0; ao++)
+            for (int bo = 0; bo < mf.n1; bo++) {
+                const R w = (R)(C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn]);
+                for (int q = 0; q < 4; q++) G[q] += w * (R)ph[(ao * mf.n1 + bo) * 16 + 12 + q];
+            }
+        const double wk = d.mort_w[(size_t)f * d.npfm + t];
+        const R nv[3] = {(R)(mf.nf[0] * wk), (R)(mf.nf[1] * wk), (R)(mf.nf[2] * wk)};
+        const R sg = mf.fine_owner ? R(-1) : R(1);
+        for (int aa = 0; aa < 3; aa++) {
+            for (int c = 0; c < 3; c++) out[3 * aa + c] += sg * (nv[aa] * (G[c] - xc[c]));
+            out[9 + aa] += sg * (nv[aa] * (G[3] - xc[3]));
+        }
+    }
+    double* mg = d.mgrad + ((size_t)slot * d.npfm + t) * 12;
+    for (int q = 0; q < 12; q++) mg[q] = (double)out[q];
+}
+
 template <typename R, typename S, int X, int Y, int Z>
 static hipError_t launch_mol_gradient_t(const DgDev& d, const MolGradIn<S>& in, R* grad, hipStream_t st) {
     const int NP = d.np;
@@ -597,6 +702,26 @@ static hipError_t launch_mol_gradient_t(const DgDev& d, const MolGradIn<S>& in, 
 template <typename R, typename S>
 hipError_t launch_mol_gradient(const DgDev& d, const double* const q[5], const S* const din[5], R* grad,
                                hipStream_t st) {
+    if (d.nmort) {
+        if (!d.mgrad || d.npfm > 256) return hipErrorInvalidValue;
+        MolGradIn<S> mi{};
+        for (int v = 0; v < 5; v++) { mi.q[v] = q[v]; mi.din[v] = din[v]; }
+        const int threads = (d.npfm + 63) / 64 * 64;
+        const size_t lds = (size_t)4 * d.npfm * sizeof(R);
+        if (din[0]) {
+            hipLaunchKernelGGL((mol_mortar_grad_kernel<R, S, true>), dim3(d.nmort), dim3(threads), lds, st, d, mi);
+            if (d.ncslot)
+                hipLaunchKernelGGL((mol_mortar_grad_gather_kernel<R, S, true>), dim3(d.ncslot), dim3(threads), 0, st,
+                                   d, mi);
+        } else {
+            hipLaunchKernelGGL((mol_mortar_grad_kernel<R, S, false>), dim3(d.nmort), dim3(threads), lds, st, d, mi);
+            if (d.ncslot)
+                hipLaunchKernelGGL((mol_mortar_grad_gather_kernel<R, S, false>), dim3(d.ncslot), dim3(threads), 0,
+                                   st, d, mi);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     MolGradIn<S> in{};
     for (int v = 0; v < 5; v++) { in.q[v] = q[v]; in.din[v] = din[v]; }
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_mol_gradient_t<R, S, 8, 8, 8>(d, in, grad, st);
@@ -659,6 +784,13 @@ static void launch_rk_metric(const DgDev& d, const RkArgs<S>& a, unsigned blocks
     {
         // 2:1 non-conforming meshes (inviscid)
         if (d.nmort) {
+            if constexpr (!is_dual<R>::value) {
+                if (a.grad) {
+                    if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, true, true>(d, a, blocks, threads, lds, st);
+                    else launch_rk_kernel<R, S, X, Y, Z, DIN, false, true, true>(d, a, blocks, threads, lds, st);
+                    return;
+                }
+            }
             if (d.affine) launch_rk_kernel<R, S, X, Y, Z, DIN, true, false, true>(d, a, blocks, threads, lds, st);
             else launch_rk_kernel<R, S, X, Y, Z, DIN, false, false, true>(d, a, blocks, threads, lds, st);
             return;
@@ -733,11 +865,26 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
     extern __shared__ unsigned char mortar_lds[];
     R* L = reinterpret_cast<R*>(mortar_lds);   // [12][npfm]
     const int np = d.npfm;
+    // artificial viscosity: the normal fluxes carry -mu grad U . n and
+    // -mu/Pr grad theta' . n (mu = rho visc), as the face flux rows do
+    const R* grd = static_cast<const R*>(a.grad);
+    auto viscous = [&](uint64_t n, const Prim<R>& P, R* v) {
+        if constexpr (!is_dual<R>::value) {
+            if (!grd) return;
+            R gg[12], vis[4];
+            for (int c = 0; c < 12; c++) gg[c] = grd[(size_t)c * d.nb + n];
+            const R mu = ((R)d.rho_ref[n] + P.rp) * (R)d.visc;
+            viscous_normal(gg, mu, (R)d.iPr, nv, vis);
+            for (int r = 0; r < 4; r++) v[1 + r] = v[1 + r] - vis[r];
+        }
+    };
     if (act) {
         Prim<R> P{};
-        load_prim(face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ), P);
+        const uint64_t ncn = face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ);
+        load_prim(ncn, P);
         R v[12];
         mortar_node_values(P, nv, T0, v);
+        viscous(ncn, P, v);
         for (int q = 0; q < 12; q++) L[q * np + t] = v[q];
     }
     __syncthreads();
@@ -746,6 +893,7 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
     load_prim(d.mort_fine[ft], Pf);
     R vf[12], vc[12];
     mortar_node_values(Pf, nv, T0, vf);
+    viscous(d.mort_fine[ft], Pf, vf);
     for (int q = 0; q < 12; q++) vc[q] = R(0);
     const int d0 = mf.side < 4 ? 0 : 1, d1 = mf.side < 2 ? 1 : 2;
     const double* R0 = d.psiRef[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;

@@ -1,5 +1,5 @@
-"""2:1 non-conforming (AMR) meshes on the HIP method-of-lines path: the
-stage kernels with the mortar pre-pass (scatter of the coarse side to each
+"""2:1 non-conforming (AMR) meshes on the HIP method-of-lines path, with
+and without artificial viscosity: the stage kernels with the mortar pre-pass (scatter of the coarse side to each
 refined face, Rusanov flux there, gather back to the coarse side) against
 the oracle's SSP-RK3 composition, whose spatial operator on these decks is
 bit-identical to the reference's euler (tests/test_oracle.py).
@@ -12,16 +12,11 @@ import numpy as np
 import pytest
 
 from _cases import AMR_CASES
-from _euler_setup import load_case
-from nebulasem_amd import DGEuler, NsemError, euler_params
 from test_gpu_dg_split import relerr, setup
 
 pytestmark = pytest.mark.gpu
 
-INVISCID = [c for c in AMR_CASES if c.get("viscosity") is None]
-
-
-@pytest.mark.parametrize("case", INVISCID, ids=[c["name"] for c in INVISCID])
+@pytest.mark.parametrize("case", AMR_CASES, ids=[c["name"] for c in AMR_CASES])
 def test_amr_rk3_fp64_matches_oracle(oracle, tmp_path, case):
     print(f"[gpu] amr rk3 fp64 {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
@@ -41,7 +36,7 @@ def test_amr_rk3_fp64_matches_oracle(oracle, tmp_path, case):
     dg.close()
 
 
-@pytest.mark.parametrize("case", INVISCID, ids=[c["name"] for c in INVISCID])
+@pytest.mark.parametrize("case", AMR_CASES, ids=[c["name"] for c in AMR_CASES])
 def test_amr_rk3_fp32_against_fp64(oracle, tmp_path, case):
     """The FP32-perturbation form with mortars against FP64 (Theta' measured
     against Theta_ref, as for the conforming decks)."""
@@ -64,19 +59,3 @@ def test_amr_rk3_fp32_against_fp64(oracle, tmp_path, case):
             assert l2 <= 1e-5, (nm, l2)
     dg.close()
     dg2.close()
-
-
-def test_amr_unsupported_paths_say_so(tmp_path):
-    """The method-of-lines stage kernels do not carry artificial viscosity
-    through mortar faces yet: they refuse a viscous problem on a
-    non-conforming mesh instead of treating its split sides as conforming.
-    (The split step carries it, test_gpu_dg_split.py; the inviscid
-    method-of-lines steps and the derivative entry points carry the mortars,
-    above and test_gpu_dg_adjoint.py.)"""
-    visc = [c for c in AMR_CASES if c.get("viscosity") is not None][0]
-    mesh, f = load_case(tmp_path / "v", visc)
-    dg = DGEuler.from_fields(mesh, euler_params(visc["dt"], diffusion=1, viscosity=visc["viscosity"]), f)
-    with pytest.raises(NsemError, match="non-conforming"):
-        dg.rk3_steps(1, precision=64)
-    dg.split_steps(1)
-    dg.close()
