@@ -60,7 +60,10 @@ typedef struct {
 int nsem_grid_generate(const char* spec_path, const char* grid_path);
 /* Load a grid file and build node geometry for polynomial orders order[3]
  * (the controls keys npx npy npz).  nonortho: 0 NONE 1 MINIMUM 2 ORTHOGONAL
- * 3 OVER_RELAXED (FV only).  As Mesh::LoadMesh(0) (field.cpp:95-167). */
+ * 3 OVER_RELAXED (FV only).  As Mesh::LoadMesh(0) (field.cpp:95-167),
+ * including grids refined by the reference's prepare tool: cells whose sides
+ * are split 2:1 get merged side outlines and mortar flags
+ * (mesh.cpp:161-446, 890-1016). */
 int nsem_mesh_load(const char* grid_path, const int32_t order[3], int32_t nonortho,
                    nsem_mesh** out);
 /* Same from a block description file, without writing the grid. */
@@ -126,7 +129,10 @@ typedef struct nsem_dg nsem_dg;
 /* Upload geometry and build the face tables.  bc_kind[4*nb] gives, per mesh
  * boundary patch, the kinds of (rho, U, T, p) (NSEM_BC_*; scalars must be
  * NEUMANN or SYMMETRY), bc_value[6*nb] the values (rho, Ux, Uy, Uz, T, p).
- * Fails with NSEM_EDEVICE when no HIP device is present. */
+ * Fails with NSEM_EDEVICE when no HIP device is present.  On a mesh with
+ * 2:1 mortar faces every step and derivative entry point applies the
+ * reference's mortar projections (scatter_non_conforming /
+ * gather_non_conforming, field.h:2018-2248). */
 int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const int32_t* bc_kind,
                    const double* bc_value, nsem_dg** out);
 void nsem_dg_destroy(nsem_dg* d);
