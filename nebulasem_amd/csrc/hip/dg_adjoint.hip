@@ -146,7 +146,9 @@ __device__ __noinline__ void ghost_face_vjp(const double QA[5], const double nvd
     }
 }
 
-template <int NXc, int NYc, int NZc>
+// DIN: the launch reads a stage increment (compile-time, so the state loads
+// carry no null-pointer branches and can be issued together).
+template <int NXc, int NYc, int NZc, bool DIN>
 __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     if (active) {
         const double cvn = d.cV[n];
         for (int v = 0; v < 5; v++) {
-            Qn[v] = a.q[v][n] + (a.din[v] ? a.din[v][n] : 0.0);
+            Qn[v] = a.q[v][n] + (DIN ? a.din[v][n] : 0.0);
             wn[v] = a.w[v][n];
             Wl[v * NP + q] = wn[v] / cvn;
             Ql[v * NP + q] = Qn[v];
@@ -241,11 +243,11 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                     Prim<double> Ap, Bp;
                     Ap.rp = QA[0]; Ap.m[0] = QA[1]; Ap.m[1] = QA[2]; Ap.m[2] = QA[3]; Ap.tp = QA[4];
                     make_prim(Ap, rrA, prA, T0, gam, gR);
-                    Bp.rp = a.q[0][m] + (a.din[0] ? a.din[0][m] : 0.0);
-                    Bp.m[0] = a.q[1][m] + (a.din[1] ? a.din[1][m] : 0.0);
-                    Bp.m[1] = a.q[2][m] + (a.din[2] ? a.din[2][m] : 0.0);
-                    Bp.m[2] = a.q[3][m] + (a.din[3] ? a.din[3][m] : 0.0);
-                    Bp.tp = a.q[4][m] + (a.din[4] ? a.din[4][m] : 0.0);
+                    Bp.rp = a.q[0][m] + (DIN ? a.din[0][m] : 0.0);
+                    Bp.m[0] = a.q[1][m] + (DIN ? a.din[1][m] : 0.0);
+                    Bp.m[1] = a.q[2][m] + (DIN ? a.din[2][m] : 0.0);
+                    Bp.m[2] = a.q[3][m] + (DIN ? a.din[3][m] : 0.0);
+                    Bp.tp = a.q[4][m] + (DIN ? a.din[4][m] : 0.0);
                     make_prim(Bp, d.rho_ref[m], d.p_ref[m], T0, gam, gR);
                     const double s = own ? -1.0 : 1.0;
                     for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - a.w[v][m] / cvm);
@@ -427,7 +429,8 @@ hipError_t launch_vjp_t(const DgDev& d, const VjpArgs& a_in, hipStream_t st) {
     const size_t lds = rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb);
     if (NP > 512 || lds > 65536) return hipErrorInvalidValue;
     a.epb = epb;
-    hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds, st, d, a);
+    if (a.din[0]) hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, true>), dim3(blocks), dim3(threads), lds, st, d, a);
+    else hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, false>), dim3(blocks), dim3(threads), lds, st, d, a);
     return hipGetLastError();
 }
 
