@@ -107,6 +107,15 @@ int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const d
  * value from bc_value[ncomp*patch], written when its magnitude exceeds 1e-7. */
 int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
                      const int32_t* bc_kind, const double* bc_value);
+/* The refinement tree amrTree_<step>.bin / .txt that the reference's prepare
+ * tool writes beside a refined grid (Mesh::Node, mesh.h:49-72, as a
+ * std::vector, util.h:129-148): nodes[4*i..] = id (the cell of a leaf),
+ * nchildren, cid (index of the first child node), level.  Read: pass
+ * nodes = NULL to get the count in *n_out, then a buffer of capacity >= that
+ * many nodes.  Replaces Prepare::refineMesh's reader / writer
+ * (field.cpp:655-667, 930-936). */
+int nsem_amr_tree_read(const char* path, uint32_t* nodes, uint64_t capacity, uint64_t* n_out);
+int nsem_amr_tree_write(const char* path, const uint32_t* nodes, uint64_t n);
 
 /* -------------------------------------------------------------- DG Euler --
  * Device-resident dGSEM compressible Euler problem on one GPU (gfx950).

@@ -147,6 +147,21 @@ def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
     return vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp)
 
 
+def read_amr_tree(path):
+    """amrTree_<step>.bin/.txt: (n, 4) uint32 nodes (id, nchildren, cid, level)."""
+    n = C.c_uint64()
+    check(lib().nsem_amr_tree_read(path.encode(), None, C.c_uint64(0), C.byref(n)))
+    out = np.empty((n.value, 4), dtype=np.uint32)
+    check(lib().nsem_amr_tree_read(path.encode(), out.ctypes.data_as(C.c_void_p), C.c_uint64(n.value),
+                                   C.byref(n)))
+    return out
+
+
+def write_amr_tree(path, nodes):
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 4)
+    check(lib().nsem_amr_tree_write(path.encode(), nodes.ctypes.data_as(C.c_void_p), C.c_uint64(nodes.shape[0])))
+
+
 def write_field(mesh, path, values, bc_kind, bc_value):
     """Write a reference field file (nsem_field_write): values (nbcs_field, ncomp)
     interior values; bc_kind (nb,), bc_value (nb, ncomp) per mesh patch."""

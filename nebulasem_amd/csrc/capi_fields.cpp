@@ -129,3 +129,22 @@ extern "C" int nsem_field_write(const nsem_mesh* m, const char* path, int32_t nc
     if (!nsem::write_field_file(path, ncomp, g.nbcs_field, values, bcs, &err)) return nsem_fail(NSEM_EIO, err);
     return NSEM_OK;
 }
+
+extern "C" int nsem_amr_tree_read(const char* path, uint32_t* nodes, uint64_t capacity, uint64_t* n_out) {
+    if (!path || !n_out) return nsem_fail(NSEM_EINVAL, "null argument");
+    std::vector<uint32_t> v;
+    std::string err;
+    if (!nsem::read_amr_tree(path, v, &err)) return nsem_fail(NSEM_EIO, err);
+    *n_out = v.size() / 4;
+    if (!nodes) return NSEM_OK;
+    if (capacity < v.size() / 4) return nsem_fail(NSEM_EINVAL, "node buffer too small");
+    std::memcpy(nodes, v.data(), v.size() * sizeof(uint32_t));
+    return NSEM_OK;
+}
+
+extern "C" int nsem_amr_tree_write(const char* path, const uint32_t* nodes, uint64_t n) {
+    if (!path || (!nodes && n)) return nsem_fail(NSEM_EINVAL, "null argument");
+    std::string err;
+    if (!nsem::write_amr_tree(path, nodes, n, &err)) return nsem_fail(NSEM_EIO, err);
+    return NSEM_OK;
+}

@@ -283,4 +283,42 @@ bool write_field_file(const std::string& path, int ncomp, uint64_t n, const doub
     return ok;
 }
 
+// ------------------------------------------------------------ amrTree ----
+// Node::operator<< (mesh.h:63-67) writes "id nchildren cid level"; the
+// vector writer (util.h:129-135) puts the count, "\n{\n", one node per line
+// and "}\n".  In the binary stream the separators reduce to the braces.
+bool read_amr_tree(const std::string& path, std::vector<uint32_t>& nodes, std::string* err) {
+    auto fail = [&](const std::string& m) { if (err) *err = path + ": " + m; return false; };
+    In in;
+    in.bin = path.size() > 4 && path.compare(path.size() - 4, 4, ".bin") == 0;
+    in.is.open(path, in.bin ? std::ios::in | std::ios::binary : std::ios::in);
+    if (!in.is) return fail("cannot open");
+    uint32_t n;
+    if (!in.u32(n) || !in.sym()) return fail("bad header");
+    nodes.assign((size_t)n * 4, 0);
+    for (size_t i = 0; i < (size_t)n * 4; i++)
+        if (!in.u32(nodes[i])) return fail("short node list");
+    if (!in.sym()) return fail("bad terminator");
+    return true;
+}
+
+bool write_amr_tree(const std::string& path, const uint32_t* nodes, uint64_t n, std::string* err) {
+    Sink o;
+    o.bin = path.size() > 4 && path.compare(path.size() - 4, 4, ".bin") == 0;
+    o.u32((uint32_t)n);
+    o.str("\n{\n");
+    for (uint64_t i = 0; i < n; i++) {
+        const uint32_t* p = nodes + 4 * i;
+        o.u32(p[0]); o.str(" "); o.u32(p[1]); o.str(" "); o.u32(p[2]); o.str(" "); o.u32(p[3]);
+        o.str("\n");
+    }
+    o.str("}\n");
+    std::ofstream f(path, std::ios::out | std::ios::binary);
+    if (!f || !f.write(o.out.data(), (std::streamsize)o.out.size())) {
+        if (err) *err = path + ": cannot write";
+        return false;
+    }
+    return true;
+}
+
 }  // namespace nsem

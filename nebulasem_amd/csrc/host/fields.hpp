@@ -35,4 +35,10 @@ bool read_field_file(const std::string& path, const FieldContext& ctx, int ncomp
 bool write_field_file(const std::string& path, int ncomp, uint64_t n, const double* values,
                       const std::vector<BoundaryCondition>& bcs, std::string* err);
 
+// The refinement tree amrTree_<step>.bin / .txt (mesh.h:49-72 written as a
+// std::vector by util.h:129-148): a count, then per node id, nchildren,
+// cid (first child's node index), level.  nodes: 4 u32 per node.
+bool read_amr_tree(const std::string& path, std::vector<uint32_t>& nodes, std::string* err);
+bool write_amr_tree(const std::string& path, const uint32_t* nodes, uint64_t n, std::string* err);
+
 }  // namespace nsem

@@ -111,6 +111,8 @@ def refine(case, d):
     # reads in preference to the deck's T0.txt (field.h:1579-1584); the
     # other fields are uniform zeros, the same either way
     shutil.copy(os.path.join(d, "T0.bin"), os.path.join(AMR, f"{case['name']}_T0.bin"))
+    # the refinement tree the prepare tool wrote beside the grid
+    shutil.copy(os.path.join(d, "amrTree_0.bin"), os.path.join(AMR, f"{case['name']}_amrTree_0.bin"))
     return out
 
 

@@ -5,14 +5,16 @@ read and written back, or regenerated from the block description, byte for
 byte."""
 import os
 
+import numpy as np
 import pytest
 
-from _cases import EULER_CASES
+from _cases import AMR_CASES, EULER_CASES
 from _euler_setup import load_case
-from nebulasem_amd import read_field, write_field
+from nebulasem_amd import Mesh, read_amr_tree, read_field, write_amr_tree, write_field
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIELDS = os.path.join(HERE, "golden", "fields")
+AMR = os.path.join(HERE, "golden", "amr")
 
 
 @pytest.mark.parametrize("name,ncomp", [("T", 1), ("U", 3)])
@@ -66,3 +68,28 @@ def test_grid_file_is_byte_identical_to_the_mesh_tool(tmp_path, ext):
     m.write_grid(out)
     ref = os.path.join(FIELDS, f"e3_n3_grid_0.{ext}")
     assert open(out, "rb").read() == open(ref, "rb").read()
+
+
+@pytest.mark.parametrize("case", AMR_CASES, ids=[c["name"] for c in AMR_CASES])
+def test_amr_tree_round_trip_and_structure(tmp_path, case):
+    """The refinement tree the reference's prepare tool wrote beside each
+    refined grid: read and written back byte for byte (binary), through the
+    text form and back to the same nodes; its leaves are the grid's cells
+    (each once), and children sit one level below their parent."""
+    src = os.path.join(AMR, f"{case['name']}_amrTree_0.bin")
+    nodes = read_amr_tree(src)
+    out = str(tmp_path / "amrTree_0.bin")
+    write_amr_tree(out, nodes)
+    assert open(out, "rb").read() == open(src, "rb").read()
+    txt = str(tmp_path / "amrTree_0.txt")
+    write_amr_tree(txt, nodes)
+    assert np.array_equal(read_amr_tree(txt), nodes)
+    mesh = Mesh.load(os.path.join(AMR, f"{case['name']}_grid_0.bin"), order=case["order"])
+    ident, nch, cid, level = nodes.T
+    leaves = ident[nch == 0]
+    assert np.array_equal(np.sort(leaves), np.arange(mesh.info.nbcs))
+    for k in np.nonzero(nch)[0]:
+        kids = np.arange(cid[k], cid[k] + nch[k])
+        assert np.all(level[kids] == level[k] + 1)
+    assert level.max() == case["refine"]["max_level"]
+
