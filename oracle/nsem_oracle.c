@@ -26,6 +26,10 @@
  *   om_euler_hydrostatic   euler.cpp:104-149 background and initial density.
  *   om_euler_diagnostics   euler.cpp:261-283, calc_courant field.cpp:440-448,
  *       reduce_* field.h:953-1006: Courant max/min/avg, mass, energy, volume.
+ *   2:1 non-conforming faces: scatter_nc / gather_nc (scatter_non_conforming,
+ *       gather_non_conforming, field.h:2018-2248) inside cds, rusanov,
+ *       div_flux and grad_flux, with the mortar matrices psiRef / psiCor
+ *       (dg.cpp:516-583) and the mortar flags FMC (mesh.cpp:433-444).
  *   om_euler_ssprk3_steps  method-of-lines SSP-RK3 (no reference counterpart),
  *       composed from the pinned operators above; "parity unpinned" as a
  *       whole: pinned only through its operators and the conservation and
@@ -33,7 +37,8 @@
  *
  * Pinning: om_euler_split_steps and om_fv_poisson are checked against the
  * outputs of the reference build (oracle/ref.mk) committed under
- * tests/golden/; see tests/test_oracle.py.
+ * tests/golden/, the split step also on grids refined 2:1 by the
+ * reference's prepare tool (tests/golden/amr/); see tests/test_oracle.py.
  */
 #include <math.h>
 #include <stdint.h>
