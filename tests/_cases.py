@@ -210,6 +210,13 @@ AMR_CASES = [
     dict(name="amr3_n5", ndiv=(2, 2, 2), size=(1000, 1000, 500), order=(5, 5, 5),
          patches=PATCHES_3D, dt=0.001, check_steps=[1], T=["gaussian 0 1.0 450 550 200 250 250 250"],
          refine=dict(direction=(1, 1, 1), field="T", field_min=0.05, field_max=0.4, max_level=2, passes=2)),
+    # the first deck with its refined cells listed in reverse order (a text
+    # grid): a refined cell then precedes its coarse neighbour and owns the
+    # mortar faces (FMC 1), the other orientation of every mortar path
+    dict(name="amr3_n2_rev", ndiv=(4, 4, 2), size=(1000, 1000, 500), order=(2, 2, 2),
+         patches=PATCHES_3D, dt=0.002, check_steps=[1, 2], reverse_cells=True,
+         T=["gaussian 0 1.0 500 500 250 200 200 200"],
+         refine=dict(direction=(1, 1, 1), field="T", field_min=0.05, field_max=0.4, max_level=2, passes=2)),
     dict(name="amr3_n3_visc", ndiv=(4, 3, 2), size=(1000, 750, 500), order=(3, 3, 3),
          patches=PATCHES_3D, dt=0.002, check_steps=[1, 2], viscosity=1.5,
          T=["gaussian 0 1.0 400 375 250 200 200 200"],

@@ -97,31 +97,83 @@ def euler_record(case, d, arrays):
     return rec
 
 
+def reverse_cells(path):
+    """Rewrite a text grid (MeshObject::writeTextMesh, mesh.h:200-216) with
+    its cells in reverse order; vertices, facets and boundaries unchanged."""
+    text = open(path).read()
+    # blocks: vertices "n\n{ ... }", facets "n\n{ ... }", cells "n\n{ ... }", boundaries
+    pos = 0
+    starts = []
+    for _ in range(3):
+        b = text.index("{", pos)
+        depth, i = 0, b
+        while True:
+            if text[i] == "{":
+                depth += 1
+            elif text[i] == "}":
+                depth -= 1
+                if depth == 0:
+                    break
+            i += 1
+        starts.append((b, i))
+        pos = i + 1
+    b, e = starts[2]
+    lines = [ln for ln in text[b + 1:e].split("\n") if ln.strip()]
+    new = text[:b + 1] + "\n" + "\n".join(reversed(lines)) + "\n" + text[e:]
+    open(path, "w").write(new)
+
+
 def refine(case, d):
     """Base grid from the mesh tool, then the prepare tool's refinement
     passes; the refined grid is kept as a fixture."""
+    if case.get("reverse_cells"):
+        # the prepare tool writes its refined grid as text
+        write_euler_case(d, dict(case, write_format="TEXT"))
     run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
     out = []
     for _ in range(case["refine"]["passes"]):
         log = run([os.path.join(REF, "prepare"), "controls", "-refine", "-start", "0"], d)
         out += [ln.strip() for ln in log.splitlines() if ln.startswith(("Level", "Refining "))]
     os.makedirs(AMR, exist_ok=True)
-    shutil.copy(os.path.join(d, "grid_0.bin"), os.path.join(AMR, f"{case['name']}_grid_0.bin"))
-    # the prepare tool's refined temperature (refineField), which euler
-    # reads in preference to the deck's T0.txt (field.h:1579-1584); the
-    # other fields are uniform zeros, the same either way
-    shutil.copy(os.path.join(d, "T0.bin"), os.path.join(AMR, f"{case['name']}_T0.bin"))
+    name = case["name"]
+    if case.get("reverse_cells"):
+        # text grid, cells reversed (the reference reads a grid's .txt
+        # before its .bin, field.cpp:107); the deck's generators for the
+        # fields, evaluated on the nodes, independent of the cell order.  The
+        # refinement tree names cells by index and is not kept.
+        reverse_cells(os.path.join(d, "grid_0.txt"))
+        shutil.copy(os.path.join(d, "grid_0.txt"), os.path.join(AMR, f"{name}_grid_0.txt"))
+        for f in os.listdir(d):
+            if f[:-4] in ("T0", "U0", "p0", "rho0", "amrTree_0"):
+                os.remove(os.path.join(d, f))
+        write_euler_case(d, case)
+        return out
+    else:
+        shutil.copy(os.path.join(d, "grid_0.bin"), os.path.join(AMR, f"{name}_grid_0.bin"))
+        # the prepare tool's refined temperature (refineField), which euler
+        # reads in preference to the deck's T0.txt (field.h:1579-1584); the
+        # other fields are uniform zeros, the same either way
+        shutil.copy(os.path.join(d, "T0.bin"), os.path.join(AMR, f"{name}_T0.bin"))
     # the refinement tree the prepare tool wrote beside the grid
-    shutil.copy(os.path.join(d, "amrTree_0.bin"), os.path.join(AMR, f"{case['name']}_amrTree_0.bin"))
+    shutil.copy(os.path.join(d, "amrTree_0.bin"), os.path.join(AMR, f"{name}_amrTree_0.bin"))
     return out
 
 
 def main():
+    # with case names as arguments, only those cases are rerun and merged
+    # into the committed fixtures
+    only = set(sys.argv[1:])
     fixtures = {"geometry": {}, "euler": {}, "potential": {}}
     arrays = {}
+    if only:
+        with open(os.path.join(HERE, "reference_fixtures.json")) as f:
+            fixtures = json.load(f)
+        with np.load(os.path.join(HERE, "reference_arrays.npz")) as z:
+            arrays = {k: z[k] for k in z.files}
+    pick = (lambda cases: [c for c in cases if c["name"] in only]) if only else (lambda cases: cases)
     tmp = tempfile.mkdtemp(prefix="nsem_golden_")
     try:
-        for case in GEOM_CASES:
+        for case in pick(GEOM_CASES):
             d = os.path.join(tmp, "g_" + case["name"])
             write_euler_case(d, case)
             fx, arr = geometry_hashes(case, d, keep_arrays=case.get("keep", ()))
@@ -129,7 +181,7 @@ def main():
             for k, v in arr.items():
                 arrays[f"geom_{case['name']}_{k}"] = v
             print("geometry", case["name"], len(fx["sha256"]), "arrays")
-        for case in EULER_CASES:
+        for case in pick(EULER_CASES):
             d = os.path.join(tmp, "e_" + case["name"])
             write_euler_case(d, case)
             run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
@@ -165,7 +217,7 @@ def main():
                 run([os.path.join(REF, "mesh"), "spec", "-o", "grid_t.txt"], dt)
                 shutil.copy(os.path.join(dt, "grid_t.txt"), os.path.join(FIELDS, f"{case['name']}_grid_0.txt"))
             print("euler", case["name"], "ok")
-        for case in AMR_CASES:
+        for case in pick(AMR_CASES):
             d = os.path.join(tmp, "a_" + case["name"])
             write_euler_case(d, case)
             passes = refine(case, d)
@@ -175,7 +227,7 @@ def main():
             rec = euler_record(case, d, arrays)
             fixtures["euler"][case["name"]] = rec
             print("amr", case["name"], passes, rec["log_tail"][-1])
-        for case in POTENTIAL_CASES:
+        for case in pick(POTENTIAL_CASES):
             d = os.path.join(tmp, "p_" + case["name"])
             write_potential_case(d, case)
             run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from _cases import AMR_CASES, GEOM_CASES, write_euler_case
+from _euler_setup import amr_grid
 from _golden import arrays, fixtures, sha
 from nebulasem_amd import Mesh
 
@@ -20,7 +21,7 @@ def test_amr_geometry_bit_exact(case):
     prepare tool, grid file committed): LoadMesh's fix-ups with split cell
     sides (merged outlines, mortar flags FMC), the DG node geometry and face
     maps, and the 2:1 mortar matrices psiRef/psiCor agree bit for bit."""
-    mesh = Mesh.load(os.path.join(AMR_DIR, f"{case['name']}_grid_0.bin"), order=case["order"])
+    mesh = Mesh.load(amr_grid(case), order=case["order"])
     fx = FX[case["name"]]
     assert mesh.boundaries == fx["bnames"]
     bad = []
@@ -30,9 +31,11 @@ def test_amr_geometry_bit_exact(case):
             bad.append(name)
     assert not bad, f"arrays differing from the reference: {bad}"
     # mortar faces are present; the prepare tool numbers a coarse cell
-    # before its refined neighbours, so the coarse cell owns them (FMC 2)
+    # before its refined neighbours, so the coarse cell owns them (FMC 2);
+    # with the cells listed in reverse the refined cell owns them (FMC 1)
     fmc = mesh.array("FMC")
-    assert (fmc == 2).sum() > 0 and set(np.unique(fmc)) <= {0, 2}
+    m = 1 if case.get("reverse_cells") else 2
+    assert (fmc == m).sum() > 0 and set(np.unique(fmc)) <= {0, m}
 
 
 @pytest.mark.parametrize("case", GEOM_CASES, ids=[c["name"] for c in GEOM_CASES])
