@@ -30,10 +30,17 @@
  *       gather_non_conforming, field.h:2018-2248) inside cds, rusanov,
  *       div_flux and grad_flux, with the mortar matrices psiRef / psiCor
  *       (dg.cpp:516-583) and the mortar flags FMC (mesh.cpp:433-444).
- *   om_euler_ssprk3_steps  method-of-lines SSP-RK3 (no reference counterpart),
- *       composed from the pinned operators above; "parity unpinned" as a
- *       whole: pinned only through its operators and the conservation and
- *       consistency properties tested in tests/test_oracle_mol.py.
+ *   om_euler_mol_rhs       the method-of-lines residual L(q) at one state: the
+ *       reference's divf(fq, false, &F, &q, &lambdaMax) of the euler.cpp
+ *       fluxes (181-251) all formed from that state.  Pinned bit for bit
+ *       against the reference's own operators evaluated at mid-RK states of
+ *       eight decks (oracle/ref_tendency.cpp; tests/test_oracle_mol_pin.py),
+ *       and against the split step by first-order consistency.
+ *   om_euler_ssprk3_steps  SSP-RK3 (Shu & Osher 1988) of that residual (no
+ *       reference counterpart: the reference's RK schemes reduce to forward
+ *       Euler on this path); pinned through the residual above and the
+ *       third-order convergence, conservation and rest-state properties of
+ *       tests/test_oracle_mol.py.
  *
  * Pinning: om_euler_split_steps and om_fv_poisson are checked against the
  * outputs of the reference build (oracle/ref.mk) committed under
@@ -81,6 +88,13 @@ typedef struct {
      * heat diffusivity mu / Pr (Fluid::viscosity, Fluid::Pr) */
     int32_t diffusion;
     double viscosity, Pr;
+    /* method-of-lines expressions: 0 = the reference's, p' = P0 (rho theta
+     * R/P0)^gamma - p_ref (euler.cpp:211-213) and the strong gradient of
+     * theta = theta' + T0 (euler.cpp:181, 250); 1 = the same quantities free
+     * of cancellation: p' = p_ref expm1(gamma log1p(Theta'/(rho_ref T0)))
+     * (Theta' = rho theta - rho_ref T0) and the gradient of theta' (the
+     * constant T0 has none) */
+    int32_t mol_exact;
 } om_euler;
 
 static double dot3(const double* p, const double* q) { /* right-nested, tensor.h:117 */
@@ -93,8 +107,16 @@ static int equal0(double p) { return fabs(p) <= 1e-7; }
 /* Built with -DOM_PERTURB_POW the equation of state's pow() is moved by
  * -1, 0 or +1 ulp pseudo-randomly per call: the tests use that build to
  * measure how far a 1-ulp libm difference moves the reference algorithm's
- * outputs (the device's pow is not glibc's). */
-#ifdef OM_PERTURB_POW
+ * outputs.  Built with -DOM_CR_POW the equation of state's and the
+ * diagnostics' pow() are the correctly rounded power (x^y in quadruple
+ * precision, libquadmath's powq, rounded once to double): the reference's
+ * algorithm with a correctly rounded pow, where glibc's pow (below 0.52 ulp)
+ * rounds the wrong way for about one argument in a thousand. */
+#if defined(OM_CR_POW)
+#include <quadmath.h>
+static double diag_pow(double a, double b) { return (double)powq((__float128)a, (__float128)b); }
+static double eos_pow(double a, double b, unsigned long long i) { (void)i; return diag_pow(a, b); }
+#elif defined(OM_PERTURB_POW)
 static unsigned long long g_seed = 1;
 static double eos_pow(double a, double b, unsigned long long i) {
     double r = pow(a, b);
@@ -104,8 +126,10 @@ static double eos_pow(double a, double b, unsigned long long i) {
     else if (k < 0) r = nextafter(r, -INFINITY);
     return r;
 }
+static double diag_pow(double a, double b) { return pow(a, b); }
 #else
 static double eos_pow(double a, double b, unsigned long long i) { (void)i; return pow(a, b); }
+static double diag_pow(double a, double b) { return pow(a, b); }
 #endif
 
 /* ---------------------------------------------------- boundary passes -- */
@@ -571,14 +595,14 @@ void om_euler_diagnostics(const om_mesh* m, const om_euler* e, const double* p_r
     for (uint64_t i = 0; i < m->nbf; i++) {
         const double* u = &U[3 * i];
         const double um = sqrt(dot3(u, u));
-        const double c = um * e->dt / pow(m->cV[i], 1.0 / 3);
+        const double c = um * e->dt / diag_pow(m->cV[i], 1.0 / 3);
         if (fabs(c) > cmax) cmax = fabs(c);
         if (fabs(c) < cmin) cmin = fabs(c);
         csum += c;
         mass += rho[i] * m->cV[i];
         const double gh = e->buoyancy ? dot3(e->g, &m->cC[3 * i]) : 0.0;
         const double th = T[i] + e->T0;
-        double sf = gh + 0.5 * dot3(u, u) + pow((p[i] + p_ref[i]) / e->P0, R / e->cp) * th * e->cv;
+        double sf = gh + 0.5 * dot3(u, u) + diag_pow((p[i] + p_ref[i]) / e->P0, R / e->cp) * th * e->cv;
         sf = rho[i] * m->cV[i] * sf;
         energy += sf;
         vol += m->cV[i];
@@ -832,7 +856,8 @@ void om_fv_correct_velocity(const om_mesh* m, const int32_t* bc_p, const double*
  * rho viscosity) at the stage state, from the split step's pinned strong
  * gradient gradf_strong and its ghost rule. */
 static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, const double* rho_ref,
-                    double* rho, double* U, double* T, double* L /* naf*5 */) {
+                    double* rho, double* U, double* T, const double* Tp, double* L /* naf*5 */,
+                    double* rout /* naf*5 or NULL */) {
     const uint64_t N = m->naf, nfn = (uint64_t)m->nfacets * m->npf;
     const double R = e->cp - e->cv, gam = e->cp / e->cv;
     bc_scalar(m, e->bc_rho, e->val_rho, rho);
@@ -851,19 +876,30 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
     double* GT = e->diffusion ? (double*)malloc(sizeof(double) * N * 3) : NULL;
     if (e->diffusion) {
         gradf_strong(m, 3, U, e->bc_U, e->val_U, GU);
-        gradf_strong(m, 1, T, e->bc_T, e->val_T, GT);
+        if (e->mol_exact) {
+            gradf_strong(m, 1, T, e->bc_T, e->val_T, GT);
+        } else {
+            for (uint64_t i = 0; i < N; i++) p[i] = T[i] + e->T0;
+            gradf_strong(m, 1, p, e->bc_T, e->val_T, GT);
+        }
     }
     for (uint64_t i = 0; i < N; i++) w[i] = sqrt(dot3(&U[3 * i], &U[3 * i]));
     cds_scalar(m, w, lam);
     for (uint64_t i = 0; i < N; i++) w[i] = sqrt(gam * R * (T[i] + e->T0));
     cds_scalar(m, w, t1);
     for (uint64_t k = 0; k < nfn; k++) lam[k] = (lam[k] + t1[k]) / 2;
-    for (uint64_t i = 0; i < N; i++) p[i] = e->P0 * pow((rho[i] * (T[i] + e->T0) * R) / e->P0, gam) - p_ref[i];
+    if (e->mol_exact && Tp) {
+        for (uint64_t i = 0; i < m->nbf; i++) p[i] = p_ref[i] * expm1(gam * log1p(Tp[i] / (rho_ref[i] * e->T0)));
+        bc_scalar(m, e->bc_p, e->val_p, p);
+    } else {
+        for (uint64_t i = 0; i < N; i++) p[i] = e->P0 * pow((rho[i] * (T[i] + e->T0) * R) / e->P0, gam) - p_ref[i];
+    }
     /* rho */
     for (uint64_t i = 0; i < N; i++)
         for (int c = 0; c < 3; c++) fq[3 * i + c] = U[3 * i + c] * rho[i];
     divf_dg(m, 1, fq, rho, lam, r);
     for (uint64_t i = 0; i < N; i++) L[5 * i] = -r[i] / m->cV[i];
+    if (rout) for (uint64_t i = 0; i < N; i++) rout[5 * i] = r[i];
     /* momentum */
     for (uint64_t i = 0; i < N; i++) {
         for (int a = 0; a < 3; a++)
@@ -877,6 +913,7 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
     for (uint64_t i = 0; i < N; i++)
         for (int c = 0; c < 3; c++)
             L[5 * i + 1 + c] = -r[3 * i + c] / m->cV[i] + (e->buoyancy ? (rho[i] - rho_ref[i]) * e->g[c] : 0.0);
+    if (rout) for (uint64_t i = 0; i < N; i++) for (int c = 0; c < 3; c++) rout[5 * i + 1 + c] = r[3 * i + c];
     /* rho theta */
     for (uint64_t i = 0; i < N; i++) {
         for (int c = 0; c < 3; c++) fq[3 * i + c] = rho[i] * U[3 * i + c] * (T[i] + e->T0);
@@ -887,44 +924,76 @@ static void mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, co
     }
     divf_dg(m, 1, fq, q, lam, r);
     for (uint64_t i = 0; i < N; i++) L[5 * i + 4] = -r[i] / m->cV[i];
+    if (rout) for (uint64_t i = 0; i < N; i++) rout[5 * i + 4] = r[i];
     free(lam); free(t1); free(w); free(fq); free(q); free(r); free(p); free(GU); free(GT);
 }
 
-/* SSP-RK3 steps of the method-of-lines form; state as the split steps use
- * it (rho, U, theta', p' on every node); p is recomputed at the end. */
+/* SSP-RK3 steps of the method-of-lines form; state in and out as the split
+ * steps use it (rho, U, theta', p' on every node).  The steps carry the
+ * conserved perturbations over the hydrostatic background, rho' = rho -
+ * rho_ref, m = rho U, Theta' = rho theta - rho_ref T0 (the device's state
+ * variables), converted from and to the primitive state as the device's
+ * set_state / get_state do; p' at the end by the form mol_exact selects. */
 int om_euler_ssprk3_steps(const om_mesh* m, const om_euler* e, const double* p_ref,
                           const double* rho_ref, double* rho, double* U, double* T, double* p,
                           int nsteps) {
-    const uint64_t N = m->naf;
-    const double R = e->cp - e->cv, gam = e->cp / e->cv;
+    const uint64_t N = m->naf, nb = m->nbf;
+    const double R = e->cp - e->cv, gam = e->cp / e->cv, T0 = e->T0;
     double* q0 = (double*)malloc(sizeof(double) * N * 5);
     double* qs = (double*)malloc(sizeof(double) * N * 5);
     double* L = (double*)malloc(sizeof(double) * N * 5);
+    double* Tp = (double*)malloc(sizeof(double) * N);
     static const double A[3] = {0.0, 0.75, 1.0 / 3.0}, B[3] = {1.0, 0.25, 2.0 / 3.0};
     int status = 0;
+    for (uint64_t i = 0; i < nb; i++) {
+        qs[5 * i] = rho[i] - rho_ref[i];
+        for (int c = 0; c < 3; c++) qs[5 * i + 1 + c] = rho[i] * U[3 * i + c];
+        qs[5 * i + 4] = rho[i] * (T[i] + T0) - rho_ref[i] * T0;
+    }
     for (int st = 0; st < nsteps; st++) {
-        for (uint64_t i = 0; i < N; i++) {
-            q0[5 * i] = rho[i];
-            for (int c = 0; c < 3; c++) q0[5 * i + 1 + c] = rho[i] * U[3 * i + c];
-            q0[5 * i + 4] = rho[i] * (T[i] + e->T0);
-        }
-        memcpy(qs, q0, sizeof(double) * N * 5);
+        memcpy(q0, qs, sizeof(double) * nb * 5);
         for (int s = 0; s < 3; s++) {
-            mol_rhs(m, e, p_ref, rho_ref, rho, U, T, L);
-            for (uint64_t i = 0; i < m->nbf; i++)
+            for (uint64_t i = 0; i < nb; i++) {
+                rho[i] = rho_ref[i] + qs[5 * i];
+                for (int c = 0; c < 3; c++) U[3 * i + c] = qs[5 * i + 1 + c] / rho[i];
+                T[i] = (qs[5 * i + 4] + rho_ref[i] * T0) / rho[i] - T0;
+                Tp[i] = qs[5 * i + 4];
+            }
+            mol_rhs(m, e, p_ref, rho_ref, rho, U, T, Tp, L, NULL);
+            for (uint64_t i = 0; i < nb; i++)
                 for (int v = 0; v < 5; v++)
                     qs[5 * i + v] = A[s] * q0[5 * i + v] + B[s] * (qs[5 * i + v] + e->dt * L[5 * i + v]);
-            for (uint64_t i = 0; i < m->nbf; i++) {
-                rho[i] = qs[5 * i];
-                for (int c = 0; c < 3; c++) U[3 * i + c] = qs[5 * i + 1 + c] / rho[i];
-                T[i] = qs[5 * i + 4] / rho[i] - e->T0;
-            }
         }
-        for (uint64_t i = 0; i < m->nbf; i++)
-            if (!isfinite(rho[i]) || !isfinite(T[i])) status = 5;
+        for (uint64_t i = 0; i < nb * 5; i++)
+            if (!isfinite(qs[i])) status = 5;
         if (status) break;
     }
-    for (uint64_t i = 0; i < N; i++) p[i] = e->P0 * pow((rho[i] * (T[i] + e->T0) * R) / e->P0, gam) - p_ref[i];
-    free(q0); free(qs); free(L);
+    for (uint64_t i = 0; i < nb; i++) {
+        rho[i] = rho_ref[i] + qs[5 * i];
+        for (int c = 0; c < 3; c++) U[3 * i + c] = qs[5 * i + 1 + c] / rho[i];
+        T[i] = (qs[5 * i + 4] + rho_ref[i] * T0) / rho[i] - T0;
+    }
+    bc_scalar(m, e->bc_rho, e->val_rho, rho);
+    bc_vector(m, e->bc_U, e->val_U, U);
+    bc_scalar(m, e->bc_T, e->val_T, T);
+    if (e->mol_exact) {
+        for (uint64_t i = 0; i < nb; i++)
+            p[i] = p_ref[i] * expm1(gam * log1p(qs[5 * i + 4] / (rho_ref[i] * T0)));
+        bc_scalar(m, e->bc_p, e->val_p, p);
+    } else {
+        for (uint64_t i = 0; i < N; i++) p[i] = e->P0 * pow((rho[i] * (T[i] + T0) * R) / e->P0, gam) - p_ref[i];
+    }
+    free(q0); free(qs); free(L); free(Tp);
     return status;
+}
+
+/* The method-of-lines residual at one state (rho, U, theta' on every node;
+ * ghosts are set from the boundary conditions; Tp = Theta' on the interior
+ * nodes, read only with mol_exact): L[5 per node] = -r / cV of the density,
+ * momentum (+ (rho - rho_ref) g) and rho theta equations, and, when r is
+ * not NULL, the Rusanov divergences r themselves (5 per node), which
+ * divf(fq, false, &F, &q, &lambdaMax) returns in the reference. */
+void om_euler_mol_rhs(const om_mesh* m, const om_euler* e, const double* p_ref, const double* rho_ref,
+                      double* rho, double* U, double* T, const double* Tp, double* L, double* r) {
+    mol_rhs(m, e, p_ref, rho_ref, rho, U, T, Tp, L, r);
 }

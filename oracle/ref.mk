@@ -42,7 +42,7 @@ APPSRC_potential := apps/potential/potential.cpp
 APPSRC_piso      := apps/piso/piso.cpp
 APPSRC_prepare   := apps/prepare/prepareApp.cpp
 
-all: $(addprefix $(OUT)/,$(APPS)) $(OUT)/geomdump
+all: $(addprefix $(OUT)/,$(APPS)) $(OUT)/geomdump $(OUT)/euler_tendency
 
 $(OBJ)/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)
@@ -65,6 +65,16 @@ $(OBJ)/geomdump.o: oracle/ref_geomdump.cpp
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
 $(OUT)/geomdump: $(OBJ)/geomdump.o $(OUT)/libnebulasem_ref.a
+	$(CXX) -o $@ $^ $(LDFLAGS)
+
+# tendency dump driver: this repository's own code, linked against the
+# reference library, evaluates the reference's divergence operators of the
+# Euler fluxes at a given state (the method-of-lines residual's pin).
+$(OBJ)/euler_tendency.o: oracle/ref_tendency.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/euler_tendency: $(OBJ)/euler_tendency.o $(OUT)/libnebulasem_ref.a
 	$(CXX) -o $@ $^ $(LDFLAGS)
 
 # The reference's OpenMP build of the Euler and Poisson apps (its USE_OMP

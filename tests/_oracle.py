@@ -15,6 +15,7 @@ OM_NEUMANN, OM_SYMMETRY, OM_DIRICHLET = 0, 1, 2
 
 
 ORACLE_ULP_SO = os.path.join(ROOT, "oracle", "liboracle_ulp.so")
+ORACLE_CR_SO = os.path.join(ROOT, "oracle", "liboracle_cr.so")
 
 
 def build_oracle():
@@ -23,6 +24,8 @@ def build_oracle():
                            ORACLE_SO, src, "-lm"])
     subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-DOM_PERTURB_POW",
                            "-o", ORACLE_ULP_SO, src, "-lm"])
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-DOM_CR_POW",
+                           "-o", ORACLE_CR_SO, src, "-lquadmath", "-lm"])
     return ORACLE_SO
 
 
@@ -44,6 +47,19 @@ def lib():
             build_oracle()
         _lib = C.CDLL(ORACLE_SO)
     return _lib
+
+
+_lib_cr = None
+
+
+def lib_cr():
+    """The oracle built with a correctly rounded pow (see nsem_oracle.c)."""
+    global _lib_cr
+    if _lib_cr is None:
+        if _stale(ORACLE_CR_SO):
+            build_oracle()
+        _lib_cr = C.CDLL(ORACLE_CR_SO)
+    return _lib_cr
 
 
 def lib_ulp():
@@ -79,7 +95,8 @@ class OmEuler(C.Structure):
                 ("cv", C.c_double), ("g", C.c_double * 3), ("buoyancy", C.c_int32),
                 ("bc_rho", PI), ("bc_U", PI), ("bc_T", PI), ("bc_p", PI),
                 ("val_rho", PD), ("val_U", PD), ("val_T", PD), ("val_p", PD),
-                ("diffusion", C.c_int32), ("viscosity", C.c_double), ("Pr", C.c_double)]
+                ("diffusion", C.c_int32), ("viscosity", C.c_double), ("Pr", C.c_double),
+                ("mol_exact", C.c_int32)]
 
 
 def _p(a, t):
@@ -141,18 +158,22 @@ def euler_hydrostatic(om, params, T, p):
     return p_ref, rho_ref, rho
 
 
-def euler_diagnostics(om, params, p_ref, rho, U, T, p):
+def euler_diagnostics(om, params, p_ref, rho, U, T, p, cr_pow=False):
     """(Courant max, min, avg, mass, energy, volume) as euler.cpp:261-283."""
     out = np.empty(6)
-    lib().om_euler_diagnostics(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho, PD), _p(U, PD),
+    (lib_cr() if cr_pow else lib()).om_euler_diagnostics(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho, PD), _p(U, PD),
                                _p(T, PD), _p(p, PD), _p(out, PD))
     return out
 
 
 def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -9.80606),
-                buoyancy=1, viscosity=None, Pr=0.9):
+                buoyancy=1, viscosity=None, Pr=0.9, mol_exact=0):
     """bcs: dict field -> (kinds int32 array, values float64 array);
-    viscosity: None for `diffusion NO`, else the artificial viscosity."""
+    viscosity: None for `diffusion NO`, else the artificial viscosity;
+    mol_exact: the method-of-lines residual with the reference's expressions
+    (0: p' = P0 (rho theta R/P0)^gamma - p_ref, gradient of theta' + T0) or
+    their cancellation-free forms (1: p' = p_ref expm1(gamma log1p(Theta'/
+    Theta_ref)), gradient of theta'), the device's forms."""
     e = OmEuler()
     e.dt, e.P0, e.T0, e.cp, e.cv = dt, P0, T0, cp, cv
     e.g[:] = list(g)
@@ -160,6 +181,7 @@ def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -
     e.diffusion = 0 if viscosity is None else 1
     e.viscosity = 0.0 if viscosity is None else viscosity
     e.Pr = Pr
+    e.mol_exact = int(mol_exact)
     keep = []
     for f in ("rho", "U", "T", "p"):
         kinds, vals = bcs[f]
@@ -172,8 +194,10 @@ def make_params(dt, bcs, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, g=(0, 0, -
     return e
 
 
-def euler_split_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps, perturb_pow=False):
-    L = lib_ulp() if perturb_pow else lib()
+def euler_split_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps, perturb_pow=False, cr_pow=False):
+    """perturb_pow: the build whose equation-of-state pow is moved by +-1 ulp;
+    cr_pow: the build with a correctly rounded pow."""
+    L = lib_ulp() if perturb_pow else (lib_cr() if cr_pow else lib())
     st = L.om_euler_split_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
                                     _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))
     return st
@@ -200,3 +224,16 @@ def fv_correct_velocity(om, bc_kind, bc_val, p, U):
 def euler_ssprk3_steps(om, params, p_ref, rho_ref, rho, U, T, p, nsteps):
     return lib().om_euler_ssprk3_steps(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD),
                                        _p(rho, PD), _p(U, PD), _p(T, PD), _p(p, PD), int(nsteps))
+
+
+def euler_mol_rhs(om, params, p_ref, rho_ref, rho, U, T, Tp=None):
+    """The method-of-lines residual at the state rho, U, theta' (every node;
+    ghosts set in place from the boundary conditions): L (naf, 5) and the
+    Rusanov divergences r (naf, 5) of the density, momentum and rho theta
+    fluxes.  Tp: Theta' on the interior nodes (for mol_exact)."""
+    n = om.info.nall_field
+    L, r = np.empty(5 * n), np.empty(5 * n)
+    tp = None if Tp is None else _p(np.ascontiguousarray(Tp, dtype=np.float64), PD)
+    lib().om_euler_mol_rhs(C.byref(om.s), C.byref(params), _p(p_ref, PD), _p(rho_ref, PD), _p(rho, PD),
+                           _p(U, PD), _p(T, PD), tp, _p(L, PD), _p(r, PD))
+    return L.reshape(n, 5), r.reshape(n, 5)

@@ -15,6 +15,6 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def oracle():
     import _oracle
-    if not (os.path.exists(_oracle.ORACLE_SO) and os.path.exists(_oracle.ORACLE_ULP_SO)):
+    if not all(os.path.exists(f) for f in (_oracle.ORACLE_SO, _oracle.ORACLE_ULP_SO, _oracle.ORACLE_CR_SO)):
         _oracle.build_oracle()
     return _oracle
