@@ -163,19 +163,16 @@ def amr_bench(steps=20, dt=0.002):
                        "a 1 K bubble, N=5, SSP-RK3 dt=%g" % dt,
            "elements": nb // 216, "nodes": nb, "dof": NQ * nb, "mortar_faces": int((fmc > 0).sum()),
            "steps": steps, "stage_kernel": "rk_stage_kernel_fx<R,S,6,6,6,W,DIN,AFF,false,true> + mortar pre-pass"}
-    q = {}
     for prec in (32, 64):
         dg.set_state(rho, U, T, p)
         dg.rk3_steps(2, prec)
         dg.synchronize()
         dg.set_state(rho, U, T, p)
         sec = dg.time_rk3(steps, prec)
-        q[prec] = dg.get_conserved()
         out[f"fp{prec}"] = {"value": NQ * nb * 3 * steps / sec, "unit": "DOF*stage/s",
                             "ms_per_step": sec / steps * 1e3}
     out["fp32_over_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
-    out["fp32_vs_fp64_rel_l2"] = {nm: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-                                  for nm, a, b in zip(("rho_p", "m", "Theta_p"), q[32], q[64])}
+    out["fp32_vs_fp64_error"] = fp32_error(dg, (rho, U, T, p), steps)
     # discrete adjoint over the same steps (FP64, checkpoints every 10):
     # J = 1/2 |Theta'(T)|^2, its gradient with respect to the initial
     # conserved state; cost relative to the forward run, a central
@@ -197,19 +194,14 @@ def amr_bench(steps=20, dt=0.002):
         return np.stack([a_, b_[:, 0], b_[:, 1], b_[:, 2], c_])
 
     set_q(q0)
-    dg.synchronize()
-    t0 = time.time()
-    dg.rk3_steps(steps, 64)
-    dg.synchronize()
-    t_fwd = time.time() - t0
+    t_fwd = dg.time_rk3(steps, 64)
     qn = run(q0)
     lam = np.zeros_like(qn)
     lam[4] = qn[4]
     set_q(q0)
-    dg.synchronize()
-    t0 = time.time()
-    gq = dg.rk3_adjoint(steps, lam, 10)
-    t_adj = time.time() - t0
+    dg.rk3_adjoint(steps, lam, 10)  # warm-up: workspace allocation
+    set_q(q0)
+    gq, t_adj = dg.time_rk3_adjoint(steps, lam, 10)
     rng = np.random.default_rng(20261204)
     scale = np.array([max(np.abs(q0[k]).max(), 1e-3) for k in range(5)])[:, None]
     v = rng.standard_normal(q0.shape) * scale
@@ -228,6 +220,7 @@ def amr_bench(steps=20, dt=0.002):
     lhs, rhs = float(np.vdot(Mv, w)), float(np.vdot(v, MTw))
     out["adjoint"] = {"objective": "1/2 |Theta'(T)|^2 after the FP64 steps, gradient w.r.t. the initial state",
                       "checkpoint": 10, "forward_s": t_fwd, "adjoint_s": t_adj,
+                      "timing": "HIP events on the library's stream, host transfers excluded",
                       "adjoint_over_forward": t_adj / t_fwd, "grad_fd": fd, "grad_adjoint": ad,
                       "grad_rel_err": abs(fd - ad) / abs(ad), "dot_tlm": lhs, "dot_adjoint": rhs,
                       "dot_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs))}
@@ -302,20 +295,17 @@ def adjoint_bench(nsteps=50, checkpoint=10, dt=0.001):
 
     q0 = q_of(th0)
     set_q(q0)
-    dg.synchronize()
-    t = time.time()
+    t_fwd = dg.time_rk3(nsteps, 64)
     qn = run(q0)
-    t_fwd = time.time() - t
     r = theta(qn) - obs
     rho = rr + qn[0]
     lam = np.zeros_like(qn)
     lam[4] = r / rho
     lam[0] = -r * (qn[4] + rr * T0) / rho ** 2
     set_q(q0)
-    dg.synchronize()
-    t = time.time()
-    gq = dg.rk3_adjoint(nsteps, lam, checkpoint)
-    t_adj = time.time() - t
+    dg.rk3_adjoint(nsteps, lam, checkpoint)  # warm-up: workspace allocation
+    set_q(q0)
+    gq, t_adj = dg.time_rk3_adjoint(nsteps, lam, checkpoint)
     dr, dtp = dq_dth(th0)
     grad = gq[0] * dr + gq[4] * dtp
     v = rng.standard_normal(nb)
@@ -335,6 +325,7 @@ def adjoint_bench(nsteps=50, checkpoint=10, dt=0.001):
     out = {"workload": f"2D bubble 64x1x64 N=5 ({nb} nodes), {nsteps} FP64 SSP-RK3 steps, dt={dt}, "
                        f"checkpoint every {checkpoint}",
            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+           "timing": "HIP events on the library's stream, host transfers excluded",
            "J": 0.5 * float(np.sum(r ** 2)), "grad_fd": fd, "grad_adjoint": ad,
            "grad_rel_err": abs(fd - ad) / abs(ad),
            "dot_tlm": lhs, "dot_adjoint": rhs, "dot_rel_err": abs(lhs - rhs) / max(abs(lhs), abs(rhs))}
@@ -361,33 +352,88 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def cpu_reference(steps=6, threads=16):
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def fp32_error(dg, state, nsteps=100):
+    """The FP32-perturbation form against FP64 after nsteps SSP-RK3 steps
+    from the same initial state (the FP64 device path being the one checked
+    against the oracle's FP64 composition to 1e-12).  Per field, relative L2
+    and max-norm, each relative to the field itself: the reference's fields
+    rho' = rho - rho_ref, U, theta', p' and the conserved state rho', m,
+    Theta' = rho theta - rho_ref T0; Theta' and p' also relative to the
+    background Theta_ref = rho_ref T0 and p_ref they perturb.  Theta' and p'
+    are small differences (at constant pressure rho' T0 and rho theta'
+    cancel), so their error relative to themselves is the largest."""
+    rho, U, T, p = state
+    out = {}
+    for prec in (64, 32):
+        dg.set_state(rho, U, T, p)
+        dg.rk3_steps(nsteps, prec)
+        out[prec] = (dg.get_conserved(), dg.get_state())
+    p_ref, rho_ref = dg.background()
+    (q64, s64), (q32, s32) = out[64], out[32]
+    fields = {"rho_p": (s32[0] - rho_ref, s64[0] - rho_ref), "U": (s32[1], s64[1]),
+              "theta_p": (s32[2], s64[2]), "p_p": (s32[3], s64[3]),
+              "m": (q32[1], q64[1]), "Theta_p": (q32[2], q64[2])}
+    err = {k: {"rel_l2": rel_l2(a, b), "rel_max": rel_max(a, b)} for k, (a, b) in fields.items()}
+    err["Theta_p"]["rel_l2_of_Theta_ref"] = float(np.linalg.norm(q32[2] - q64[2]) / np.linalg.norm(rho_ref * 300.0))
+    err["p_p"]["rel_l2_of_p_ref"] = float(np.linalg.norm(s32[3] - s64[3]) / np.linalg.norm(p_ref))
+    err["steps"] = nsteps
+    err["prognostic_max_rel_l2"] = max(err[k]["rel_l2"] for k in ("rho_p", "U", "theta_p"))
+    return err
+
+
+def host_cpu():
+    """The host's CPU model and the threads this process may use (the
+    scheduler affinity, or OMP_NUM_THREADS where the environment caps it)."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return model, max(n, 1)
+
+
+def _ref_deck(d, ndiv, steps):
+    from _cases import PATCHES_3D, _spec, euler_controls, field_file as ref_field_file
+    lx, ly, lz = 10000.0 * ndiv[0] / 64, 10000.0 * ndiv[1] / 64, 5000.0 * ndiv[2] / 32
+    case = dict(ndiv=ndiv, size=(lx, ly, lz), order=(7, 7, 7), patches=PATCHES_3D,
+                dt=0.005, check_steps=[steps], T=[f"gaussian 0 0.5 {lx / 2} {ly / 2} {lz / 2} 400 400 400"])
+    names = sorted(n for n, _ in PATCHES_3D)
+    with open(os.path.join(d, "spec"), "w") as f:
+        f.write(_spec(case))
+    with open(os.path.join(d, "controls"), "w") as f:
+        f.write(euler_controls(case).replace("write_interval               1\n",
+                                             "write_interval               100000\n"))
+    for fn, nc, gens, kind in (("T0.txt", 1, case["T"], "NEUMANN"), ("U0.txt", 3, ["uniform 0 0 0"], "SYMMETRY"),
+                               ("p0.txt", 1, ["uniform 0"], "NEUMANN"), ("rho0.txt", 1, ["uniform 0"], "NEUMANN")):
+        with open(os.path.join(d, fn), "w") as f:
+            f.write(ref_field_file(nc, gens, [(n, kind) for n in names]))
+
+
+def cpu_reference(threads, ndiv, steps):
     """The reference's own CPU path (apps/euler, its OpenMP build compiled
     from the reference sources into oracle/_ref by oracle/ref.mk) on a
-    bounded sample of the workload: 16x16x8 elements at N = 7 (1,048,576
-    nodes, the workload's element shape and order), split forward-Euler
-    steps, `threads` OpenMP threads.  Step time = spacing of the per-step
-    log lines (setup excluded), median over the steps."""
+    bounded sample of the workload: ndiv elements at N = 7 (the workload's
+    element shape and order), split forward-Euler steps (1 stage each),
+    `threads` OpenMP threads.  Step time = spacing of the per-step log
+    lines (set-up excluded), median over the steps."""
     import subprocess
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from _cases import PATCHES_3D, _spec, euler_controls, field_file as ref_field_file
     exe = os.path.join(ROOT, "oracle", "_ref", "euler_omp")
     mesher = os.path.join(ROOT, "oracle", "_ref", "mesh")
     if not (os.path.exists(exe) and os.path.exists(mesher)):
         return None
-    case = dict(ndiv=(16, 16, 8), size=(2500.0, 2500.0, 1250.0), order=(7, 7, 7), patches=PATCHES_3D,
-                dt=0.005, check_steps=[steps], T=["gaussian 0 0.5 1250 1250 600 400 400 400"])
-    names = sorted(n for n, _ in PATCHES_3D)
     with tempfile.TemporaryDirectory() as d:
-        with open(os.path.join(d, "spec"), "w") as f:
-            f.write(_spec(case))
-        with open(os.path.join(d, "controls"), "w") as f:
-            f.write(euler_controls(case).replace("write_interval               1\n",
-                                                 "write_interval               100000\n"))
-        for fn, nc, gens, kind in (("T0.txt", 1, case["T"], "NEUMANN"), ("U0.txt", 3, ["uniform 0 0 0"], "SYMMETRY"),
-                                   ("p0.txt", 1, ["uniform 0"], "NEUMANN"), ("rho0.txt", 1, ["uniform 0"], "NEUMANN")):
-            with open(os.path.join(d, fn), "w") as f:
-                f.write(ref_field_file(nc, gens, [(n, kind) for n in names]))
+        _ref_deck(d, ndiv, steps)
         subprocess.run([mesher, "spec", "-o", "grid_0.bin"], cwd=d, check=True, capture_output=True)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         t = time.time()
@@ -401,11 +447,48 @@ def cpu_reference(steps=6, threads=16):
     if len(dts) < 1:
         return None
     step = float(np.median(dts))
-    nodes = 16 * 16 * 8 * 512
+    nodes = ndiv[0] * ndiv[1] * ndiv[2] * 512
     return {"value": NQ * nodes / step, "unit": "DOF*stage/s", "cores": threads, "kind": "reference",
             "sample": f"reference apps/euler (OpenMP build, {threads} threads), split forward-Euler step "
-                      f"(1 stage/step), 16x16x8 elements N=7 = {nodes} nodes, median of {len(dts)} steps "
-                      f"{step * 1e3:.0f} ms/step ({wall:.1f}s run incl. setup)"}
+                      f"(1 stage/step), {ndiv[0]}x{ndiv[1]}x{ndiv[2]} elements N=7 = {nodes} nodes, median of "
+                      f"{len(dts)} steps {step * 1e3:.0f} ms/step ({wall:.1f}s run incl. setup)"}
+
+
+def cpu_reference_pcg(threads, n=128):
+    """The reference's own Poisson solve (apps/potential, OpenMP build,
+    SYMM-FULL-DIAG-PCG to its criterion 1e-10) on an n^3 unit cube with the
+    benchmark's boundary conditions and seeded divergent velocity; time from
+    its 'Starting iterations' log line to the solver's line (divergence,
+    assembly and PCG), per cell-iteration."""
+    import subprocess
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _cases import write_potential_case
+    exe = os.path.join(ROOT, "oracle", "_ref", "potential_omp")
+    mesher = os.path.join(ROOT, "oracle", "_ref", "mesh")
+    if not (os.path.exists(exe) and os.path.exists(mesher)):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        write_potential_case(d, dict(name=f"pot_{n}", ndiv=(n, n, n)))
+        subprocess.run([mesher, "spec", "-o", "grid_0.bin"], cwd=d, check=True, capture_output=True)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([exe, "controls"], cwd=d, env=env, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            log(f"reference potential failed: {r.stderr[-300:]}")
+            return None
+        t0 = t1 = its = None
+        for ln in r.stdout.splitlines():
+            if "Starting iterations" in ln:
+                t0 = int(ln.split()[0])
+            if "PCG" in ln and "Iterations" in ln:
+                t1 = int(ln.split()[0])
+                its = int(ln.split("Iterations")[1].split()[0])
+    if t0 is None or t1 is None:
+        return None
+    sec = (t1 - t0) * 1e-3
+    return {"seconds": sec, "iterations": its, "cells": n ** 3, "cores": threads, "kind": "reference",
+            "cell_iterations_per_s": n ** 3 * its / sec,
+            "sample": f"reference apps/potential (OpenMP build, {threads} threads), {n}^3 cells, DIAG-PCG to "
+                      f"its criterion 1e-10: {its} iterations in {sec:.2f}s (divergence + assembly + PCG)"}
 
 
 def cpu_baseline(seconds_budget=20.0):
@@ -530,10 +613,9 @@ def main():
         xfer.setdefault("get_conserved_s", time.time() - t0)
         res[prec] = {"seconds": t, "rate": NQ * nb * 3 * K / t, "stage_s": t / (3 * K), "q": q}
         log(f"rk3 fp{prec}: {t * 1e3:.1f} ms for {K} steps, {res[prec]['rate'] / 1e9:.2f} G DOF*stage/s")
-    err = {}
-    for name, a, b in zip(("rho_p", "m", "Theta_p"), res[32]["q"], res[64]["q"]):
-        err[name] = rel_l2(a, b)
-    log(f"fp32 vs fp64 after {W + K} steps: {err}")
+    err = fp32_error(dg, (rho, U, T, p), 100)
+    log("fp32 vs fp64 after 100 steps (rel L2): " +
+        ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in err.items() if isinstance(v, dict)))
     # the reference's split forward-Euler step (1 stage/step)
     dg.set_state(rho, U, T, p)
     ks = max(5, K // 5)
@@ -574,10 +656,23 @@ def main():
         fv = fv_bench(args.fv_n)
     cpu = None
     if not args.no_cpu:
+        model, ncores = host_cpu()
         port = cpu_baseline(seconds_budget=10.0)
-        cpu = cpu_reference() or port
+        cpu = cpu_reference(ncores, (16, 16, 8), 6) or port
         if cpu is not port:
+            one = cpu_reference(1, (8, 8, 4), 4)
+            if one:
+                cpu["reference_single_thread"] = one
             cpu["port_single_thread"] = port
+            pcg = cpu_reference_pcg(ncores)
+            if pcg:
+                cpu["pcg"] = pcg
+                if fv:
+                    cpu["pcg"]["gpu_fp64_cell_iterations_per_s"] = (
+                        fv["cells"] * fv["fp64"]["iterations"] / fv["fp64"]["seconds"])
+        cpu["cpu_model"] = model
+        cpu["host_threads_available"] = ncores
+        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M DOF*stage/s on {cpu['cores']} threads ({model})")
     line = {
         "metric": "dGSEM DOF*stage/s (FP32-perturbation SSP-RK3, N=7)",
         "value": res[32]["rate"],
@@ -595,7 +690,7 @@ def main():
                                f"SSP-RK3 dt={dt}", "nodes": nb, "dof": NQ * nb},
         "fp64": {"value": res[64]["rate"], "ms_per_step": res[64]["seconds"] / K * 1e3},
         "fp32_over_fp64": res[32]["rate"] / res[64]["rate"],
-        "fp32_vs_fp64_rel_l2": err,
+        "fp32_vs_fp64_error": err,
         "split_reference_step_fp64": {"value": split_rate, "ms_per_step": ts / ks * 1e3},
         # host <-> device state transfers through the C-ABI (host arrays,
         # pageable memory, layout conversion included), once per run

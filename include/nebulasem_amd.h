@@ -123,8 +123,8 @@ int nsem_amr_tree_write(const char* path, const uint32_t* nodes, uint64_t n);
  * and the controls keys dt / gravity; buoyancy and diffusion as the euler
  * block (euler.cpp:23, 42-45, 104-131, 188-190).  diffusion != 0 adds the
  * artificial viscosity mu = rho viscosity (theta diffusivity mu / Pr) to the
- * split step; the method-of-lines entry points then fail with
- * NSEM_EUNSUPPORTED.                                                       */
+ * split step and the method-of-lines steps; the derivative entry points
+ * (tangent-linear, adjoint) then fail with NSEM_EUNSUPPORTED.              */
 typedef struct {
     double dt;
     double P0, T0, cp, cv;
@@ -146,8 +146,12 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
                    const double* bc_value, nsem_dg** out);
 void nsem_dg_destroy(nsem_dg* d);
 /* State as the reference stores it, interior nodes only (host arrays):
- * rho, U (xyz per node), T = theta', p = p'.  Setting it also sets the
- * method-of-lines state; getting it returns whichever path ran last. */
+ * rho, U (xyz per node), T = theta', p = p'.  Setting it sets both the split
+ * state and the method-of-lines state.  The two forms stay one state: the
+ * entry point that advances (or reads) one form first rebuilds it on the
+ * device from the other when the other was advanced last, so split steps
+ * and SSP-RK3 steps may alternate.  Getting it returns the current state
+ * (p' without cancellation when the method-of-lines form ran last). */
 int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const double* T, const double* p);
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
 /* ---- derivatives of the FP64 method-of-lines step (no reference
@@ -160,12 +164,17 @@ int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p);
 int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL);
 /* t = (dL/dq)^T w at the current state. */
 int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t);
-/* Advance nsteps FP64 SSP-RK3 steps and carry the tangent dq (in place) along. */
+/* Advance nsteps FP64 SSP-RK3 steps and carry the tangent dq (in place) along.
+ * NSEM_ENONFINITE when the state or the tangent is not finite. */
 int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq);
 /* Advance nsteps FP64 SSP-RK3 steps from q0 and return, in place of
  * lam = dJ/dq_N, the gradient dJ/dq0 = (dq_N/dq0)^T lam.  States are
- * checkpointed every `checkpoint` steps and recomputed segment by segment. */
+ * checkpointed every `checkpoint` steps and recomputed segment by segment.
+ * NSEM_ENONFINITE when the forward state or the gradient is not finite. */
 int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam);
+/* Same, returning the device time of the sweeps (forward with checkpoints,
+ * recomputation, backward), host transfers of lam excluded. */
+int nsem_dg_time_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam, double* seconds);
 /* Per-step diagnostics of the current state (split or method-of-lines),
  * reduced on the device over the interior nodes: out[6] = Courant number
  * max, min, avg (Mesh::calc_courant, field.cpp:440-448), total mass
@@ -180,7 +189,9 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
  * rho, then p, then U and theta, each a Rusanov dGSEM residual with the
  * explicit ddt/Solve update (field.h:3740-3920, solve.cpp:563-570); with
  * prm.diffusion the U and theta fluxes carry -mu gradf<true>(U) and
- * -mu/Pr gradf<true>(theta) (field.h:3327-3372). */
+ * -mu/Pr gradf<true>(theta) (field.h:3327-3372).  Waits for the steps and
+ * returns NSEM_ENONFINITE when a NaN or Inf appeared in the state (the
+ * reference runs on and writes NaN fields). */
 int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps);
 /* nsteps of SSP-RK3 (Shu-Osher) on the method-of-lines form of the same
  * spatial operator, state = conserved perturbations (rho', rho U,

@@ -298,6 +298,13 @@ class DGEuler:
         check(lib().nsem_dg_rk3_adjoint(self._h, int(n), int(checkpoint), _pd(lam)))
         return lam
 
+    def time_rk3_adjoint(self, n, lam, checkpoint=10):
+        """rk3_adjoint, also returning the device time of the sweeps."""
+        lam = np.array(lam, dtype=np.float64).reshape(5, -1)
+        s = C.c_double()
+        check(lib().nsem_dg_time_rk3_adjoint(self._h, int(n), int(checkpoint), _pd(lam), C.byref(s)))
+        return lam, s.value
+
     def set_conserved(self, rho_p, m, theta_p):
         """Set the method-of-lines state from conserved perturbations."""
         check(lib().nsem_dg_set_conserved(self._h, _pd(np.ascontiguousarray(rho_p, dtype=np.float64)),

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <memory>
 #include <vector>
 
@@ -529,8 +530,15 @@ hipError_t fwd_stage(nsem_dg* d, int stage, const double* const q[5], const doub
     a.b = kB[stage];
     a.bdt = kB[stage] * d->dev.dt;
     a.last = stage == 2;
+    a.check = stage == 2;
     a.flag = d->flag.as<int>();
     return launch_rk_stage<double, double>(d->dev, a, d->stream);
+}
+
+bool all_finite(const double* v, uint64_t n) {
+    for (uint64_t i = 0; i < n; i++)
+        if (!std::isfinite(v[i])) return false;
+    return true;
 }
 
 struct AdjointWork {
@@ -583,6 +591,7 @@ int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t) {
     if (!d || !w || !t) return nsem_fail(NSEM_EINVAL, "null argument");
     int rc = check_adjoint_support(d);
     if (rc) return rc;
+    if ((rc = nsem_dg_require_mol(d))) return rc;
     const uint64_t nb = d->nb;
     std::vector<Field5> ws;
     HIP_TRY(workspace(d, 2, ws));
@@ -601,6 +610,7 @@ int nsem_dg_residual_vjp(nsem_dg* d, const double* w, double* t) {
 int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
     if (!d || !dq || !L || !dL) return nsem_fail(NSEM_EINVAL, "null argument");
     if (int rc0 = nsem_mol_supported(d)) return rc0;
+    if (int rc0 = nsem_dg_require_mol(d)) return rc0;
     const uint64_t nb = d->nb;
     std::vector<Field5> ws;
     HIP_TRY(workspace(d, 3, ws));
@@ -628,6 +638,7 @@ int nsem_dg_residual_tlm(nsem_dg* d, const double* dq, double* L, double* dL) {
 int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq) {
     if (!d || !dq || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
     if (int rc0 = nsem_mol_supported(d)) return rc0;
+    if (int rc0 = nsem_dg_require_mol(d)) return rc0;
     const uint64_t nb = d->nb;
     std::vector<Field5> ws;
     HIP_TRY(workspace(d, 4, ws));
@@ -652,6 +663,7 @@ int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq) {
             a.b = kB[stage];
             a.bdt = kB[stage] * d->dev.dt;
             a.last = stage == 2;
+            a.check = stage == 2;
             a.flag = d->flag.as<int>();
             hipError_t e = launch_rk_stage<Dual<1>, double>(d->dev, a, d->stream);
             if (e == hipErrorInvalidValue)
@@ -662,14 +674,24 @@ int nsem_dg_rk3_tlm(nsem_dg* d, int32_t nsteps, double* dq) {
         tc = 1 - tc;
     }
     HIP_TRY(download5(dq, tq[tc], nb, d->stream));
-    d->mol_current = true;
+    if (nsteps > 0) {
+        d->mol_valid = true;
+        d->split_valid = false;
+    }
+    if (int rc0 = nsem_dg_take_flag(d, "non-finite state in the tangent-linear steps")) return rc0;
+    if (!all_finite(dq, 5 * nb)) return nsem_fail(NSEM_ENONFINITE, "non-finite tangent");
     return NSEM_OK;
 }
 
-int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam) {
+}  // extern "C"
+
+// The adjoint sweep; with `seconds`, the device time from after the upload
+// of lam to before the download of the gradient (HIP events on the stream).
+static int rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam, double* seconds) {
     if (!d || !lam || nsteps < 0 || checkpoint < 1) return nsem_fail(NSEM_EINVAL, "bad argument");
     int rc = check_adjoint_support(d);
     if (rc) return rc;
+    if ((rc = nsem_dg_require_mol(d))) return rc;
     const uint64_t nb = d->nb;
     const int nseg = (nsteps + checkpoint - 1) / checkpoint;
     // checkpoints; one segment's states and each step's stage increments
@@ -683,6 +705,7 @@ int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* 
     Field5 L = sd2[checkpoint];
     AdjointWork w{sd2[checkpoint + 1], sd2[checkpoint + 2], sd2[checkpoint + 3]};
     HIP_TRY(upload5(L, lam, nb, d->stream));
+    if (seconds) HIP_TRY(hipEventRecord(d->ev0, d->stream));
     // one segment forward from its first state seg[0], recording each step's
     // state and stage increments; the state after the segment goes to `last`
     // (if given)
@@ -724,7 +747,8 @@ int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* 
         HIP_TRY(copy5(seg[0], src, nb, d->stream));
         if ((rc = record(last0, nsteps, fin))) return rc;
         d->mcur = 1 - d->mcur;
-        d->mol_current = true;
+        d->mol_valid = true;
+        d->split_valid = false;
     }
     // backward sweep, segment by segment (recomputed from its checkpoint
     // except the last, recorded above)
@@ -742,8 +766,28 @@ int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* 
             if ((rc = adjoint_step(d, q, d1, d2, L, w))) return rc;
         }
     }
+    if (seconds) {
+        HIP_TRY(hipEventRecord(d->ev1, d->stream));
+        HIP_TRY(hipEventSynchronize(d->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+        *seconds = ms * 1e-3;
+    }
     HIP_TRY(download5(lam, L, nb, d->stream));
+    if ((rc = nsem_dg_take_flag(d, "non-finite state in the adjoint's forward sweep"))) return rc;
+    if (!all_finite(lam, 5 * nb)) return nsem_fail(NSEM_ENONFINITE, "non-finite gradient");
     return NSEM_OK;
+}
+
+extern "C" {
+
+int nsem_dg_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam) {
+    return rk3_adjoint(d, nsteps, checkpoint, lam, nullptr);
+}
+
+int nsem_dg_time_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* lam, double* seconds) {
+    if (!seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    return rk3_adjoint(d, nsteps, checkpoint, lam, seconds);
 }
 
 }  // extern "C"

@@ -109,6 +109,9 @@ struct DgDev {
     double* madj;
     // the split step's strong-gradient lift of the mortar sides [slot][npfm][12]
     double* mgrad;
+    // set to 1 by a kernel that produced a NaN or Inf in the state (the
+    // split step's update; the RK stage kernels take it in RkArgs)
+    int* flag;
 };
 
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,

@@ -60,6 +60,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     d->prm = *prm;
     const uint64_t nb = g.nbcs_field;
     d->nb = nb;
+    HIP_TRY(hipGetDevice(&d->device));
     HIP_TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&d->ev0));
     HIP_TRY(hipEventCreate(&d->ev1));
@@ -445,6 +446,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.msurf_t = d->msurf_t.as<double>();
     dv.madj = d->madj.as<double>();
     dv.mgrad = d->mgrad.as<double>();
+    dv.flag = d->flag.as<int>();
     *out = d.release();
     return NSEM_OK;
 }
@@ -476,7 +478,7 @@ int nsem_dg_set_state(nsem_dg* d, const double* rho, const double* U, const doub
         HIP_TRY(hipMemcpyAsync(d->mq[d->mcur][v].p, q[v].data(), nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    d->mol_current = false;
+    d->split_valid = d->mol_valid = true;
     return NSEM_OK;
 }
 
@@ -488,13 +490,16 @@ int nsem_dg_set_conserved(nsem_dg* d, const double* rho_p, const double* m, cons
     const double* src[5] = {rho_p, mx.data(), my.data(), mz.data(), theta_p};
     for (int v = 0; v < 5; v++)
         HIP_TRY(hipMemcpyAsync(d->mq[d->mcur][v].p, src[v], nb * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    d->mol_current = true;
+    d->mol_valid = true;
+    d->split_valid = false;
     return NSEM_OK;
 }
 
 int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p) {
     if (!d || !rho_p || !m || !theta_p) return nsem_fail(NSEM_EINVAL, "null argument");
+    if (int rc = nsem_dg_require_mol(d)) return rc;
     const uint64_t nb = d->nb;
     std::vector<double> mx(nb), my(nb), mz(nb);
     double* dst[5] = {rho_p, mx.data(), my.data(), mz.data(), theta_p};
@@ -508,7 +513,7 @@ int nsem_dg_get_conserved(nsem_dg* d, double* rho_p, double* m, double* theta_p)
 int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p) {
     if (!d || !rho || !U || !T || !p) return nsem_fail(NSEM_EINVAL, "null argument");
     const uint64_t nb = d->nb;
-    if (d->mol_current) {
+    if (!d->split_valid) {
         std::vector<double> rp(nb), m(3 * nb), tp(nb);
         int rc = nsem_dg_get_conserved(d, rp.data(), m.data(), tp.data());
         if (rc) return rc;
@@ -533,15 +538,7 @@ int nsem_dg_get_state(nsem_dg* d, double* rho, double* U, double* T, double* p) 
 
 int nsem_dg_diagnostics(nsem_dg* d, double* out) {
     if (!d || !out) return nsem_fail(NSEM_EINVAL, "null argument");
-    DiagState ds{};
-    ds.mol = d->mol_current ? 1 : 0;
-    ds.rho = d->st[d->cur][0].as<double>();
-    ds.ux = d->st[d->cur][1].as<double>();
-    ds.uy = d->st[d->cur][2].as<double>();
-    ds.uz = d->st[d->cur][3].as<double>();
-    ds.T = d->st[d->cur][4].as<double>();
-    ds.p = d->st[d->cur][5].as<double>();
-    for (int v = 0; v < 5; v++) ds.q[v] = d->mq[d->mcur][v].as<double>();
+    const DiagState ds = nsem_dg_diag_state(d);
     HIP_TRY(launch_diagnostics(d->dev, ds, d->gh.as<double>(), d->diag_slots.as<double>(),
                                d->diag_out.as<double>(), d->stream));
     HIP_TRY(hipMemcpyAsync(out, d->diag_out.p, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -567,8 +564,11 @@ int nsem_euler_initial_density(const nsem_dg* d, const double* T, const double* 
     return NSEM_OK;
 }
 
-int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
-    if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+}  // extern "C"
+
+// nsteps split steps queued on the handle's stream (no wait, flag unread).
+static int split_steps_queue(nsem_dg* d, int32_t nsteps) {
+    if (int rc = nsem_dg_require_split(d)) return rc;
     for (int32_t s = 0; s < nsteps; s++) {
         DevBuf* A = d->st[d->cur];
         DevBuf* B = d->st[1 - d->cur];
@@ -586,8 +586,19 @@ int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
         }
         d->cur = 1 - d->cur;
     }
-    d->mol_current = false;
+    if (nsteps > 0) {
+        d->split_valid = true;
+        d->mol_valid = false;
+    }
     return NSEM_OK;
+}
+
+extern "C" {
+
+int nsem_dg_split_steps(nsem_dg* d, int32_t nsteps) {
+    if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    if (int rc = split_steps_queue(d, nsteps)) return rc;
+    return nsem_dg_take_flag(d, "non-finite state in the split step");
 }
 
 }  // extern "C"
@@ -623,13 +634,51 @@ int rk3_steps_t(nsem_dg* d, int32_t nsteps, DevBuf (&dl)[2][5]) {
         }
         d->mcur = 1 - d->mcur;
     }
-    d->mol_current = true;
+    if (nsteps > 0) {
+        d->mol_valid = true;
+        d->split_valid = false;
+    }
     return NSEM_OK;
 }
 
 }  // namespace
 
-int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) { return rk3_steps_t<double, double>(d, nsteps, d->d64); }
+int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps) {
+    if (int rc = nsem_dg_require_mol(d)) return rc;
+    return rk3_steps_t<double, double>(d, nsteps, d->d64);
+}
+
+int nsem_dg_require_mol(nsem_dg* d) {
+    if (d->mol_valid) return NSEM_OK;
+    DevBuf* A = d->st[d->cur];
+    SplitState s{A[0].as<double>(), A[1].as<double>(), A[2].as<double>(), A[3].as<double>(), A[4].as<double>()};
+    ConsOut q{};
+    for (int v = 0; v < 5; v++) q.q[v] = d->mq[d->mcur][v].as<double>();
+    HIP_TRY(launch_prim_to_cons(d->dev, s, q, d->stream));
+    d->mol_valid = true;
+    return NSEM_OK;
+}
+
+int nsem_dg_require_split(nsem_dg* d) {
+    if (d->split_valid) return NSEM_OK;
+    DevBuf* A = d->st[d->cur];
+    ConsIn q{};
+    for (int v = 0; v < 5; v++) q.q[v] = d->mq[d->mcur][v].as<double>();
+    SplitOut o{A[1].as<double>(), A[2].as<double>(), A[3].as<double>(), A[4].as<double>(), A[5].as<double>()};
+    HIP_TRY(launch_cons_to_prim(d->dev, q, A[0].as<double>(), o, d->stream));
+    d->split_valid = true;
+    return NSEM_OK;
+}
+
+int nsem_dg_take_flag(nsem_dg* d, const char* what) {
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, d->flag.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (!flag) return NSEM_OK;
+    HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return nsem_fail(NSEM_ENONFINITE, what);
+}
 
 int nsem_mol_supported(const nsem_dg* d) {
     if (d->prm.diffusion)
@@ -643,55 +692,44 @@ extern "C" {
 
 int nsem_dg_rk3_steps(nsem_dg* d, int32_t nsteps, int32_t precision) {
     if (!d || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
-    int rc;
-    if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
-    else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
-    else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    if (precision != 64 && precision != 32) return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    int rc = nsem_dg_require_mol(d);
     if (rc) return rc;
-    int flag = 0;
-    HIP_TRY(hipMemcpyAsync(&flag, d->flag.p, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    if (flag) {
-        HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
-        return nsem_fail(NSEM_ENONFINITE, "non-finite residual in the RK stages");
-    }
-    return NSEM_OK;
+    if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
+    else rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
+    if (rc) return rc;
+    return nsem_dg_take_flag(d, "non-finite residual in the RK stages");
 }
 
 int nsem_dg_time_rk3(nsem_dg* d, int32_t nsteps, int32_t precision, double* seconds) {
     if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
-    int rc;
+    if (precision != 64 && precision != 32) return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    int rc = nsem_dg_require_mol(d);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
     if (precision == 64) rc = rk3_steps_t<double, double>(d, nsteps, d->d64);
-    else if (precision == 32) rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
-    else return nsem_fail(NSEM_EINVAL, "precision must be 64 or 32");
+    else rc = rk3_steps_t<float, float>(d, nsteps, d->d32);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(d->ev1, d->stream));
     HIP_TRY(hipEventSynchronize(d->ev1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     *seconds = ms * 1e-3;
-    int flag = 0;
-    HIP_TRY(hipMemcpy(&flag, d->flag.p, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag) {
-        HIP_TRY(hipMemset(d->flag.p, 0, sizeof(int)));
-        return nsem_fail(NSEM_ENONFINITE, "non-finite residual in the RK stages");
-    }
-    return NSEM_OK;
+    return nsem_dg_take_flag(d, "non-finite residual in the RK stages");
 }
 
 int nsem_dg_time_split(nsem_dg* d, int32_t nsteps, double* seconds) {
-    if (!d || !seconds) return nsem_fail(NSEM_EINVAL, "null argument");
+    if (!d || !seconds || nsteps < 0) return nsem_fail(NSEM_EINVAL, "bad argument");
+    if (int rc = nsem_dg_require_split(d)) return rc;
     HIP_TRY(hipEventRecord(d->ev0, d->stream));
-    int rc = nsem_dg_split_steps(d, nsteps);
+    int rc = split_steps_queue(d, nsteps);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(d->ev1, d->stream));
     HIP_TRY(hipEventSynchronize(d->ev1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     *seconds = ms * 1e-3;
-    return NSEM_OK;
+    return nsem_dg_take_flag(d, "non-finite state in the split step");
 }
 
 int nsem_dg_affine(const nsem_dg* d, int32_t* all_affine) {

@@ -35,6 +35,7 @@ struct nsem_dg {
     using DgDev = nsem::DgDev;
     DgDev dev{};
     nsem_euler_params prm{};
+    int device = 0;  // the HIP device current at nsem_dg_create
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t nb = 0;
@@ -56,7 +57,11 @@ struct nsem_dg {
     // workspace of the derivative entry points (grown on demand, kept)
     DevBuf adj_ws;
     int mcur = 0;
-    bool mol_current = false;
+    // which of the two forms holds the current state: both after set_state,
+    // the one advanced last otherwise (the other is rebuilt from it on the
+    // device before it is advanced or read)
+    bool split_valid = true;
+    bool mol_valid = true;
     // profiling of the dominant kernels
     bool profile = false;
     double prof_ms[4] = {0, 0, 0, 0};
@@ -72,6 +77,30 @@ struct nsem_dg {
 // nsteps FP64 SSP-RK3 steps of the method-of-lines state, queued on the
 // handle's stream without waiting or checking the non-finite flag.
 int nsem_rk3_fp64_queue(nsem_dg* d, int32_t nsteps);
+
+// Make the method-of-lines (conserved) state current, converting from the
+// split state on the device when the split step ran last; likewise the
+// split state.  Both queue on the handle's stream.
+int nsem_dg_require_mol(nsem_dg* d);
+int nsem_dg_require_split(nsem_dg* d);
+// Wait for the handle's stream, then read and clear the non-finite flag:
+// NSEM_ENONFINITE (naming `what`) when a kernel raised it, else NSEM_OK.
+int nsem_dg_take_flag(nsem_dg* d, const char* what);
+
+// The current state (split or method-of-lines form) as the diagnostics
+// kernels read it.
+inline nsem::DiagState nsem_dg_diag_state(const nsem_dg* d) {
+    nsem::DiagState ds{};
+    ds.mol = d->split_valid ? 0 : 1;
+    ds.rho = d->st[d->cur][0].as<double>();
+    ds.ux = d->st[d->cur][1].as<double>();
+    ds.uy = d->st[d->cur][2].as<double>();
+    ds.uz = d->st[d->cur][3].as<double>();
+    ds.T = d->st[d->cur][4].as<double>();
+    ds.p = d->st[d->cur][5].as<double>();
+    for (int v = 0; v < 5; v++) ds.q[v] = d->mq[d->mcur][v].as<double>();
+    return ds;
+}
 
 // NSEM_OK, or NSEM_EUNSUPPORTED when the problem has artificial viscosity,
 // which the derivative (tangent-linear, adjoint) entry points do not carry.

@@ -6,6 +6,7 @@
 // grid-stride nodes in order, block partials to slots, then one block
 // reducing the slots in a fixed order (after a kernel boundary, so every
 // slot is visible across XCDs).
+#include "dd_math.hpp"
 #include "dg_kernels.hpp"
 
 namespace nsem {
@@ -67,12 +68,12 @@ __global__ void __launch_bounds__(kDiagThreads) diag_partial_kernel(DgDev d, Dia
         }
         const double cvn = d.cV[n];
         const double uu = dot3r(u[0], u[1], u[2], u[0], u[1], u[2]);
-        const double c = sqrt(uu) * d.dt / pow(cvn, 1.0 / 3);
+        const double c = sqrt(uu) * d.dt / dd::pow_dd(cvn, 1.0 / 3);
         a.cmax = fmax(a.cmax, fabs(c));
         a.cmin = fmin(a.cmin, fabs(c));
         a.csum += c;
         a.mass += rho * cvn;
-        double sf = gh[n] + 0.5 * uu + pow((pp + d.p_ref[n]) / d.P0, R / d.cp) * th * cv;
+        double sf = gh[n] + 0.5 * uu + dd::pow_dd((pp + d.p_ref[n]) / d.P0, R / d.cp) * th * cv;
         a.energy += rho * cvn * sf;
         a.vol += cvn;
     }

@@ -113,7 +113,20 @@ hipError_t launch_diagnostics(const DgDev& d, const DiagState& s, const double* 
 
 // One split forward-Euler step (two fused element kernels; with
 // d.diffusion a gradient kernel first, writing grad, which d.grad reads).
+// A NaN or Inf in the new state sets *d.flag.
 hipError_t launch_split_step(const DgDev& d, const SplitState& s, double* rho1, const SplitOut& o,
                              double* grad, hipStream_t st);
+
+// Conserved perturbations (rho', m_x, m_y, m_z, Theta'), SoA node arrays.
+struct ConsOut {
+    double* q[5];
+};
+struct ConsIn {
+    const double* q[5];
+};
+// Primitive split state -> conserved perturbations (the form set_state uses).
+hipError_t launch_prim_to_cons(const DgDev& d, const SplitState& s, const ConsOut& q, hipStream_t st);
+// Conserved perturbations -> rho, U, theta', p' (p' without cancellation).
+hipError_t launch_cons_to_prim(const DgDev& d, const ConsIn& q, double* rho, const SplitOut& o, hipStream_t st);
 
 }  // namespace nsem

@@ -20,6 +20,7 @@
 // neighbour's trace straight from global memory, and updates itself
 // (field.h:3740-3920 with solve.cpp:563-570).  Ghost traces come from the
 // boundary condition of the owner state, never from storage.
+#include "dd_math.hpp"
 #include "dg_kernels.hpp"
 
 namespace nsem {
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
         th = s.T[n] + d.T0;
-        pp = d.P0 * pow((rho * th * R) / d.P0, gam) - d.p_ref[n];
+        pp = d.P0 * dd::pow_dd((rho * th * R) / d.P0, gam) - d.p_ref[n];
         const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
@@ -353,7 +354,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
                 orho0 = s.rho[m]; orho = rho1[m];
                 ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
                 oth = s.T[m] + d.T0;
-                opp = d.P0 * pow((orho * oth * R) / d.P0, gam) - d.p_ref[m];
+                opp = d.P0 * dd::pow_dd((orho * oth * R) / d.P0, gam) - d.p_ref[m];
                 if constexpr (VISC)
                     for (int c = 0; c < 12; c++) go[c] = d.grad[(size_t)c * d.nb + m];
             } else {
@@ -439,13 +440,19 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     const double ap0 = (-1.0 / d.dt) * cv;
     const double ap = ap0 * rho;
     double* outs[3] = {o.ux, o.uy, o.uz};
+    double un[3];
     for (int c = 0; c < 3; c++) {
         const double sc = d.buoyancy ? (rho - d.rho_ref[n]) * d.g[c] : 0.0;
-        outs[c][n] = ((res[c] - sc * cv) + (u[c] * rho0) * ap0) / ap;
+        un[c] = ((res[c] - sc * cv) + (u[c] * rho0) * ap0) / ap;
+        outs[c][n] = un[c];
     }
     const double thn = (res[3] + (th * rho0) * ap0) / ap;
     o.T[n] = thn - d.T0;
     o.p[n] = pp;
+    // the new rho (this pass's input) and U, theta: any NaN or Inf raises
+    // the problem's flag, which the step's caller reads
+    if (!(isfinite(rho) && isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]) && isfinite(thn)))
+        *d.flag = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -545,7 +552,7 @@ __global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitStat
         const double rho0 = s.rho[m], rho = rho1[m];
         const double u[3] = {s.ux[m], s.uy[m], s.uz[m]};
         const double th = s.T[m] + d.T0;
-        const double pp = d.P0 * pow((rho * th * R) / d.P0, gam) - d.p_ref[m];
+        const double pp = d.P0 * dd::pow_dd((rho * th * R) / d.P0, gam) - d.p_ref[m];
         const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) v[3 * row + c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);

@@ -39,25 +39,48 @@ def test_rk3_fp64_matches_oracle(oracle, tmp_path, case):
     print(f"[gpu] rk3 fp64 {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field
-    oracle.euler_ssprk3_steps(om, prm, p_ref, rho_ref, rho, U, T, p, 3)
-    dg.rk3_steps(3, 64)
-    g_rho, g_U, g_T, g_p = dg.get_state()
-    # The device form differs from the oracle's composition in arithmetic only
-    # (conserved perturbation variables, p' by the exact-difference formula,
-    # sum-factorised volume term).  rho and theta agree to ~1e-15; the
-    # velocity is a small residual of nearly cancelling pressure and buoyancy
-    # forces early in the run, so its relative difference is larger.
-    tol = {"rho": 1e-13, "U": 2e-9, "T": 1e-11}
-    for name, a, b in (("rho", g_rho, rho[:nb]), ("U", g_U, U[:3 * nb]), ("T", g_T, T[:nb])):
-        l2, linf = relerr(a, b)
-        print(f"  {name}: rel L2 {l2:.2e} rel max {linf:.2e}", flush=True)
-    for name, a, b in (("rho", g_rho, rho[:nb]), ("U", g_U, U[:3 * nb]), ("T", g_T, T[:nb])):
-        l2, linf = relerr(a, b)
-        assert l2 <= tol[name], (name, l2, linf)
-    # p' by the exact-difference formula vs P0 pow(...) - p_ref: a difference
-    # of ~1e5 Pa numbers, compared relative to the full pressure
-    assert np.linalg.norm(g_p - p[:nb]) / np.linalg.norm(p_ref[:nb]) <= 1e-13
+    check_rk3_against_oracle(oracle, om, prm, dg, nb, p_ref, rho_ref, rho, U, T, p, 3)
     dg.close()
+
+
+def check_rk3_against_oracle(oracle, om, prm, dg, nb, p_ref, rho_ref, rho, U, T, p, nsteps):
+    """nsteps FP64 SSP-RK3 steps on the device and in the oracle's composition
+    of the reference-pinned residual (tests/test_oracle_mol_pin.py), the
+    oracle run in both of its pinned forms: the reference's expressions
+    (mol_exact 0) and the cancellation-free ones the device uses
+    (mol_exact 1: p' = p_ref expm1(gamma log1p(Theta'/Theta_ref)), the
+    gradient of theta').
+
+    rho and theta': within 1e-12 relative L2 and max-norm of the oracle.
+    U and p': these follow Theta' = rho theta - rho_ref T0, a difference of
+    ~350 kg K m^-3 terms that cancel to ~1e-5 of themselves in a bubble near
+    pressure balance, so any two orders of the same FP64 arithmetic move
+    Theta' by ~1e-14 absolute, p' by ~1e-9 of itself and U (a small
+    residual of pressure gradient and buoyancy early in the run) by up to
+    ~1e-10 of itself.  The bound is measured per case: the device must
+    agree with the oracle's cancellation-free form at least as closely as
+    the oracle's two pinned forms agree with each other (and within 1e-12
+    when that spread is smaller).  p' is also held to 1e-13 of the full
+    pressure it perturbs."""
+    o0 = [a.copy() for a in (rho, U, T, p)]
+    prm.mol_exact = 0
+    assert oracle.euler_ssprk3_steps(om, prm, p_ref, rho_ref, *o0, nsteps) == 0
+    prm.mol_exact = 1
+    assert oracle.euler_ssprk3_steps(om, prm, p_ref, rho_ref, rho, U, T, p, nsteps) == 0
+    dg.rk3_steps(nsteps, 64)
+    g = dict(zip(("rho", "U", "T", "p"), dg.get_state()))
+    ref = {"rho": rho[:nb], "U": U[:3 * nb], "T": T[:nb], "p": p[:nb]}
+    alt = {"rho": o0[0][:nb], "U": o0[1][:3 * nb], "T": o0[2][:nb], "p": o0[3][:nb]}
+    for name in ("rho", "U", "T", "p"):
+        l2, linf = relerr(g[name], ref[name])
+        s2, si = relerr(alt[name], ref[name])
+        print(f"  {name}: rel L2 {l2:.2e} rel max {linf:.2e} (oracle form-to-form spread {s2:.1e} / {si:.1e})",
+              flush=True)
+        if name in ("rho", "T"):
+            assert l2 <= 1e-12 and linf <= 1e-12, (name, l2, linf)
+        else:
+            assert l2 <= max(1e-12, s2) and linf <= max(1e-12, si), (name, l2, linf, s2, si)
+    assert np.linalg.norm(g["p"] - ref["p"]) / np.linalg.norm(p_ref[:nb]) <= 1e-13
 
 
 def test_rk3_fp32_perturbation_error(oracle, tmp_path):

@@ -1,15 +1,22 @@
 """HIP split-step (reference-mode) Euler path vs the oracle, on the same
 decks whose oracle results are bit-identical to the reference build.
 
-Tolerance.  rho and theta: 1e-12 norm-relative (L2 and max-norm).
-p' = P0 (rho theta R/P0)^gamma - p_ref is a difference of two ~1e5 Pa
-numbers, so it is compared relative to the full pressure: 1e-13 of p_ref.
-U's tendency is a near-balance of pressure gradient and buoyancy, and the
-device's pow() is not glibc's (it can differ by an ulp), which moves U by far
-more than 1e-12.  Its bound is therefore measured, not assumed: the oracle is
-run a second time with its equation-of-state pow() moved by a pseudo-random
-+-1 ulp, and the device must stay within 10x that 1-ulp spread (and never
-above 1e-8)."""
+The equation of state p = P0 (rho theta R/P0)^gamma is the only libm call
+on this path.  The device's pow is correctly rounded (double-double,
+dd_math.hpp); glibc's, which the reference and the oracle call, is below
+0.52 ulp and rounds the other way for about one argument in a thousand.
+So each deck is compared with two builds of the oracle:
+
+* the oracle with a correctly rounded pow (quadruple-precision powq):
+  rho, U, theta', p' are bit-identical at every node and checked step on
+  conforming decks; on the 2:1 refined decks the mortar pre-pass rounds a
+  few near-zero velocities differently (within 1e-15 relative);
+* the oracle as the reference runs it (glibc pow): rho and theta' within
+  1e-12 (relative L2 and max-norm); at the first step p' differs only at
+  the nodes where glibc's pow misrounds, by at most one ulp of the full
+  pressure (the count is printed and bounded); afterwards those ulps move
+  U, whose difference from the glibc run is then exactly the correctly
+  rounded oracle's own difference from it (printed, bounded by 1e-9)."""
 import sys
 
 import numpy as np
@@ -60,29 +67,46 @@ def test_split_steps_match_oracle(oracle, tmp_path, case):
     print(f"[gpu] split steps {case['name']}", flush=True)
     mesh, om, prm, dg, (p_ref, rho_ref, rho, U, T, p) = setup(oracle, tmp_path, case)
     nb = mesh.info.nbcs_field
-    pert = [a.copy() for a in (rho, U, T, p)]
+    cr = [a.copy() for a in (rho, U, T, p)]
     done = 0
     for s in case["check_steps"][:2]:
         oracle.euler_split_steps(om, prm, p_ref, rho_ref, rho, U, T, p, s - done)
-        oracle.euler_split_steps(om, prm, p_ref, rho_ref, *pert, s - done, perturb_pow=True)
+        oracle.euler_split_steps(om, prm, p_ref, rho_ref, *cr, s - done, cr_pow=True)
         dg.split_steps(s - done)
+        first = done == 0
         done = s
-        g_rho, g_U, g_T, g_p = dg.get_state()
-        for name, a, b, c, n in (("rho", g_rho, rho, pert[0], nb), ("U", g_U, U, pert[1], 3 * nb),
-                                 ("T", g_T, T, pert[2], nb), ("p", g_p, p, pert[3], nb)):
-            l2, linf = relerr(a, b[:n])
-            b2, bi = relerr(c[:n], b[:n])
-            tol2, toli = TOL, TOL
-            if name == "U":
-                tol2, toli = min(max(TOL, 10 * b2), 1e-8), min(max(TOL, 10 * bi), 1e-8)
-            if name == "p":
-                pr = p_ref[:nb]
-                l2 = np.linalg.norm(a - b[:n]) / np.linalg.norm(pr)
-                linf = np.abs(a - b[:n]).max() / np.abs(pr).max()
-                tol2, toli = 1e-13, 1e-13
-            print(f"  step {s} {name}: rel L2 {l2:.2e} rel max {linf:.2e} "
-                  f"(1-ulp pow spread {b2:.1e} / {bi:.1e})", flush=True)
-            assert l2 <= tol2 and linf <= toli, (s, name, l2, linf, tol2, toli)
+        g = dict(zip(("rho", "U", "T", "p"), dg.get_state()))
+        n_of = {"rho": nb, "U": 3 * nb, "T": nb, "p": nb}
+        # 1. the reference algorithm with a correctly rounded pow
+        for name, b in zip(("rho", "U", "T", "p"), cr):
+            n = n_of[name]
+            l2, linf = relerr(g[name], b[:n])
+            same = np.count_nonzero(g[name] == b[:n])
+            print(f"  step {s} {name} vs oracle (correctly rounded pow): rel L2 {l2:.2e} max {linf:.2e}, "
+                  f"{same}/{n} identical", flush=True)
+            # conforming decks: every node identical; the 2:1 mortar
+            # pre-pass rounds a few near-zero velocities differently
+            assert l2 <= 1e-15 and linf <= 1e-15, (s, name, l2, linf)
+            if not case.get("refine"):
+                assert same == n, (s, name, same, n)
+        # 2. the reference algorithm as the reference runs it (glibc pow)
+        for name, b in zip(("rho", "U", "T"), (rho, U, T)):
+            n = n_of[name]
+            l2, linf = relerr(g[name], b[:n])
+            print(f"  step {s} {name} vs oracle (glibc pow): rel L2 {l2:.2e} max {linf:.2e}", flush=True)
+            tol = 1e-9 if name == "U" else TOL
+            assert l2 <= tol and linf <= tol, (s, name, l2, linf)
+        dp = g["p"] - p[:nb]
+        ndiff = np.count_nonzero(dp)
+        l2p, linfp = relerr(g["p"], p[:nb])
+        print(f"  step {s} p' vs oracle (glibc pow): {ndiff}/{nb} nodes differ, rel L2 {l2p:.2e} "
+              f"(relative to p' itself)", flush=True)
+        if first:
+            # the pow arguments are identical at the first step: the nodes that
+            # differ are glibc's misroundings, one ulp of the full pressure each
+            ulp = np.spacing(np.abs(p[:nb] + p_ref[:nb]))
+            assert np.all(np.abs(dp) <= ulp * 1.0001), np.abs(dp / ulp).max()
+            assert ndiff <= max(3, 3e-3 * nb), ndiff
     dg.close()
 
 

@@ -84,3 +84,36 @@ def test_fv_nonuniform_coefficient_keeps_stored_couplings(oracle, tmp_path):
     assert err <= 1e-9
     assert np.allclose(A, A.T, rtol=0, atol=1e-12 * np.abs(A).max())
     fv.close()
+
+
+def test_fv_iteration_limit_reports_not_converged(tmp_path):
+    """max_it reached before the tolerance: NSEM_ENOTCONVERGED with the
+    iterate and its residuals returned."""
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, POTENTIAL_CASES[1])
+    n = mesh.info.nbcs
+    fv = FVPoisson(mesh, kp, vp)
+    fv.set_rhs_divergence(U[:n], np.zeros(len(mesh.boundaries), dtype=np.int32),
+                          np.zeros((len(mesh.boundaries), 3)))
+    for mode in (0, 1, 2):
+        x, res, rc = fv.solve(mode=mode, tol=1e-10, max_it=3, raise_on_error=False)
+        print(f"  mode {mode}: rc {rc} iters {res.iterations} res {res.res_ref:.2e} true {res.res_true:.2e}",
+              flush=True)
+        assert rc == 7
+        assert res.iterations <= 3 * max(res.outer_iterations, 1)
+        assert np.all(np.isfinite(x)) and res.res_true > 1e-10
+    fv.close()
+
+
+def test_fv_indefinite_operator_reports_breakdown(tmp_path):
+    """A negative coefficient (mu = -1) makes lap(p, mu) negative definite,
+    so p.Ap < 0 at the first iteration: NSEM_EBREAKDOWN, not a NaN iterate."""
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, POTENTIAL_CASES[0])
+    n = mesh.info.nbcs
+    mu = -np.ones(mesh.info.ncells)
+    fv = FVPoisson(mesh, kp, vp, mu=mu)
+    fv.set_rhs_divergence(U[:n], np.zeros(len(mesh.boundaries), dtype=np.int32),
+                          np.zeros((len(mesh.boundaries), 3)))
+    x, res, rc = fv.solve(mode=0, tol=1e-10, max_it=100, raise_on_error=False)
+    print(f"  rc {rc} breakdown {res.breakdown} iters {res.iterations}", flush=True)
+    assert rc == 6 and res.breakdown == 1
+    fv.close()
