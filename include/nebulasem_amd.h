@@ -181,6 +181,14 @@ int nsem_dg_time_rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, dou
  * sum rho cV, energy sum rho cV (g.x + |U|^2/2 + (p/P0)^(R/cp) theta cv) and
  * volume sum cV (euler.cpp:261-283, reduce_* field.h:953-1006). */
 int nsem_dg_diagnostics(nsem_dg* d, double* out);
+/* nsem_dg_diagnostics without any device memory of the handle's or any
+ * allocation: the block partials and the six results live in a fixed
+ * 49,200-byte scratch array of the library's own, one per device for the
+ * life of the process, the same size for every problem.  No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Calls that
+ * overlap on one device race on the scratch. */
+int nsem_dg_diagnostics_noalloc(nsem_dg* d, double* out);
 /* Hydrostatic reference p_ref, rho_ref at the interior nodes (euler.cpp:121-123). */
 int nsem_dg_get_background(const nsem_dg* d, double* p_ref, double* rho_ref);
 /* Initial density from theta' and p' as euler.cpp:136-149. */

@@ -317,6 +317,12 @@ class DGEuler:
         check(lib().nsem_dg_diagnostics(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
+    def diagnostics_noalloc(self):
+        """diagnostics() through the entry point that allocates nothing."""
+        out = np.empty(6)
+        check(lib().nsem_dg_diagnostics_noalloc(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def synchronize(self):
         check(lib().nsem_dg_synchronize(self._h))
 

@@ -6,7 +6,10 @@
 // grid-stride nodes in order, block partials to slots, then one block
 // reducing the slots in a fixed order (after a kernel boundary, so every
 // slot is visible across XCDs).
+#include <mutex>
+
 #include "dd_math.hpp"
+#include "dg_context.hpp"
 #include "dg_kernels.hpp"
 
 namespace nsem {
@@ -110,3 +113,56 @@ hipError_t launch_diagnostics(const DgDev& d, const DiagState& s, const double* 
 }
 
 }  // namespace nsem
+
+// ---------------------------------------------------------------------------
+// Scratch of nsem_dg_diagnostics_noalloc: the kDiagBlocks block partials (6
+// doubles each) followed by the 6 results, 49,200 bytes, the same for every
+// problem size.  Each device holds one copy until the process ends; every
+// call on a device shares it (each call rewrites all of it before reading),
+// so calls that overlap on one device race.
+constexpr size_t kDiagScratchDoubles = (size_t)nsem::kDiagBlocks * 6 + 6;
+static __device__ __align__(256) double scratch[kDiagScratchDoubles];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is not a device address).
+hipError_t scratch_address(int device, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int nsem_dg_diagnostics_noalloc(nsem_dg* d, double* out) {
+    if (!d || !out) return nsem_fail(NSEM_EINVAL, "null argument");
+    static_assert(kDiagScratchDoubles >= (size_t)nsem::kDiagBlocks * 6 + 6, "diagnostics scratch too small");
+    if (d->device < 0 || d->device >= kMaxScratchDevices)
+        return nsem_fail(NSEM_EUNSUPPORTED, "device index beyond the scratch table (64 devices)");
+    int current = 0;
+    HIP_TRY(hipGetDevice(&current));
+    if (current != d->device) HIP_TRY(hipSetDevice(d->device));
+    double* ptr = nullptr;
+    hipError_t e = scratch_address(d->device, &ptr);
+    if (e == hipSuccess) {
+        double* slots = ptr;
+        double* res = ptr + (size_t)nsem::kDiagBlocks * 6;
+        e = nsem::launch_diagnostics(d->dev, nsem_dg_diag_state(d), d->gh.as<double>(), slots, res, d->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, res, 6 * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    }
+    if (current != d->device) (void)hipSetDevice(current);
+    if (e != hipSuccess) return nsem_fail(NSEM_EDEVICE, std::string("diagnostics (no allocation): ") + hipGetErrorString(e));
+    return NSEM_OK;
+}
