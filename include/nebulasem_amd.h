@@ -244,6 +244,14 @@ typedef struct {
  * bc_kind/bc_value per boundary patch (NEUMANN or DIRICHLET). */
 int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_kind,
                    const double* bc_value, nsem_fv** out);
+/* The same solver from an operator assembled by the caller: ap per cell
+ * (ghosts included, ncells), ano / ann per facet with the boundary
+ * conditions already folded into ap and the source (applyImplicitBCs,
+ * field.h:2541-2582), so couplings to ghost cells are ignored.  The source
+ * is set with nsem_fv_set_rhs.  For the source-compatible field API's
+ * Solve(lap(...) == ...) (solve.cpp:574-593). */
+int nsem_fv_create_coeffs(const nsem_mesh* mesh, const double* ap, const double* ano, const double* ann,
+                          nsem_fv** out);
 void nsem_fv_destroy(nsem_fv* fv);
 /* *structured: 0 general mesh (six stored couplings per cell), 1 structured
  * block numbering (three stored coupling arrays), 2 structured with every

@@ -1,6 +1,8 @@
-"""Build the native library in-tree: host C++ with g++, HIP kernels with hipcc
-for gfx950, linked into nebulasem_amd/libnebulasem_amd.so.  Also builds the
-oracle's C restatement (oracle/liboracle.so) when asked."""
+"""Build the native libraries in-tree: host C++ with g++, HIP kernels with
+hipcc for gfx950, linked into nebulasem_amd/libnebulasem_amd.so (the C-ABI);
+the field API into nebulasem_amd/libnebulasem_field.so; and, where the
+reference's source tree is present, its solver apps compiled unchanged
+against the field API."""
 import os
 import subprocess
 import sys
@@ -60,5 +62,56 @@ def build(verbose=False):
     return LIB
 
 
+FIELD = os.path.join(HERE, "field")
+FIELD_LIB = os.path.join(HERE, "libnebulasem_field.so")
+APPS = os.path.join(FIELD, "_apps")
+REFERENCE = "/root/reference"
+# the reference's solvers compiled unchanged against the field API
+API_APPS = ("euler", "potential")
+
+
+def build_field(verbose=False):
+    """The source-compatible field API (nebulasem_amd/field): its operator
+    kernels and run-time, linked against the C-ABI library."""
+    flags = HIP_FLAGS + ["-I" + os.path.join(CSRC, "hip"), "-I" + FIELD]
+    srcs = sorted(os.path.join(FIELD, f) for f in os.listdir(FIELD) if f.endswith(".hip"))
+    objs = [os.path.join(OBJ, "field_" + os.path.basename(s) + ".o") for s in srcs]
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        for r in ex.map(_run, [[HIPCC] + flags + ["-c", s, "-o", o] for s, o in zip(srcs, objs)]):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout + r.stderr)
+    tmp = FIELD_LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
+         + ["-L" + HERE, "-lnebulasem_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FIELD_LIB)
+    return FIELD_LIB
+
+
+def build_apps(reference=REFERENCE):
+    """The reference's apps/<name>/<name>.cpp, read where they lie and
+    compiled without change against the field API's headers, into
+    nebulasem_amd/field/_apps/<name>.  Needs the reference's source tree."""
+    built = []
+    if not os.path.isdir(reference):
+        return built
+    os.makedirs(APPS, exist_ok=True)
+    flags = HIP_FLAGS + ["-x", "hip", "-O2", "-I" + os.path.join(CSRC, "hip"), "-I" + FIELD]
+    jobs = []
+    for a in API_APPS:
+        src = os.path.join(reference, "apps", a, a + ".cpp")
+        jobs.append([HIPCC] + flags + ["-c", src, "-o", os.path.join(OBJ, "app_" + a + ".o")])
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(_run, jobs))
+    for a in API_APPS:
+        out = os.path.join(APPS, a)
+        _run([HIPCC, "--offload-arch=gfx950", "-o", out + ".tmp", os.path.join(OBJ, "app_" + a + ".o"),
+              "-L" + HERE, "-lnebulasem_field", "-lnebulasem_amd", "-Wl,-rpath,$ORIGIN/../.."])
+        os.replace(out + ".tmp", out)
+        built.append(out)
+    return built
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_field(verbose="-v" in sys.argv))
+    print(build_apps())

@@ -1,0 +1,56 @@
+// The weak-form DG volume term in the reference's accumulation order,
+// shared by the split step and the field API's divergence kernel.
+#pragma once
+#include "dg_common.hpp"
+
+namespace nsem {
+
+// Volume term of the reference's weak divergence (field.h:3443-3464) at
+// target node (i,j,k), subtracted from r in the reference's accumulation
+// order.  The reference walks source nodes s lexicographically and, at each,
+// subtracts dot(F_s, Jin_s grad psi_t(x_s)) from every target on the three
+// GLL lines through s.  Seen from the target the sources come as: x-line
+// a < i, y-line b < j, z-line c < k, itself, z-line c > k, y-line b > j,
+// x-line a > i.  Off the diagonal grad psi has one non-zero component, so
+// Jin grad psi is one column of Jin times a derivative value, exactly as the
+// reference's 3x3 product evaluates it.  F: LDS rows x 3 x NP, J: LDS 9 x NP.
+template <int ROWS, int VECMASK>
+__device__ __forceinline__ void volume_reference_order(double* r, const double* F, const double* J, int NP,
+                                                       int i, int j, int k, int NX, int NY, int NZ,
+                                                       const double* D0, const double* D1,
+                                                       const double* D2) {
+    auto sub = [&](int s, double w0, double w1, double w2) {
+        for (int row = 0; row < ROWS; row++) {
+            const double f0 = F[(row * 3 + 0) * NP + s];
+            const double f1 = F[(row * 3 + 1) * NP + s];
+            const double f2 = F[(row * 3 + 2) * NP + s];
+            // vector flux: right-nested dot; tensor flux: row dot left to right
+            const double c = ((VECMASK >> row) & 1) ? dot3r(f0, f1, f2, w0, w1, w2) : f0 * w0 + f1 * w1 + f2 * w2;
+            r[row] -= c;
+        }
+    };
+    // column d of Jin (XX0 YY1 ZZ2 XY3 YZ4 XZ5 YX6 ZY7 ZX8)
+    auto col = [&](int s, int d, double D) {
+        const int c0 = d == 0 ? 0 : (d == 1 ? 3 : 5);
+        const int c1 = d == 0 ? 6 : (d == 1 ? 1 : 4);
+        const int c2 = d == 0 ? 8 : (d == 1 ? 7 : 2);
+        sub(s, J[c0 * NP + s] * D, J[c1 * NP + s] * D, J[c2 * NP + s] * D);
+    };
+    const int syz = NY * NZ;
+    const int t = i * syz + j * NZ + k;
+    for (int a = 0; a < i; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
+    for (int b = 0; b < j; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
+    for (int c = 0; c < k; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
+    {
+        const double d0 = D0[i * NX + i], d1 = D1[j * NY + j], d2 = D2[k * NZ + k];
+        const double w0 = J[0 * NP + t] * d0 + J[3 * NP + t] * d1 + J[5 * NP + t] * d2;
+        const double w1 = J[6 * NP + t] * d0 + J[1 * NP + t] * d1 + J[4 * NP + t] * d2;
+        const double w2 = J[8 * NP + t] * d0 + J[7 * NP + t] * d1 + J[2 * NP + t] * d2;
+        sub(t, w0, w1, w2);
+    }
+    for (int c = k + 1; c < NZ; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
+    for (int b = j + 1; b < NY; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
+    for (int a = i + 1; a < NX; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
+}
+
+}  // namespace nsem

@@ -77,6 +77,11 @@ struct nsem_fv {
     }
 };
 
+static int fv_build(nsem_fv& d, const Geometry& g, size_t npatch, const std::vector<double>& ap,
+                    const std::vector<double>& ano, const std::vector<double>& ann, const std::vector<double>& subc,
+                    const std::vector<int32_t>& patch_of_ghost, const std::vector<int32_t>& pg_kind,
+                    const std::vector<double>& pg_off);
+
 extern "C" {
 
 int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_kind, const double* bc_value,
@@ -143,6 +148,26 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
             b++;
         }
     }
+    int rc = fv_build(*d, g, names.size(), ap, ano, ann, subc, patch_of_ghost, pg_kind, pg_off);
+    if (rc) return rc;
+    *out = d.release();
+    return NSEM_OK;
+}
+
+}  // extern "C"
+
+// The operator layout (structured block or six slots per cell), the FP32
+// copies, the divergence and correction inputs and the CG work vectors,
+// from assembled coefficients: ap per cell, ano / ann per face (boundary
+// faces already folded into ap and the source), subc the boundary part of
+// the source.
+static int fv_build(nsem_fv& dref, const Geometry& g, size_t npatch, const std::vector<double>& ap,
+                    const std::vector<double>& ano, const std::vector<double>& ann, const std::vector<double>& subc,
+                    const std::vector<int32_t>& patch_of_ghost, const std::vector<int32_t>& pg_kind,
+                    const std::vector<double>& pg_off) {
+    nsem_fv* d = &dref;
+    const uint64_t n = g.nbcs, nf = g.nfacets;
+    hipStream_t s = d->stream;
     std::vector<double> dg(ap.begin(), ap.begin() + n), idg(n);
     for (uint64_t i = 0; i < n; i++) idg[i] = 1 / dg[i];
 
@@ -283,8 +308,8 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
         (e = put(d->pg_kind, pg_kind, s)) || (e = put(d->pg_off, pg_off, s)) ||
         (e = put(d->cV, std::vector<double>(g.cV.begin(), g.cV.begin() + n), s)))
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
-    HIP_TRY(d->ubc_kind.alloc(names.size() * sizeof(int32_t) + 8));
-    HIP_TRY(d->ubc_value.alloc(names.size() * 3 * sizeof(double) + 8));
+    HIP_TRY(d->ubc_kind.alloc(npatch * sizeof(int32_t) + 8));
+    HIP_TRY(d->ubc_value.alloc(npatch * 3 * sizeof(double) + 8));
     HIP_TRY(d->U.alloc(3 * n * sizeof(double)));
     FvDivInput& dv = d->div;
     dv.ncells = n;
@@ -302,6 +327,37 @@ int nsem_fv_create(const nsem_mesh* mesh, const double* mu, const int32_t* bc_ki
     HIP_TRY(d->tmp.alloc(8 * sizeof(double)));
     HIP_TRY(hipMemcpyAsync(d->su.p, subc.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return NSEM_OK;
+}
+
+extern "C" {
+
+int nsem_fv_create_coeffs(const nsem_mesh* mesh, const double* ap, const double* ano, const double* ann,
+                          nsem_fv** out) {
+    if (!mesh || !ap || !ano || !ann || !out) return nsem_fail(NSEM_EINVAL, "null argument");
+    const Geometry& g = nsem_mesh_geometry(mesh);
+    if (g.np != 1) return nsem_fail(NSEM_EINVAL, "FV path needs a mesh loaded with order 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return nsem_fail(NSEM_EDEVICE, "no HIP device available");
+    const uint64_t n = g.nbcs, nc = g.ncells, nf = g.nfacets;
+    for (uint64_t i = 0; i < n; i++)
+        if (!(ap[i] != 0) || !std::isfinite(ap[i])) return nsem_fail(NSEM_EINVAL, "zero or non-finite diagonal");
+    auto d = std::make_unique<nsem_fv>();
+    d->n = n;
+    HIP_TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&d->ev0));
+    HIP_TRY(hipEventCreate(&d->ev1));
+    d->nghost = nc - n;
+    std::vector<double> vap(ap, ap + nc), vano(ano, ano + nf), vann(ann, ann + nf), subc(n, 0.0);
+    // couplings to ghost cells: the caller has folded the boundary
+    // conditions into the diagonal and the source
+    for (uint64_t k = 0; k < nf; k++)
+        if (g.FN[k] >= n) vann[k] = 0;
+    std::vector<int32_t> pog(nc - n, 0), pgk(nc - n, 0);
+    std::vector<double> pgo(nc - n, 0.0);
+    int rc = fv_build(*d, g, nsem_mesh_boundary_names(mesh).size(), vap, vano, vann, subc, pog, pgk, pgo);
+    if (rc) return rc;
     *out = d.release();
     return NSEM_OK;
 }

@@ -1,0 +1,114 @@
+"""The reference's own solvers, apps/euler/euler.cpp and
+apps/potential/potential.cpp, compiled without a change against the field
+API (nebulasem_amd/field: its MeshField / MeshMatrix / divf / gradf / lap /
+Solve run on the device), run on the golden decks; their output fields are
+compared with those of the reference's CPU build (tests/golden).
+
+Tolerances: the API evaluates pow and exp correctly rounded where the
+reference calls glibc's (which differs from the correctly rounded value in
+the last bit for a few arguments), and sums the conjugate gradient's dot
+products in a different order.  Per field and checked step, max |api - ref|
+is bounded by 1e-10 of max |ref| (1e-8 for the velocity, whose magnitude is
+10^-6..10^-3 of the sound speed that carries the rounding, and after 100
+steps; for the pressure perturbation p' = p - p_ref, at least 1e-13 of P0,
+the rounding of p itself); the pressure solve's iteration count by +-1 and its solution by
+1e-9 relative."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from _cases import EULER_CASES, POTENTIAL_CASES, write_euler_case, write_potential_case
+from _fieldio import read_field_bin
+from nebulasem_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APPS = os.path.join(ROOT, "nebulasem_amd", "field", "_apps")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _fixtures():
+    with open(os.path.join(GOLDEN, "reference_fixtures.json")) as f:
+        return json.load(f)
+
+
+def _arrays():
+    with np.load(os.path.join(GOLDEN, "reference_arrays.npz")) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _run(app, d, timeout=240):
+    exe = os.path.join(APPS, app)
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (nebulasem_amd.build.build_apps needs the reference's sources)")
+    r = subprocess.run([exe, "controls"], cwd=d, capture_output=True, text=True, timeout=timeout)
+    with open(os.path.join(d, "app.log"), "w") as f:
+        f.write(r.stdout + r.stderr)
+    if r.returncode != 0:
+        raise AssertionError(f"{app} exited {r.returncode}:\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
+    return r.stdout
+
+
+@pytest.mark.parametrize("case", EULER_CASES, ids=[c["name"] for c in EULER_CASES])
+def test_reference_euler_app_on_field_api(tmp_path, case):
+    print(f"[gpu] field API euler {case['name']}", flush=True)
+    d = str(tmp_path)
+    write_euler_case(d, case)
+    _lib.grid_generate(os.path.join(d, "spec"), os.path.join(d, "grid_0.bin"))
+    log = _run("euler", d)
+    assert "Exiting application run" in log
+    fx = _fixtures()["euler"][case["name"]]
+    arrays = _arrays()
+    worst = {}
+    for s in case["check_steps"]:
+        for name in ("rho", "U", "T", "p"):
+            v = read_field_bin(os.path.join(d, f"{name}{s}.bin")).reshape(-1)
+            ref = fx["steps"][str(s)][name]
+            assert v.size == ref["n"], (name, s, v.size, ref["n"])
+            tol = 1e-8 if (name == "U" or s >= 100) else 1e-10
+            l2 = float(np.linalg.norm(v))
+            # relative to the field's largest magnitude (U starts at zero);
+            # p is written as p' = p - p_ref, and its rounding is that of p
+            # itself: 1e-13 of P0 = 101325 Pa (one ulp of p is 1.5e-11)
+            atol = tol * ref["max"]
+            if name == "p":
+                atol = max(atol, 1e-13 * 101325.0)
+            assert abs(l2 - ref["l2"]) <= atol * np.sqrt(v.size), (name, s, l2, ref["l2"])
+            assert abs(float(np.max(np.abs(v))) - ref["max"]) <= atol, (name, s)
+            key = f"euler_{case['name']}_{name}{s}"
+            if key in arrays:
+                err = float(np.max(np.abs(v - arrays[key])))
+                worst[f"{name}{s}"] = err / max(ref["max"], 1e-300)
+                assert err <= atol, (name, s, err, atol)
+    print(f"  max |api - ref| / max|ref| (fields kept as arrays): {worst}", flush=True)
+
+
+@pytest.mark.parametrize("case", POTENTIAL_CASES, ids=[c["name"] for c in POTENTIAL_CASES])
+def test_reference_potential_app_on_field_api(tmp_path, case):
+    print(f"[gpu] field API potential {case['name']}", flush=True)
+    d = str(tmp_path)
+    write_potential_case(d, case)
+    _lib.grid_generate(os.path.join(d, "spec"), os.path.join(d, "grid_0.bin"))
+    log = _run("potential", d)
+    fx = _fixtures()["potential"][case["name"]]
+    line = [ln for ln in log.splitlines() if "PCG" in ln][-1]
+    toks = line.split()
+    it = int(toks[[t.endswith("Iterations") for t in toks].index(True) + 1])
+    print(f"  {line.strip()}  (reference: {fx['iterations']} iterations)", flush=True)
+    assert abs(it - fx["iterations"]) <= 1
+    p = read_field_bin(os.path.join(d, "p1.bin")).reshape(-1)
+    u = read_field_bin(os.path.join(d, "U1.bin")).reshape(-1)
+    assert p.size == fx["n"] and u.size == fx["U_n"]
+    assert abs(np.linalg.norm(p) - fx["p_l2"]) <= 1e-9 * fx["p_l2"]
+    assert abs(np.linalg.norm(u) - fx["U_l2"]) <= 1e-9 * fx["U_l2"]
+    arrays = _arrays()
+    key = f"potential_{case['name']}_p"
+    if key in arrays:
+        ep = np.linalg.norm(p - arrays[key]) / np.linalg.norm(arrays[key])
+        eu = np.linalg.norm(u - arrays[f"potential_{case['name']}_U"]) / np.linalg.norm(u)
+        print(f"  p rel err {ep:.2e}, U rel err {eu:.2e}", flush=True)
+        assert ep <= 1e-9 and eu <= 1e-9
