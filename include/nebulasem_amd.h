@@ -270,7 +270,13 @@ int nsem_fv_apply(nsem_fv* fv, const double* x, double* y);
  * PCG stopped on the reference's criterion sqrt(|sum z^2|/|sum x^2|) <= tol
  * (solve.cpp:11-35, 441-444); mode 1: FP64 PCG to |b-Ax|/|b| <= tol; mode 2:
  * FP32 inner PCG with FP64 iterative refinement to |b-Ax|/|b| <= tol.
- * Returns NSEM_ENOTCONVERGED when max_it is reached first. */
+ * Returns NSEM_ENOTCONVERGED when max_it is reached first, NSEM_EBREAKDOWN
+ * when p.Ap <= 0 stops the iteration.  mode 0 | NSEM_FV_KEEP_GOING: the
+ * reference's loop, which does not test p.Ap and carries on with
+ * alpha = sdiv(r.z, p.Ap) (solve.cpp:424-427; sdiv is 0 for a zero
+ * divisor), as the field API's Solve of a singular (all-Neumann) pressure
+ * system needs. */
+#define NSEM_FV_KEEP_GOING 4
 int nsem_fv_solve(nsem_fv* fv, double* x, int32_t mode, double tol, int32_t max_it,
                   nsem_fv_result* res);
 /* Velocity correction after the solve, U -= gradf(p, true) on the interior

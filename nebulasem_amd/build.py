@@ -67,7 +67,7 @@ FIELD_LIB = os.path.join(HERE, "libnebulasem_field.so")
 APPS = os.path.join(FIELD, "_apps")
 REFERENCE = "/root/reference"
 # the reference's solvers compiled unchanged against the field API
-API_APPS = ("euler", "potential")
+API_APPS = ("euler", "potential", "piso")
 
 
 def build_field(verbose=False):

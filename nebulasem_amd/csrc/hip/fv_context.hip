@@ -434,7 +434,8 @@ int run_cg(nsem_fv* d, const FvOp<R>& A, const R* b, R* x, R* r, R* z, R* p, R* 
     CgScalars init{};
     init.tol = tol;
     init.max_it = max_it;
-    init.mode = mode;
+    init.mode = mode & 3;
+    init.keep_going = (mode & NSEM_FV_KEEP_GOING) ? 1 : 0;
     HIP_TRY(hipMemcpyAsync(sc, &init, sizeof(init), hipMemcpyHostToDevice, s));
     HIP_TRY(fv_cg_init<R>(A, b, x, r, z, p, sc, d->slots.as<double>(), s));
     int launched = 0;
@@ -455,7 +456,10 @@ int run_cg(nsem_fv* d, const FvOp<R>& A, const R* b, R* x, R* r, R* z, R* p, R* 
 extern "C" int nsem_fv_solve(nsem_fv* d, double* x_io, int32_t mode, double tol, int32_t max_it,
                              nsem_fv_result* res) {
     if (!d || !x_io || !res) return nsem_fail(NSEM_EINVAL, "null argument");
+    const int32_t keep_going = mode & NSEM_FV_KEEP_GOING;
+    mode &= ~NSEM_FV_KEEP_GOING;
     if (mode < 0 || mode > 2 || !(tol > 0) || max_it < 1) return nsem_fail(NSEM_EINVAL, "bad solver settings");
+    if (keep_going && mode != 0) return nsem_fail(NSEM_EINVAL, "NSEM_FV_KEEP_GOING applies to mode 0 only");
     std::memset(res, 0, sizeof(*res));
     hipStream_t s = d->stream;
     const uint64_t n = d->n;
@@ -465,7 +469,7 @@ extern "C" int nsem_fv_solve(nsem_fv* d, double* x_io, int32_t mode, double tol,
     int rc = NSEM_OK;
     if (mode <= 1) {
         rc = run_cg<double>(d, d->A64, d->su.as<double>(), d->x.as<double>(), d->r.as<double>(), d->z.as<double>(),
-                            d->p.as<double>(), d->Ap.as<double>(), mode, tol, max_it, hs);
+                            d->p.as<double>(), d->Ap.as<double>(), mode | keep_going, tol, max_it, hs);
         if (rc) return rc;
         res->iterations = hs.iters;
         res->res0_ref = hs.res0_ref;

@@ -29,6 +29,7 @@ struct CgScalars {
     double alpha, beta, rz, pAp, bb, bnorm;
     double res_ref, res0_ref, res_true, rnorm0, tol;
     int32_t iters, max_it, done, breakdown, mode;  // mode 0: reference criterion, 1: |r|/|b|
+    int32_t keep_going;  // 1: a non-positive p.Ap does not stop the iteration (the reference's loop)
 };
 
 struct FvDivInput {

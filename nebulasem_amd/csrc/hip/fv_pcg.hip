@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kThreads) alpha_kernel(CgScalars* sc, const do
     if (threadIdx.x == 0) {
         sc->pAp = tot[0];
         sc->alpha = sdiv(sc->rz, tot[0]);
-        if (!(tot[0] > 0) && sc->rz != 0) sc->breakdown = 1;
+        if (!(tot[0] > 0) && sc->rz != 0 && !sc->keep_going) sc->breakdown = 1;
     }
 }
 

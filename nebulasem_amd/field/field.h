@@ -108,6 +108,8 @@ NSEM_FN2(FPow, pow(x, y))
 NSEM_FN2(FMax, max(x, y))
 NSEM_FN2(FMin, min(x, y))
 NSEM_FN2(FSdiv, sdiv(x, y))
+NSEM_FN2(FDev, dev(x, y))
+NSEM_FN2(FHyd, hyd(x, y))
 #undef NSEM_FN2
 #define NSEM_FN1(name, expr)                                                          \
     struct name {                                                                     \
@@ -427,6 +429,8 @@ NSEM_EXPR_FN2(pow, FPow)
 NSEM_EXPR_FN2(max, FMax)
 NSEM_EXPR_FN2(min, FMin)
 NSEM_EXPR_FN2(sdiv, FSdiv)
+NSEM_EXPR_FN2(dev, FDev)
+NSEM_EXPR_FN2(hyd, FHyd)
 #undef NSEM_EXPR_FN2
 #define NSEM_EXPR_FN1(name, F)                                                        \
     template <class X, class = nsem_field::unary_ok<X>>                               \

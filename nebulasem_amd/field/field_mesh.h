@@ -25,11 +25,17 @@ struct MeshTables {
     const double* cV = nullptr;
     const double* JcV = nullptr;  // 9 per node: Jinv cV (DG)
     const double* D[3] = {nullptr, nullptr, nullptr};  // dpsi per direction (DG)
-    // interior node t: entries ent[ent_off[t] .. ent_off[t+1]) = (k << 1) | side,
-    // side 0 when t is the face node's owner node FO[k], 1 when it is FN[k],
-    // in the order the reference's div_flux visits them
+    // node t (interior and ghost): entries ent[ent_off[t] .. ent_off[t+1]) =
+    // (k << 1) | side, side 0 when t is the face node's owner node FO[k], 1
+    // when it is FN[k], in the order the reference's cell loops (div_flux,
+    // mul, the matrix diagonals) visit the faces of t's own cell
     const uint32_t* ent_off = nullptr;
     const uint64_t* ent = nullptr;
+    // host copies for the level schedules of the sequential solvers
+    std::vector<uint32_t> ent_off_h, FO_h, FN_h;
+    std::vector<uint64_t> ent_h;
+    std::vector<uint8_t> has_boundary_h;  // per interior cell: a face on a boundary patch
+    uint64_t generation = 0;              // bumped by every mesh load
     // ghost nodes and their owner nodes
     const uint32_t* ghost_owner = nullptr;
     const uint32_t* ghost_node = nullptr;

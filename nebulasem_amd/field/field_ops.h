@@ -26,21 +26,37 @@ void grad_flux(const Scalar* p, int nc, const Scalar* fF, bool strong, Scalar* r
 void grad_volume(const Scalar* p, int nc, Scalar* r);
 // r = sum over the cell's faces of +-fF (sum, field.h:2990-3014)
 void sum_facet(const Scalar* fF, int nc, Scalar* r);
-// r -= x[other] a[k] over the cell's faces (the off-diagonal part of mul,
-// field.h:2369-2420): ann for the owner side, ano for the neighbour side
-void mul_offdiag(const Scalar* x, int nc, const Scalar* ano, const Scalar* ann, Scalar* r);
+// r -= (add: +=) x[other] a over the faces of each interior node's cell,
+// a = c0[k] on the owner side, c1[k] on the neighbour side: mul
+// (field.h:2369-2420) with c0 = ann, c1 = ano; its transpose mult
+// (2429-2480) with c0 = ano, c1 = ann; getRHS (2483-2534) adds with ann, ano
+void offdiag(const Scalar* x, int nc, const Scalar* c0, const Scalar* c1, bool add, Scalar* r);
+// FV implicit convection: ano, ann per face and ap += their sums
+// (div_flux_implicit, field.h:3133-3189); mu_facet only for HYBRID
+void conv_coeffs(const Scalar* flux, const Scalar* mu_facet, int scheme, Scalar blend, Scalar* ano, Scalar* ann,
+                 Scalar* ap);
 // lap coefficients: ano = ann = fD mu (penalty or FV) or mu, ap = sum of
 // the cell's couplings (field.h:3580-3612)
 void lap_coeffs(const Scalar* mu_facet, bool use_fD, Scalar* ano, Scalar* ann, Scalar* ap);
 // FV non-orthogonal correction: r = dot(g, K) with K = fN - fD (cC[FN] -
 // cC[FO]), limited to half of ano (x[FN] - x[FO]) (field.h:3676-3699)
-void nonortho_flux(const Scalar* gface, const Scalar* x, const Scalar* ano, Scalar* r);
+// (scalar fields with their vector gradient, vector fields with their
+// tensor gradient; nc components of x and r)
+void nonortho_flux(const Scalar* gface, const Scalar* x, const Scalar* ano, Scalar* r, int nc);
 // applyImplicitBCs (field.h:2541-2582) of the field fIndex's conditions
 void implicit_bcs(Scalar* ap, Scalar* ann, Scalar* Su, const Scalar* cF, int nc, Int fIndex);
 // SolveT for a scalar FV system (ap, ano, ann after applyImplicitBCs, Su,
 // solution x in place; solve.cpp:39-559 PCG with the DIAG preconditioner)
+// by the library's conjugate gradient
 void solve_fv_pcg(const Scalar* ap, const Scalar* ano, const Scalar* ann, const Scalar* Su, Scalar* x,
                   Int fIndex);
+// SolveT for any FV system and every method and preconditioner: Jacobi,
+// SOR, conjugate gradient (symmetric) or biconjugate gradient, with none,
+// DIAG, SSOR or DILU preconditioning, in the reference's order of
+// operations throughout (level-scheduled substitutions and sweeps,
+// sequential-order dot products)
+void solve_general(const Scalar* ap, const Scalar* ano, const Scalar* ann, const Scalar* Su, Scalar* x, int nc,
+                   int flags);
 // entity sizes, DG shape
 bool dg();
 }  // namespace nsem_field
@@ -254,8 +270,26 @@ template <class T1, class T2, class T3>
 MeshField<T3, CELL> mul(const MeshMatrix<T1, T2, T3>& p, const MeshField<T1, CELL>& q, const bool = false) {
     MeshField<T3, CELL> r = q * p.ap;
     if (p.flags & p.DIAGONAL) return r;
-    nsem_field::mul_offdiag(nsem_field::S(q.dptr()), nsem_field::NC<T1>, p.ano.dptr(), p.ann.dptr(),
-                            nsem_field::S(r.dptr_w()));
+    nsem_field::offdiag(nsem_field::S(q.dptr()), nsem_field::NC<T1>, p.ann.dptr(), p.ano.dptr(), false,
+                        nsem_field::S(r.dptr_w()));
+    return r;
+}
+// A^T x (mult, field.h:2429-2480)
+template <class T1, class T2, class T3>
+MeshField<T3, CELL> mult(const MeshMatrix<T1, T2, T3>& p, const MeshField<T1, CELL>& q, const bool = false) {
+    MeshField<T3, CELL> r = q * p.ap;
+    if (p.flags & p.DIAGONAL) return r;
+    nsem_field::offdiag(nsem_field::S(q.dptr()), nsem_field::NC<T1>, p.ano.dptr(), p.ann.dptr(), false,
+                        nsem_field::S(r.dptr_w()));
+    return r;
+}
+// b - (L + U) x (getRHS, field.h:2483-2534)
+template <class T1, class T2, class T3>
+MeshField<T3, CELL> getRHS(const MeshMatrix<T1, T2, T3>& p, const MeshField<T1, CELL>& q, const bool = false) {
+    MeshField<T3, CELL> r = p.Su;
+    if (p.flags & p.DIAGONAL) return r;
+    nsem_field::offdiag(nsem_field::S(q.dptr()), nsem_field::NC<T1>, p.ann.dptr(), p.ano.dptr(), true,
+                        nsem_field::S(r.dptr_w()));
     return r;
 }
 template <class T1, class T2, class T3, class A>
@@ -300,7 +334,7 @@ MeshMatrix<type> lap(MeshField<type, CELL>& cF, const ScalarCellField& muc, cons
             cds(muc * gradf<false>(cF, true));
         MeshField<type, FACET> r;
         nsem_field::nonortho_flux(nsem_field::S(g.dptr()), nsem_field::S(cF.dptr()), m.ano.dptr(),
-                                  nsem_field::S(r.dptr_w()));
+                                  nsem_field::S(r.dptr_w()), nsem_field::NC<type>);
         m.Su = sum(r);
     }
     (void)penalty;
@@ -309,6 +343,34 @@ MeshMatrix<type> lap(MeshField<type, CELL>& cF, const ScalarCellField& muc, cons
 template <bool strong = false, class type>
 MeshMatrix<type> lap(MeshField<type, CELL>& cF, const Scalar& mu, const bool penalty = false) {
     return lap<strong>(cF, ScalarCellField(mu), penalty);
+}
+
+// FV implicit convection (div, field.h:3496-3567, and the FV part of
+// div_flux_implicit, 3133-3189); the deferred-correction schemes (LUD and
+// after) and the DG implicit form are not built
+template <bool strong = false, class type>
+MeshMatrix<type> div(MeshField<type, CELL>& cF, const VectorCellField& fluxc, const ScalarFacetField& flux,
+                     const ScalarFacetField* lambdaMax = 0, const ScalarCellField* muc = 0) {
+    using namespace Controls;
+    if (nsem_field::dg()) nsem_field::fail("div: the implicit DG convection is not part of this build");
+    if (convection_scheme >= LUD)
+        nsem_field::fail("div: the deferred-correction convection schemes (LUD and after) are not part of this build");
+    (void)fluxc;
+    (void)lambdaMax;
+    MeshMatrix<type> m;
+    m.cF = &cF;
+    m.flags = 0;
+    m.Su = type(0);
+    m.ap = Scalar(0);
+    ScalarFacetField mu_f(false);
+    const Scalar* mup = nullptr;
+    if (convection_scheme == HYBRID && muc) {
+        mu_f = cds(*muc);
+        mup = mu_f.dptr();
+    }
+    nsem_field::conv_coeffs(flux.dptr(), mup, (int)convection_scheme, blend_factor, m.ano.dptr_w(), m.ann.dptr_w(),
+                            m.ap.dptr_w());
+    return m;
 }
 
 // -------------------------------------------------- temporal derivative --
@@ -397,6 +459,60 @@ void addTemporal(MeshMatrix<type>& M, Scalar cF_UR, ScalarCellField* rho = 0, Sc
     } else {
         nsem_field::fail("addTemporal: the Adams schemes are not part of this build");
     }
+}
+
+
+// transport-equation forms (field.h:3924-3982)
+template <class type>
+MeshMatrix<type> diffusion(MeshField<type, CELL>& cF, const ScalarCellField& mu, Scalar cF_UR,
+                           ScalarCellField* rho = 0, ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = -lap(cF, mu, true);
+    M.cF = &cF;
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
+}
+template <class type>
+MeshMatrix<type> diffusion(MeshField<type, CELL>& cF, const ScalarCellField& mu, Scalar cF_UR,
+                           const MeshField<type, CELL>& Su, const ScalarCellField& Sp, ScalarCellField* rho = 0,
+                           ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = -lap(cF, mu, true) - src(cF, Su, Sp);
+    M.cF = &cF;
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
+}
+template <class type>
+MeshMatrix<type> convection(MeshField<type, CELL>& cF, const VectorCellField& Fc, const ScalarFacetField& F,
+                            Scalar cF_UR, const ScalarFacetField* lambdaMax = 0, ScalarCellField* rho = 0,
+                            ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = div(cF, Fc, F, lambdaMax);
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
+}
+template <class type>
+MeshMatrix<type> convection(MeshField<type, CELL>& cF, const VectorCellField& Fc, const ScalarFacetField& F,
+                            Scalar cF_UR, const MeshField<type, CELL>& Su, const ScalarCellField& Sp,
+                            const ScalarFacetField* lambdaMax = 0, ScalarCellField* rho = 0,
+                            ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = div(cF, Fc, F, lambdaMax) - src(cF, Su, Sp);
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
+}
+template <class type>
+MeshMatrix<type> transport(MeshField<type, CELL>& cF, const VectorCellField& Fc, const ScalarFacetField& F,
+                           const ScalarCellField& mu, Scalar cF_UR, const ScalarFacetField* lambdaMax = 0,
+                           ScalarCellField* rho = 0, ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = div(cF, Fc, F, lambdaMax, &mu) - lap(cF, mu);
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
+}
+template <class type>
+MeshMatrix<type> transport(MeshField<type, CELL>& cF, const VectorCellField& Fc, const ScalarFacetField& F,
+                           const ScalarCellField& mu, Scalar cF_UR, const MeshField<type, CELL>& Su,
+                           const ScalarCellField& Sp, const ScalarFacetField* lambdaMax = 0,
+                           ScalarCellField* rho = 0, ScalarCellField* rho0 = 0) {
+    MeshMatrix<type> M = div(cF, Fc, F, lambdaMax, &mu) - lap(cF, mu) - src(cF, Su, Sp);
+    addTemporal<1>(M, cF_UR, rho, rho0);
+    return M;
 }
 
 #endif

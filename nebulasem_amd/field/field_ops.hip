@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(512) grad_volume_kernel(DevTables t, const T* 
 template <class T>
 __global__ void sum_facet_kernel(DevTables t, const T* fF, T* r) {
     const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= t.nbf) return;
+    if (n >= t.nall) return;
     T acc(0);
     for (uint32_t e = t.ent_off[n]; e < t.ent_off[n + 1]; e++) {
         const uint64_t v = t.ent[e];
@@ -186,16 +186,23 @@ __global__ void sum_facet_kernel(DevTables t, const T* fF, T* r) {
     r[n] = acc;
 }
 
+// r -= (or +=) x[other node] a over the faces of the node's cell: a = c0[k]
+// where the node owns face node k, c1[k] where it is the neighbour
 template <class T>
-__global__ void mul_offdiag_kernel(DevTables t, const T* x, const Scalar* ano, const Scalar* ann, T* r) {
+__global__ void offdiag_kernel(DevTables t, const T* x, const Scalar* c0, const Scalar* c1, int add, T* r) {
     const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= t.nbf) return;
     T acc = r[n];
     for (uint32_t e = t.ent_off[n]; e < t.ent_off[n + 1]; e++) {
         const uint64_t v = t.ent[e];
         const uint64_t k = v >> 1;
-        if (v & 1) acc -= x[t.FO[k]] * ano[k];
-        else acc -= x[t.FN[k]] * ann[k];
+        if (add) {
+            if (v & 1) acc += x[t.FO[k]] * c1[k];
+            else acc += x[t.FN[k]] * c0[k];
+        } else {
+            if (v & 1) acc -= x[t.FO[k]] * c1[k];
+            else acc -= x[t.FN[k]] * c0[k];
+        }
     }
     r[n] = acc;
 }
@@ -207,15 +214,40 @@ __global__ void lap_face_kernel(DevTables t, const Scalar* mu, int use_fD, Scala
     ano[k] = a;
     ann[k] = a;
 }
-__global__ void lap_diag_kernel(DevTables t, const Scalar* ano, const Scalar* ann, Scalar* ap) {
+// ap += the couplings of the faces of each node's cell, ghost cells
+// included (the reference's loops run over gNCells)
+__global__ void diag_sum_kernel(DevTables t, const Scalar* ano, const Scalar* ann, Scalar* ap) {
     const uint64_t n = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= t.nbf) return;
+    if (n >= t.nall) return;
     Scalar acc = ap[n];
     for (uint32_t e = t.ent_off[n]; e < t.ent_off[n + 1]; e++) {
         const uint64_t v = t.ent[e];
         acc += (v & 1) ? ann[v >> 1] : ano[v >> 1];
     }
     ap[n] = acc;
+}
+
+// FV implicit convection coefficients (div_flux_implicit, field.h:3147-3189):
+// blending factor gamma per face (CDS 1, BLENDED the blend factor, HYBRID
+// from the face Peclet comparison, otherwise upwind), then
+// ano = G (-F fI) + (1 - G)(-max(F, 0)), ann = G (F (1 - fI)) + (1 - G)(-max(-F, 0))
+__global__ void conv_face_kernel(DevTables t, const Scalar* flux, const Scalar* mu, int scheme, Scalar blend,
+                                 Scalar* ano, Scalar* ann) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= t.nfn) return;
+    const Scalar F = flux[k], fI = t.fI[k];
+    Scalar G = 0;
+    if (scheme == Controls::CDS) G = 1;
+    else if (scheme == Controls::BLENDED) G = blend;
+    else if (scheme == Controls::HYBRID && mu && t.FN[k] < t.nall) {
+        const Scalar D = t.fD[k] * mu[k];
+        if (F < 0) G = (F * fI + D >= 0) ? 1 : 0;
+        else G = (F * (1 - fI) - D > 0) ? 0 : 1;
+    }
+    const Scalar mF = F >= 0 ? F : Scalar(0);
+    const Scalar nF = -F >= 0 ? -F : Scalar(0);
+    ano[k] = ((G) * (-F * (fI)) + (1 - G) * (-mF));
+    ann[k] = ((G) * (F * (1 - fI)) + (1 - G) * (-nF));
 }
 
 template <class T, class G>
@@ -507,26 +539,41 @@ void grad_volume(const Scalar* p, int nc, Scalar* r) {
 }
 void sum_facet(const Scalar* fF, int nc, Scalar* r) {
     const DevTables t = dev();
-    NSEM_DISPATCH(nc, hipLaunchKernelGGL((sum_facet_kernel<T>), dim3(blocks_for(t.nbf)), dim3(kThreads), 0, stream(), t,
+    NSEM_DISPATCH(nc, hipLaunchKernelGGL((sum_facet_kernel<T>), dim3(blocks_for(t.nall)), dim3(kThreads), 0, stream(), t,
                                          reinterpret_cast<const T*>(fF), reinterpret_cast<T*>(r)));
     check(hipGetLastError(), "sum");
 }
-void mul_offdiag(const Scalar* x, int nc, const Scalar* ano, const Scalar* ann, Scalar* r) {
+void offdiag(const Scalar* x, int nc, const Scalar* c0, const Scalar* c1, bool add, Scalar* r) {
     const DevTables t = dev();
-    NSEM_DISPATCH(nc, hipLaunchKernelGGL((mul_offdiag_kernel<T>), dim3(blocks_for(t.nbf)), dim3(kThreads), 0, stream(),
-                                         t, reinterpret_cast<const T*>(x), ano, ann, reinterpret_cast<T*>(r)));
-    check(hipGetLastError(), "mul");
+    NSEM_DISPATCH(nc, hipLaunchKernelGGL((offdiag_kernel<T>), dim3(blocks_for(t.nbf)), dim3(kThreads), 0, stream(), t,
+                                         reinterpret_cast<const T*>(x), c0, c1, add ? 1 : 0, reinterpret_cast<T*>(r)));
+    check(hipGetLastError(), "matrix off-diagonal product");
 }
 void lap_coeffs(const Scalar* mu, bool use_fD, Scalar* ano, Scalar* ann, Scalar* ap) {
     const DevTables t = dev();
     hipLaunchKernelGGL(lap_face_kernel, dim3(blocks_for(t.nfn)), dim3(kThreads), 0, stream(), t, mu, use_fD ? 1 : 0, ano, ann);
-    hipLaunchKernelGGL(lap_diag_kernel, dim3(blocks_for(t.nbf)), dim3(kThreads), 0, stream(), t, ano, ann, ap);
+    hipLaunchKernelGGL(diag_sum_kernel, dim3(blocks_for(t.nall)), dim3(kThreads), 0, stream(), t, ano, ann, ap);
     check(hipGetLastError(), "lap");
 }
-void nonortho_flux(const Scalar* g, const Scalar* x, const Scalar* ano, Scalar* r) {
+void conv_coeffs(const Scalar* flux, const Scalar* mu_facet, int scheme, Scalar blend, Scalar* ano, Scalar* ann,
+                 Scalar* ap) {
     const DevTables t = dev();
-    hipLaunchKernelGGL((nonortho_kernel<Scalar, Vector>), dim3(blocks_for(t.nfn)), dim3(kThreads), 0, stream(), t,
-                       reinterpret_cast<const Vector*>(g), x, ano, r);
+    hipLaunchKernelGGL(conv_face_kernel, dim3(blocks_for(t.nfn)), dim3(kThreads), 0, stream(), t, flux, mu_facet, scheme,
+                       blend, ano, ann);
+    hipLaunchKernelGGL(diag_sum_kernel, dim3(blocks_for(t.nall)), dim3(kThreads), 0, stream(), t, ano, ann, ap);
+    check(hipGetLastError(), "convection");
+}
+void nonortho_flux(const Scalar* g, const Scalar* x, const Scalar* ano, Scalar* r, int nc) {
+    const DevTables t = dev();
+    if (nc == 1)
+        hipLaunchKernelGGL((nonortho_kernel<Scalar, Vector>), dim3(blocks_for(t.nfn)), dim3(kThreads), 0, stream(), t,
+                           reinterpret_cast<const Vector*>(g), x, ano, r);
+    else if (nc == 3)
+        hipLaunchKernelGGL((nonortho_kernel<Vector, Tensor>), dim3(blocks_for(t.nfn)), dim3(kThreads), 0, stream(), t,
+                           reinterpret_cast<const Tensor*>(g), reinterpret_cast<const Vector*>(x), ano,
+                           reinterpret_cast<Vector*>(r));
+    else
+        fail("lap: the non-orthogonal correction of a tensor field is not built");
     check(hipGetLastError(), "non-orthogonal correction");
 }
 void implicit_bcs(Scalar* ap, Scalar* ann, Scalar* Su, const Scalar* cF, int nc, Int fIndex) {
@@ -574,7 +621,8 @@ void solve_fv_pcg(const Scalar* ap, const Scalar* ano, const Scalar* ann, const 
         fail(std::string("Solve: ") + nsem_last_error());
     if (nsem_fv_set_rhs(fv, hsu.data())) fail(std::string("Solve: ") + nsem_last_error());
     nsem_fv_result res{};
-    const int rc = nsem_fv_solve(fv, hx.data(), 0, Controls::tolerance, (int32_t)Controls::max_iterations, &res);
+    const int rc = nsem_fv_solve(fv, hx.data(), 0 | NSEM_FV_KEEP_GOING, Controls::tolerance,
+                                 (int32_t)Controls::max_iterations, &res);
     nsem_fv_destroy(fv);
     if (rc != NSEM_OK && rc != NSEM_ENOTCONVERGED) fail(std::string("Solve: ") + nsem_last_error());
     upload(x, hx.data(), m.nbcs * sizeof(double));
@@ -595,6 +643,7 @@ static void solve_impl(const MeshMatrix<T1, T2, T3>& cA) {
     // applyImplicitBCs (field.h:2541-2582)
     nsem_field::implicit_bcs(A.ap.dptr_w(), A.ann.dptr_w(), nsem_field::S(A.Su.dptr_w()),
                              nsem_field::S(A.cF->dptr()), NC, A.cF->fIndex);
+    using namespace Controls;
     if (A.flags & A.DIAGONAL) {
         // SolveTexplicit (solve.cpp:563-570)
         *A.cF = A.Su / A.ap;
@@ -602,11 +651,13 @@ static void solve_impl(const MeshMatrix<T1, T2, T3>& cA) {
             MP::printH("SYMM-DIAG-DIAG-JAC:");
             MP::print("Iterations %d Initial Residual %.5e Final Residual %.5e\n", 1, 0.0, 0.0);
         }
-    } else if constexpr (NC == 1) {
-        nsem_field::solve_fv_pcg(A.ap.dptr(), A.ano.dptr(), A.ann.dptr(), A.Su.dptr(), A.cF->dptr_w(),
-                                 A.cF->fIndex);
+    } else if (NC == 1 && Solver == PCG && Preconditioner == DIAG && (A.flags & A.SYMMETRIC)) {
+        // the pressure-equation case: the library's conjugate gradient
+        nsem_field::solve_fv_pcg(A.ap.dptr(), A.ano.dptr(), A.ann.dptr(), nsem_field::S(A.Su.dptr()),
+                                 nsem_field::S(A.cF->dptr_w()), A.cF->fIndex);
     } else {
-        nsem_field::fail("Solve: only scalar FV systems are solved iteratively here");
+        nsem_field::solve_general(A.ap.dptr(), A.ano.dptr(), A.ann.dptr(), nsem_field::S(A.Su.dptr()),
+                                  nsem_field::S(A.cF->dptr_w()), NC, (int)A.flags);
     }
     applyExplicitBCs(*A.cF, true);
 }

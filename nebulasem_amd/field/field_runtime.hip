@@ -20,7 +20,9 @@
 #include "iteration.h"
 #include "nebulasem_amd.h"
 #include "properties.h"
+#include "calc_walldist.h"
 #include "solve.h"
+#include "turbulence.h"
 #include "wrapper.h"
 
 // ------------------------------------------------------------- globals --
@@ -435,8 +437,8 @@ static void load_mesh_file(const std::string& path) {
     // the order div_flux visits them (the cell's faces in allFaces order,
     // then the face's nodes), with the side: +1 owner (FO), -1 neighbour (FN)
     {
-        std::vector<std::vector<int64_t>> ent(m.nbf);
-        for (uint64_t i = 0; i < m.nbcs; i++)
+        std::vector<std::vector<int64_t>> ent(m.nall);
+        for (uint64_t i = 0; i < m.ncells; i++)
             for (uint32_t f = fi0[i]; f < fi1[i]; f++)
                 for (uint32_t n = 0; n < m.npf; n++) {
                     const uint64_t k = (uint64_t)allFaces[f] * m.npf + n;
@@ -444,14 +446,22 @@ static void load_mesh_file(const std::string& path) {
                     if (c1 >= i * m.np && c1 < (i + 1) * m.np) ent[c1].push_back((int64_t)(k << 1));
                     else if (c2 < m.nall) ent[c2].push_back((int64_t)((k << 1) | 1));
                 }
-        std::vector<uint32_t> off(m.nbf + 1, 0);
+        std::vector<uint32_t> off(m.nall + 1, 0);
         std::vector<uint64_t> lst;
-        for (uint64_t t = 0; t < m.nbf; t++) {
+        for (uint64_t t = 0; t < m.nall; t++) {
             for (int64_t e : ent[t]) lst.push_back((uint64_t)e);
             off[t + 1] = (uint32_t)lst.size();
         }
         m.ent_off = put(off);
         m.ent = put(lst);
+        m.ent_off_h = std::move(off);
+        m.ent_h = std::move(lst);
+        m.FO_h = FO;
+        m.FN_h = FN;
+        std::vector<uint32_t> hb = mesh_array<uint32_t>("has_boundary");
+        m.has_boundary_h.assign(hb.begin(), hb.end());
+        m.has_boundary_h.resize(m.nbcs, 0);
+        m.generation++;
     }
     // ghost nodes: owner node of each ghost node (fillBCs, field.h:2733-2743)
     {
@@ -644,3 +654,24 @@ Vector Mesh::calc_courant(const VectorCellField& U, Scalar dt) {
     const Scalar avgc = reduce_sum(Courant) / Scalar(gBCSfield);
     return Vector(maxc, minc, avgc);
 }
+
+// ------------------------------------------------------------ turbulence --
+// the reference's default selection is k-e (turbulence.cpp:8)
+Int Turbulence_Model::turb_model = 2;
+bool Turbulence_Model::bneedWallDist = false;
+void Turbulence_Model::RegisterTable(Util::ParamList& params) {
+    params.enroll("turbulence_model",
+                  new Util::Option(&turb_model, {"NONE", "MIXING_LENGTH", "KE", "RNG_KE", "REALIZABLE_KE", "KW", "LES"}));
+}
+// Select (turbulence.cpp:21-57): NONE is the base (laminar) model
+Turbulence_Model* Turbulence_Model::Select(VectorCellField& U, VectorCellField& Fc, ScalarFacetField& F,
+                                           ScalarCellField& rho, ScalarCellField& mu) {
+    bneedWallDist = false;
+    if (turb_model != 0)
+        nsem_field::fail("turbulence_model: only NONE (laminar) is part of this build; the RANS and LES models "
+                         "are not");
+    Turbulence_Model* t = new Turbulence_Model(U, Fc, F, rho, mu);
+    t->enroll();
+    return t;
+}
+void Mesh::calc_walldist(Int, Int) { nsem_field::fail("wall distance is not part of this build"); }

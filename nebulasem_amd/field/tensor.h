@@ -130,6 +130,26 @@ template <int N>
 NSEM_HD TTensor<N> mul(const TTensor<N>& p, const Scalar& q) { return p * q; }
 template <int N>
 NSEM_HD Scalar tr(const TTensor<N>& p) { return p[0] + p[1] + p[2]; }
+// deviatoric part: the trace times factor / 3 off the diagonal entries
+template <int N>
+NSEM_HD TTensor<N> dev(const TTensor<N>& p, const Scalar& factor) {
+    TTensor<N> r = p;
+    const Scalar t = tr(p) * factor / 3;
+    r[0] -= t;
+    r[1] -= t;
+    r[2] -= t;
+    return r;
+}
+// hydrostatic part: the trace times factor / 3 on the diagonal, the rest 0
+template <int N>
+NSEM_HD TTensor<N> hyd(const TTensor<N>& p, const Scalar& factor) {
+    TTensor<N> r(1, 1, 1);
+    const Scalar t = tr(p) * factor / 3;
+    r[0] = t;
+    r[1] = t;
+    r[2] = t;
+    return r;
+}
 
 // component-wise functions of the C library, as the reference applies them
 #define NSEM_TT_FN1(name, impl)                                                \

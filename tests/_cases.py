@@ -138,9 +138,9 @@ def write_potential_case(d, case):
 }
 potential
 {
-    n_ORTHO                      0
+    n_ORTHO                      %d
 }
-""" % case.get("tol", 1e-10))
+""" % (case.get("tol", 1e-10), case.get("n_ortho", 0)))
     U = potential_velocity(case)
     names = patch_names(c)
     with open(os.path.join(d, "U0.txt"), "w") as f:
@@ -235,4 +235,109 @@ POTENTIAL_CASES = [
     dict(name="pot_8", ndiv=(8, 8, 8), keep_fields=True),
     dict(name="pot_16", ndiv=(16, 16, 16)),
     dict(name="pot_12x8x6", ndiv=(12, 8, 6), seed=7),
+]
+
+# a sheared unit cube (x shifted by 0.3 z, y by 0.2 x): every face is
+# non-orthogonal, so the laminar correction of lap (field.h:3676-3699) is
+# live; with n_ORTHO 2 the pressure system is assembled three times, the
+# later ones with the correction of the previous solution
+POTENTIAL_SKEW_CASES = [
+    dict(name="pot_skew", ndiv=(10, 8, 6), seed=5, n_ortho=2, keep_fields=True,
+         corners=[(0, 0, 0), (1, 0.2, 0), (1.3, 0.2, 1), (0.3, 0, 1),
+                  (0, 1, 0), (1, 1.2, 0), (1.3, 1.2, 1), (0.3, 1, 1)]),
+]
+
+
+# --------------------------------------------------------------- PISO --
+# lid-driven cavity decks for the piso solver (apps/piso/piso.cpp), after
+# the reference's examples/cavity: rho 1, viscosity 0.1, HYBRID convection,
+# OVER_RELAXED non-orthogonal correction, laminar (turbulence_model NONE),
+# two PISO correctors; the lid moves at U = (1, 0, 0).  Each case picks a
+# linear-solver method / preconditioner (the default preconditioner is
+# SSOR) and whether the momentum equation is solved (momentum_predictor,
+# a non-symmetric system: biconjugate gradient).
+PISO_PATCHES_3D = [("top_WALL", [(3, 2, 6, 7)]),
+                   ("all_WALL", [(0, 1, 5, 4), (0, 1, 2, 3), (4, 5, 6, 7), (1, 5, 6, 2), (0, 4, 7, 3)])]
+PISO_PATCHES_2D = [("top_WALL", [(3, 2, 6, 7)]), ("delete", [(0, 1, 2, 3), (4, 5, 6, 7)]),
+                   ("all_WALL", [(1, 5, 6, 2), (0, 1, 5, 4), (0, 4, 7, 3)])]
+
+
+def write_piso_case(d, case):
+    os.makedirs(d, exist_ok=True)
+    c = dict(case, order=(0, 0, 0))
+    with open(os.path.join(d, "spec"), "w") as f:
+        f.write(_spec(c))
+    extra = "".join("    %-28s %s\n" % kv for kv in case.get("general", {}).items())
+    piso = "".join("    %-28s %s\n" % kv for kv in case.get("piso", {}).items())
+    with open(os.path.join(d, "controls"), "w") as f:
+        f.write("""general
+{
+    solver                       piso
+    mesh                         grid
+    rho                          1
+    viscosity                    1e-1
+    state                        %s
+    start_step                   0
+    end_step                     %d
+    write_interval               1
+    dt                           %r
+    convection_scheme            HYBRID
+    nonortho_scheme              OVER_RELAXED
+    blend_factor                 0.2
+    method                       %s
+    tolerance                    %r
+    max_iterations               6400
+    SOR_omega                    1.7
+    gravity                      0 0 0
+    npx                          0
+    npy                          0
+    npz                          0
+%s}
+piso
+{
+    turbulence_model             NONE
+    velocity_UR                  0.7
+    pressure_UR                  0.3
+    n_PISO                       2
+    n_ORTHO                      0
+%s}
+""" % (case.get("state", "TRANSIENT"), max(case["check_steps"]), case.get("dt", 0.005),
+       case.get("method", "PCG"), case.get("tol", 1e-5), extra, piso))
+    names = sorted(n for n, _ in case["patches"] if n != "delete")
+    with open(os.path.join(d, "U0.txt"), "w") as f:
+        f.write("size 3\ninternal 1\n{\n    uniform 0 0 0\n}\nboundary %d\n{\n" % len(names))
+        for n in names:
+            v = "1.0 0 0" if n == "top_WALL" else "0 0 0"
+            f.write("    %s {\n        type DIRICHLET\n        value %s\n    }\n" % (n, v))
+        f.write("}\n")
+    with open(os.path.join(d, "p0.txt"), "w") as f:
+        f.write(field_file(1, ["uniform %r" % case.get("p0", 0.0)], [(n, "NEUMANN") for n in names]))
+
+
+# p starts at 0 and the tolerance is 1e-8, so that every solve iterates
+# (the residual is relative to the solution's norm)
+_CAV3D = dict(ndiv=(6, 6, 6), size=(1, 1, 1), patches=PISO_PATCHES_3D, check_steps=[1, 3], tol=1e-8)
+PISO_CASES = [
+    # pressure: conjugate gradient with the default SSOR preconditioner
+    dict(_CAV3D, name="cav3d_ssor", keep_fields=True),
+    # momentum solved (biconjugate gradient) and pressure, both DILU
+    dict(_CAV3D, name="cav3d_dilu_mp", general={"preconditioner": "DILU"},
+         piso={"momentum_predictor": "YES"}),
+    # DIAG: pressure by the library's conjugate gradient, momentum by BiCG
+    dict(_CAV3D, name="cav3d_diag_mp", general={"preconditioner": "DIAG"},
+         piso={"momentum_predictor": "YES"}),
+    # no preconditioner, BiCG momentum
+    dict(_CAV3D, name="cav3d_none_mp", general={"preconditioner": "NONE"},
+         piso={"momentum_predictor": "YES"}),
+    # SOR and Jacobi iterations for both equations
+    dict(_CAV3D, name="cav3d_sor_mp", method="SOR", piso={"momentum_predictor": "YES"}),
+    dict(_CAV3D, name="cav3d_jac_mp", method="JAC", piso={"momentum_predictor": "YES"}),
+    # steady state: under-relaxation (Relax) instead of the time derivative
+    dict(_CAV3D, name="cav3d_steady", state="STEADY", check_steps=[1, 4]),
+    # the reference's 2D cavity (the y faces deleted), 20 x 20 in x-z
+    dict(name="cav2d_20", ndiv=(20, 1, 20), size=(1, 0.1, 1), patches=PISO_PATCHES_2D, check_steps=[1, 5],
+         keep_fields=True, p0=101325.0, tol=1e-5),
+    # the same with p from 0 and a tight tolerance
+    dict(name="cav2d_20_tight", ndiv=(20, 1, 20), size=(1, 0.1, 1), patches=PISO_PATCHES_2D, check_steps=[1, 5],
+         tol=1e-8),
 ]

@@ -27,8 +27,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from _cases import (AMR_CASES, EULER_CASES, GEOM_CASES, POTENTIAL_CASES, write_euler_case,  # noqa: E402
-                    write_potential_case)
+from _cases import (AMR_CASES, EULER_CASES, GEOM_CASES, PISO_CASES, POTENTIAL_CASES,  # noqa: E402
+                    POTENTIAL_SKEW_CASES, write_euler_case, write_piso_case, write_potential_case)
 from _fieldio import read_field_bin  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
@@ -227,7 +227,7 @@ def main():
             rec = euler_record(case, d, arrays)
             fixtures["euler"][case["name"]] = rec
             print("amr", case["name"], passes, rec["log_tail"][-1])
-        for case in pick(POTENTIAL_CASES):
+        for case in pick(POTENTIAL_CASES + POTENTIAL_SKEW_CASES):
             d = os.path.join(tmp, "p_" + case["name"])
             write_potential_case(d, case)
             run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
@@ -247,6 +247,23 @@ def main():
                 arrays[f"potential_{case['name']}_p"] = p
                 arrays[f"potential_{case['name']}_U"] = u
             print("potential", case["name"], it, res)
+        for case in pick(PISO_CASES):
+            d = os.path.join(tmp, "s_" + case["name"])
+            write_piso_case(d, case)
+            run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)
+            log = run([os.path.join(REF, "piso"), "controls"], d)
+            rec = {"steps": {}, "solver_lines": [ln.split("]", 1)[-1].strip() for ln in log.splitlines()
+                                                 if "Iterations" in ln]}
+            for s in case["check_steps"]:
+                rec["steps"][str(s)] = {}
+                for name in ("U", "p"):
+                    v = read_field_bin(os.path.join(d, f"{name}{s}.bin")).reshape(-1)
+                    rec["steps"][str(s)][name] = {"sha256": sha(v), "l2": float(np.linalg.norm(v)),
+                                                  "max": float(np.max(np.abs(v))), "n": int(v.size)}
+                    if case.get("keep_fields"):
+                        arrays[f"piso_{case['name']}_{name}{s}"] = v
+            fixtures.setdefault("piso", {})[case["name"]] = rec
+            print("piso", case["name"], len(rec["solver_lines"]), "solves")
     finally:
         shutil.rmtree(tmp)
     with open(os.path.join(HERE, "reference_fixtures.json"), "w") as f:
