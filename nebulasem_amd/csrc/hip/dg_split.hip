@@ -28,7 +28,8 @@ namespace nsem {
 
 // ---------------------------------------------------------------------------
 template <int NXc, int NYc, int NZc>
-__global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out) {
+__global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out,
+                                                       double* __restrict__ p_out) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int epb = blockDim.x / NP;
@@ -108,7 +109,11 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     double r[1] = {surf};
     volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
     const double ap = (-1.0 / d.dt) * d.cV[n];
-    rho_out[n] = (r[0] + rho * ap) / ap;
+    const double rho1 = (r[0] + rho * ap) / ap;
+    rho_out[n] = rho1;
+    // p = P0 (rho T R / P0)^gamma with the new rho, then p - p_ref
+    // (euler.cpp:210-212), once per node for the momentum and theta pass
+    p_out[n] = d.P0 * dd::pow_dd((rho1 * th * d.R) / d.P0, d.gamma) - d.p_ref[n];
 }
 
 // ---------------------------------------------------------------------------
@@ -265,7 +270,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
         th = s.T[n] + d.T0;
-        pp = d.P0 * dd::pow_dd((rho * th * R) / d.P0, gam) - d.p_ref[n];
+        pp = o.p[n];
         const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
@@ -307,7 +312,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
                 orho0 = s.rho[m]; orho = rho1[m];
                 ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
                 oth = s.T[m] + d.T0;
-                opp = d.P0 * dd::pow_dd((orho * oth * R) / d.P0, gam) - d.p_ref[m];
+                opp = o.p[m];
                 if constexpr (VISC)
                     for (int c = 0; c < 12; c++) go[c] = d.grad[(size_t)c * d.nb + m];
             } else {
@@ -401,7 +406,6 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     }
     const double thn = (res[3] + (th * rho0) * ap0) / ap;
     o.T[n] = thn - d.T0;
-    o.p[n] = pp;
     // the new rho (this pass's input) and U, theta: any NaN or Inf raises
     // the problem's flag, which the step's caller reads
     if (!(isfinite(rho) && isfinite(un[0]) && isfinite(un[1]) && isfinite(un[2]) && isfinite(thn)))
@@ -491,7 +495,8 @@ __global__ void __launch_bounds__(256) split_mortar_rho_kernel(DgDev d, SplitSta
 // F_theta = (rho^n U) theta, q_theta = rho^{n+1} theta; lambda of the first
 // pass
 template <bool VISC>
-__global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitState s, const double* rho1) {
+__global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitState s, const double* rho1,
+                                                              const double* p1) {
     const uint32_t f = blockIdx.x;
     const int t = threadIdx.x;
     const MortarFace mf = d.mort[f];
@@ -505,7 +510,7 @@ __global__ void __launch_bounds__(256) split_mortar_ut_kernel(DgDev d, SplitStat
         const double rho0 = s.rho[m], rho = rho1[m];
         const double u[3] = {s.ux[m], s.uy[m], s.uz[m]};
         const double th = s.T[m] + d.T0;
-        const double pp = d.P0 * dd::pow_dd((rho * th * R) / d.P0, gam) - d.p_ref[m];
+        const double pp = p1[m];
         const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
         for (int row = 0; row < 3; row++)
             for (int c = 0; c < 3; c++) v[3 * row + c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
@@ -764,15 +769,17 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
         if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<false>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
-    hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1);
+    hipLaunchKernelGGL((split_rho_kernel<X, Y, Z>), dim3(blocks), dim3(threads), lds_a, st, d, s, rho1, o.p);
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (d.nmort) {
         if (d.diffusion)
             hipLaunchKernelGGL(split_mortar_ut_kernel<true>, dim3(d.nmort), dim3(mthreads),
-                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
+                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1,
+                               (const double*)o.p);
         else
             hipLaunchKernelGGL(split_mortar_ut_kernel<false>, dim3(d.nmort), dim3(mthreads),
-                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1);
+                               (size_t)16 * d.npfm * sizeof(double), st, d, s, (const double*)rho1,
+                               (const double*)o.p);
         if (d.ncslot) hipLaunchKernelGGL(split_mortar_gather_kernel<true>, dim3(d.ncslot), dim3(mthreads), 0, st, d);
         if ((err = hipGetLastError()) != hipSuccess) return err;
     }
