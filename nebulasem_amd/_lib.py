@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnebulasem_amd.so")
+# NEBULASEM_AMD_LIB selects another build of the same library (the host
+# sanitizer build, nebulasem_amd/build.py: build_sanitized)
+LIB_PATH = os.environ.get("NEBULASEM_AMD_LIB", os.path.join(_HERE, "libnebulasem_amd.so"))
 
 _lib = None
 

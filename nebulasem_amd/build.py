@@ -62,6 +62,38 @@ def build(verbose=False):
     return LIB
 
 
+ASAN_LIB = os.path.join(HERE, "_obj_asan", "libnebulasem_amd_asan.so")
+
+
+def build_sanitized():
+    """The C-ABI library with its host C++ (mesh, geometry, grid and field
+    files, the C-ABI glue) compiled with -fsanitize=address,undefined by
+    g++; the HIP sources are compiled as usual (no device sanitizer) and
+    linked in.  For the CPU test suite run under the sanitizers:
+
+        NEBULASEM_AMD_LIB=<this library> LD_PRELOAD="$(g++ -print-file-name=libasan.so)
+        $(g++ -print-file-name=libubsan.so)" ASAN_OPTIONS=detect_leaks=0 python -m pytest tests -m "not gpu"
+    """
+    out = os.path.dirname(ASAN_LIB)
+    os.makedirs(out, exist_ok=True)
+    host, hip = _sources()
+    san = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined", "-g"]
+    objs = []
+    jobs = []
+    for s in host:
+        o = os.path.join(out, os.path.relpath(s, CSRC).replace(os.sep, "_") + ".o")
+        jobs.append(["g++"] + HOST_FLAGS + san + ["-c", s, "-o", o])
+        objs.append(o)
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        list(ex.map(_run, jobs))
+    missing = [s for s in hip if not os.path.exists(_obj(s))]
+    if missing:
+        build()
+    _run(["g++", "-shared", "-o", ASAN_LIB] + san + objs + [_obj(s) for s in hip]
+         + ["-L/opt/rocm/lib", "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
+    return ASAN_LIB
+
+
 FIELD = os.path.join(HERE, "field")
 FIELD_LIB = os.path.join(HERE, "libnebulasem_field.so")
 APPS = os.path.join(FIELD, "_apps")
