@@ -590,6 +590,10 @@ def main():
     ap.add_argument("--no-fv", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-amr", action="store_true")
+    # for profiling one leg at a time: skip the 100-step FP32-vs-FP64 error
+    # runs and the reference-order split step
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--fv-n", type=int, default=256)
     ap.add_argument("--copy-only", type=int, default=0, metavar="MIB",
                     help="only time the plain copy kernel over MIB MiB (counter calibration)")
@@ -628,16 +632,20 @@ def main():
         xfer.setdefault("get_conserved_s", time.time() - t0)
         res[prec] = {"seconds": t, "rate": NQ * nb * 3 * K / t, "stage_s": t / (3 * K), "q": q}
         log(f"rk3 fp{prec}: {t * 1e3:.1f} ms for {K} steps, {res[prec]['rate'] / 1e9:.2f} G DOF*stage/s")
-    err = fp32_error(dg, (rho, U, T, p), 100)
-    log("fp32 vs fp64 after 100 steps (rel L2): " +
-        ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in err.items() if isinstance(v, dict)))
+    err = None
+    if not args.no_error:
+        err = fp32_error(dg, (rho, U, T, p), 100)
+        log("fp32 vs fp64 after 100 steps (rel L2): " +
+            ", ".join(f"{k} {v['rel_l2']:.2e}" for k, v in err.items() if isinstance(v, dict)))
     # the reference's split forward-Euler step (1 stage/step)
-    dg.set_state(rho, U, T, p)
-    ks = max(5, K // 5)
     affine = dg.affine()
-    ts = dg.time_split(ks)
-    split_rate = NQ * nb * ks / ts
-    log(f"split step fp64: {ts / ks * 1e3:.2f} ms/step, {split_rate / 1e9:.2f} G DOF*stage/s")
+    ks = max(5, K // 5)
+    ts = split_rate = None
+    if not args.no_split:
+        dg.set_state(rho, U, T, p)
+        ts = dg.time_split(ks)
+        split_rate = NQ * nb * ks / ts
+        log(f"split step fp64: {ts / ks * 1e3:.2f} ms/step, {split_rate / 1e9:.2f} G DOF*stage/s")
     dg.close()
     del res[64]["q"], res[32]["q"]
 
@@ -706,7 +714,8 @@ def main():
         "fp64": {"value": res[64]["rate"], "ms_per_step": res[64]["seconds"] / K * 1e3},
         "fp32_over_fp64": res[32]["rate"] / res[64]["rate"],
         "fp32_vs_fp64_error": err,
-        "split_reference_step_fp64": {"value": split_rate, "ms_per_step": ts / ks * 1e3},
+        "split_reference_step_fp64": ({"value": split_rate, "ms_per_step": ts / ks * 1e3}
+                                      if ts else None),
         # host <-> device state transfers through the C-ABI (host arrays,
         # pageable memory, layout conversion included), once per run
         "host_transfer": dict(xfer, state_bytes=6 * 8 * nb,
