@@ -27,18 +27,90 @@
 namespace nsem {
 
 // ---------------------------------------------------------------------------
+// LDS of the density and momentum/theta kernels: per element FR flux rows
+// (3 x NP each), then per element Jin (9 x NP), then -- unless they fit in
+// the flux rows, which are written only after the face terms are summed --
+// the signed face fluxes of the face tasks (SFR values per task).
+__host__ __device__ __forceinline__ int split_nfn(int NX, int NY, int NZ) {
+    return 2 * (NX * NY + NX * NZ + NY * NZ);
+}
+__host__ __device__ __forceinline__ bool split_sf_overlaid(int epb, int NP, int NFN, int FR, int SFR) {
+    return NFN * SFR <= FR * NP;
+}
+__host__ __device__ __forceinline__ size_t split_lds_doubles(int epb, int NP, int NFN, int FR, int SFR) {
+    return (size_t)epb * (FR + 9) * NP + (split_sf_overlaid(epb, NP, NFN, FR, SFR) ? 0 : (size_t)epb * NFN * SFR);
+}
+
+// Rusanov flux of the density pass at one face node (field.h:2927-2943 with
+// div_flux, field.h:3093-3114), signed as the node's element adds it: + on
+// the owner side, - on the neighbour side; 2:1 mortar sides take the signed
+// term of the pre-pass.  Own values are re-read from global memory (the
+// work-group has just loaded them).
+__device__ __forceinline__ bool split_rho_face(const DgDev& d, const SplitState& s, uint32_t e, int g, int a, int b,
+                                               int node, int NX, int NY, int NZ, double& out) {
+    const EFace ef = d.eface[(size_t)e * 6 + g];
+    if (!((ef.meta >> 4) & 1)) return false;
+    if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed term from the pre-pass
+        out = d.msurf[((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5];
+        return true;
+    }
+    const uint64_t n = (uint64_t)e * (NX * NY * NZ) + node;
+    const double rho = s.rho[n], u0 = s.ux[n], u1 = s.uy[n], u2 = s.uz[n];
+    const double th = s.T[n] + d.T0;
+    const double gRT = d.gamma * d.R;
+    const bool own = (ef.meta >> 3) & 1;
+    const bool ghost = (ef.meta >> 5) & 1;
+    const int og = ef.meta & 7;
+    const double wgt = face_weight(d, own ? g : og, a, b);
+    const double nx = d.gFN[3 * (size_t)ef.facet] * wgt;
+    const double ny = d.gFN[3 * (size_t)ef.facet + 1] * wgt;
+    const double nz = d.gFN[3 * (size_t)ef.facet + 2] * wgt;
+    const double nv[3] = {nx, ny, nz};
+    double orho, ou[3], oth;
+    if (!ghost) {
+        const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+        orho = s.rho[m]; ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+        oth = s.T[m] + d.T0;
+    } else {
+        // scalars: zero-gradient copies; velocity by its own condition
+        const double uu[3] = {u0, u1, u2};
+        ghost_velocity(*d.bc, ef.meta >> 8, uu, nv, ou);
+        orho = rho;
+        oth = th;
+    }
+    const double rO = own ? rho : orho, rN = own ? orho : rho;
+    const double uO[3] = {own ? u0 : ou[0], own ? u1 : ou[1], own ? u2 : ou[2]};
+    const double uN[3] = {own ? ou[0] : u0, own ? ou[1] : u1, own ? ou[2] : u2};
+    const double thO = own ? th : oth, thN = own ? oth : th;
+    const double fI = ghost ? 0.0 : 0.5;
+    const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
+    const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
+    const double cO = sqrt(gRT * thO), cN = sqrt(gRT * thN);
+    const double lam = ((magO * fI + magN * (1 - fI)) + (cO * fI + cN * (1 - fI))) / 2;
+    const double mn = sqrt(dot3r(nx, ny, nz, nx, ny, nz));
+    const double en[3] = {nx / mn, ny / mn, nz / mn};
+    const double dq = lam * (rN - rO);
+    double fF[3];
+    for (int c = 0; c < 3; c++) fF[c] = (uO[c] * rO) * fI + (uN[c] * rN) * (1 - fI) - en[c] * dq;
+    const double flux = dot3r(fF[0], fF[1], fF[2], nx, ny, nz);
+    out = own ? flux : -flux;
+    return true;
+}
+
 template <int NXc, int NYc, int NZc>
 __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, double* __restrict__ rho_out,
                                                        double* __restrict__ p_out) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
+    const int NFN = split_nfn(NX, NY, NZ);
     const int epb = blockDim.x / NP;
     const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds[];
-    double* F = lds + (size_t)le * 12 * NP;  // flux U rho (3), then Jin (9)
-    double* J = F + 3 * NP;
+    double* F = lds + (size_t)le * 3 * NP;  // flux U rho (3 rows)
+    double* J = lds + (size_t)epb * 3 * NP + (size_t)le * 9 * NP;
+    double* SF = split_sf_overlaid(epb, NP, NFN, 3, 1) ? lds : lds + (size_t)epb * 12 * NP;
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
@@ -46,66 +118,40 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     if (active) {
         rho = s.rho[n]; u0 = s.ux[n]; u1 = s.uy[n]; u2 = s.uz[n];
         th = s.T[n] + d.T0;
+        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
+    }
+    // surface: one thread per face node of the work-group's elements (for
+    // N = 7 one face per wavefront, so the face branch is uniform), each
+    // node then adds the faces through it in the cell's face order
+    // (div_flux, field.h:3093-3114)
+    for (int t = threadIdx.x; t < epb * NFN; t += blockDim.x) {
+        const int fl = t / NFN, task = t - fl * NFN;
+        const uint32_t fe = blockIdx.x * epb + fl;
+        if (fe >= d.nelem) break;
+        int g, fa, fb, node;
+        face_task(task, NX, NY, NZ, g, fa, fb, node);
+        double v;
+        if (split_rho_face(d, s, fe, g, fa, fb, node, NX, NY, NZ, v)) SF[t] = v;
+    }
+    __syncthreads();
+    double surf = 0;
+    if (active)
+        for (int g = 0; g < 6; g++) {
+            if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
+            const EFace ef = d.eface[(size_t)e * 6 + g];
+            if (!((ef.meta >> 4) & 1)) continue;
+            int a, b;
+            face_ab(g, i, j, k, a, b);
+            surf += SF[le * NFN + face_task_index(g, a, b, NX, NY, NZ)];
+        }
+    __syncthreads();
+    if (active) {
         F[0 * NP + q] = u0 * rho;
         F[1 * NP + q] = u1 * rho;
         F[2 * NP + q] = u2 * rho;
-        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
     }
     __syncthreads();
     if (!active) return;
-
-    // surface: Rusanov flux of the faces through this node, in the cell's
-    // face order (div_flux, field.h:3093-3114)
-    double surf = 0;
-    const double gRT = d.gamma * d.R;
-    for (int g = 0; g < 6; g++) {
-        if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
-        const EFace ef = d.eface[(size_t)e * 6 + g];
-        if (!((ef.meta >> 4) & 1)) continue;
-        const bool own = (ef.meta >> 3) & 1;
-        const bool ghost = (ef.meta >> 5) & 1;
-        const int og = ef.meta & 7;
-        int a, b;
-        face_ab(g, i, j, k, a, b);
-        if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed term from the pre-pass
-            surf += d.msurf[((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5];
-            continue;
-        }
-        const double wgt = face_weight(d, own ? g : og, a, b);
-        const double nx = d.gFN[3 * (size_t)ef.facet] * wgt;
-        const double ny = d.gFN[3 * (size_t)ef.facet + 1] * wgt;
-        const double nz = d.gFN[3 * (size_t)ef.facet + 2] * wgt;
-        const double nv[3] = {nx, ny, nz};
-        double orho, ou[3], oth;
-        if (!ghost) {
-            const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
-            orho = s.rho[m]; ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
-            oth = s.T[m] + d.T0;
-        } else {
-            // scalars: zero-gradient copies; velocity by its own condition
-            const double uu[3] = {u0, u1, u2};
-            ghost_velocity(*d.bc, ef.meta >> 8, uu, nv, ou);
-            orho = rho;
-            oth = th;
-        }
-        const double rO = own ? rho : orho, rN = own ? orho : rho;
-        const double uO[3] = {own ? u0 : ou[0], own ? u1 : ou[1], own ? u2 : ou[2]};
-        const double uN[3] = {own ? ou[0] : u0, own ? ou[1] : u1, own ? ou[2] : u2};
-        const double thO = own ? th : oth, thN = own ? oth : th;
-        const double fI = ghost ? 0.0 : 0.5;
-        const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
-        const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
-        const double cO = sqrt(gRT * thO), cN = sqrt(gRT * thN);
-        const double lam = ((magO * fI + magN * (1 - fI)) + (cO * fI + cN * (1 - fI))) / 2;
-        const double mn = sqrt(dot3r(nx, ny, nz, nx, ny, nz));
-        const double en[3] = {nx / mn, ny / mn, nz / mn};
-        const double dq = lam * (rN - rO);
-        double fF[3];
-        for (int c = 0; c < 3; c++) fF[c] = (uO[c] * rO) * fI + (uN[c] * rN) * (1 - fI) - en[c] * dq;
-        const double flux = dot3r(fF[0], fF[1], fF[2], nx, ny, nz);
-        if (own) surf += flux;
-        else surf -= flux;
-    }
     double r[1] = {surf};
     volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
     const double ap = (-1.0 / d.dt) * d.cV[n];
@@ -247,25 +293,129 @@ __device__ __forceinline__ void ghost_gradient(const BcTable& bt, int pi, const 
 }
 
 // ---------------------------------------------------------------------------
+// Rusanov fluxes of the momentum and theta pass at one face node, signed as
+// the node's element adds them (out[0..2] momentum rows, out[3] theta); 2:1
+// mortar sides take the signed terms of the pre-pass.
+template <bool VISC>
+__device__ __forceinline__ bool split_ut_face(const DgDev& d, const SplitState& s, const double* __restrict__ rho1,
+                                              const double* __restrict__ p1, uint32_t e, int g, int a, int b,
+                                              int node, int NX, int NY, int NZ, double* out) {
+    const EFace ef = d.eface[(size_t)e * 6 + g];
+    if (!((ef.meta >> 4) & 1)) return false;
+    if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed terms from the pre-pass
+        const double* ms = d.msurf + ((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5;
+        for (int r = 0; r < 4; r++) out[r] = ms[1 + r];
+        return true;
+    }
+    const uint64_t n = (uint64_t)e * (NX * NY * NZ) + node;
+    const double rho0 = s.rho[n], rho = rho1[n];
+    const double u[3] = {s.ux[n], s.uy[n], s.uz[n]};
+    const double th = s.T[n] + d.T0;
+    const double pp = p1[n];
+    double gown[VISC ? 12 : 1];
+    if constexpr (VISC)
+        for (int c = 0; c < 12; c++) gown[c] = d.grad[(size_t)c * d.nb + n];
+    const double gRT = d.gamma * d.R;
+    const bool own = (ef.meta >> 3) & 1;
+    const bool ghost = (ef.meta >> 5) & 1;
+    const int og = ef.meta & 7;
+    const double wgt = face_weight(d, own ? g : og, a, b);
+    const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                          d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+    double orho0, orho, ou[3], oth, opp;
+    double go[VISC ? 12 : 1];
+    if (!ghost) {
+        const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
+        orho0 = s.rho[m]; orho = rho1[m];
+        ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
+        oth = s.T[m] + d.T0;
+        opp = p1[m];
+        if constexpr (VISC)
+            for (int c = 0; c < 12; c++) go[c] = d.grad[(size_t)c * d.nb + m];
+    } else {
+        ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
+        orho0 = rho0;
+        orho = rho;
+        oth = th;
+        opp = pp;
+        if constexpr (VISC) ghost_gradient(*d.bc, ef.meta >> 8, gown, go);
+    }
+    const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
+    const double rO = own ? rho : orho, rN = own ? orho : rho;
+    double uO[3], uN[3];
+    for (int c = 0; c < 3; c++) { uO[c] = own ? u[c] : ou[c]; uN[c] = own ? ou[c] : u[c]; }
+    const double thO = own ? th : oth, thN = own ? oth : th;
+    const double pO = own ? pp : opp, pN = own ? opp : pp;
+    const double fI = ghost ? 0.0 : 0.5;
+    const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
+    const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
+    const double lam = ((magO * fI + magN * (1 - fI)) +
+                        (sqrt(gRT * thO) * fI + sqrt(gRT * thN) * (1 - fI))) / 2;
+    const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
+    const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
+    const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
+    const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
+    // viscous parts: mu = rho^n visc on each side, gradients G
+    double muO = 0, muN = 0;
+    const double* gO = nullptr;
+    const double* gN = nullptr;
+    if constexpr (VISC) {
+        muO = r0O * d.visc;
+        muN = r0N * d.visc;
+        gO = own ? gown : go;
+        gN = own ? go : gown;
+    }
+    for (int row = 0; row < 3; row++) {
+        double fF[3];
+        for (int c = 0; c < 3; c++) {
+            double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
+            double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
+            if constexpr (VISC) {
+                FO = FO - muO * gO[3 * row + c];
+                FN = FN - muN * gN[3 * row + c];
+            }
+            const double dq = lam * (rN * uN[c] - rO * uO[c]);
+            fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
+        }
+        const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
+        out[row] = own ? flux : -flux;
+    }
+    {
+        const double dq = lam * (rN * thN - rO * thO);
+        double fF[3];
+        for (int c = 0; c < 3; c++) {
+            double FO = fcO[c] * thO, FN = fcN[c] * thN;
+            if constexpr (VISC) {
+                FO = FO - (muO * d.iPr) * gO[9 + c];
+                FN = FN - (muN * d.iPr) * gN[9 + c];
+            }
+            fF[c] = FO * fI + FN * (1 - fI) - en[c] * dq;
+        }
+        const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
+        out[3] = own ? flux : -flux;
+    }
+    return true;
+}
+
 template <int NXc, int NYc, int NZc, bool VISC>
 __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, const double* __restrict__ rho1,
                                                       SplitOut o) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
+    const int NFN = split_nfn(NX, NY, NZ);
     const int epb = blockDim.x / NP;
     const int le = threadIdx.x / NP, q = threadIdx.x - le * NP;
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds[];
-    double* F = lds + (size_t)le * 15 * NP;  // two flux rows (6), then Jin (9)
-    double* J = F + 6 * NP;
+    double* F = lds + (size_t)le * 6 * NP;  // two flux rows at a time
+    double* J = lds + (size_t)epb * 6 * NP + (size_t)le * 9 * NP;
+    double* SF = split_sf_overlaid(epb, NP, NFN, 6, 4) ? lds : lds + (size_t)epb * 15 * NP;
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
-    const double R = d.R, gam = d.gamma;
 
     double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
     double fr[4][3];  // own flux rows: U_x U_y U_z (tensor rows), theta (vector)
-    double gown[VISC ? 12 : 1];
     if (active) {
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
@@ -276,6 +426,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
             for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
         for (int c = 0; c < 3; c++) fr[3][c] = fc[c] * th;
         if constexpr (VISC) {
+            double gown[12];
             for (int c = 0; c < 12; c++) gown[c] = d.grad[(size_t)c * d.nb + n];
             const double mu = rho0 * d.visc, imu = mu * d.iPr;
             for (int row = 0; row < 3; row++)
@@ -284,103 +435,31 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
         }
         for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
     }
-
+    // surface: face tasks (one face per wavefront for N = 7), then each node
+    // adds the faces through it in the cell's face order
+    for (int t = threadIdx.x; t < epb * NFN; t += blockDim.x) {
+        const int fl = t / NFN, task = t - fl * NFN;
+        const uint32_t fe = blockIdx.x * epb + fl;
+        if (fe >= d.nelem) break;
+        int g, fa, fb, node;
+        face_task(task, NX, NY, NZ, g, fa, fb, node);
+        double v[4];
+        if (split_ut_face<VISC>(d, s, rho1, o.p, fe, g, fa, fb, node, NX, NY, NZ, v))
+            for (int r = 0; r < 4; r++) SF[4 * t + r] = v[r];
+    }
+    __syncthreads();
     double res[4] = {0, 0, 0, 0};
-    if (active) {
-        const double gRT = gam * R;
+    if (active)
         for (int g = 0; g < 6; g++) {
             if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
             const EFace ef = d.eface[(size_t)e * 6 + g];
             if (!((ef.meta >> 4) & 1)) continue;
-            const bool own = (ef.meta >> 3) & 1;
-            const bool ghost = (ef.meta >> 5) & 1;
-            const int og = ef.meta & 7;
             int a, b;
             face_ab(g, i, j, k, a, b);
-            if ((ef.meta >> 6) & 1) {  // 2:1 mortar side: signed terms from the pre-pass
-                const double* ms = d.msurf + ((size_t)(ef.meta >> 8) * d.npfm + a * (g < 2 ? NY : NZ) + b) * 5;
-                for (int r = 0; r < 4; r++) res[r] += ms[1 + r];
-                continue;
-            }
-            const double wgt = face_weight(d, own ? g : og, a, b);
-            const double nv[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
-                                  d.gFN[3 * (size_t)ef.facet + 2] * wgt};
-            double orho0, orho, ou[3], oth, opp;
-            double go[VISC ? 12 : 1];
-            if (!ghost) {
-                const uint64_t m = face_node(ef.other, og, a, b, NX, NY, NZ);
-                orho0 = s.rho[m]; orho = rho1[m];
-                ou[0] = s.ux[m]; ou[1] = s.uy[m]; ou[2] = s.uz[m];
-                oth = s.T[m] + d.T0;
-                opp = o.p[m];
-                if constexpr (VISC)
-                    for (int c = 0; c < 12; c++) go[c] = d.grad[(size_t)c * d.nb + m];
-            } else {
-                ghost_velocity(*d.bc, ef.meta >> 8, u, nv, ou);
-                orho0 = rho0;
-                orho = rho;
-                oth = th;
-                opp = pp;
-                if constexpr (VISC) ghost_gradient(*d.bc, ef.meta >> 8, gown, go);
-            }
-            const double r0O = own ? rho0 : orho0, r0N = own ? orho0 : rho0;
-            const double rO = own ? rho : orho, rN = own ? orho : rho;
-            double uO[3], uN[3];
-            for (int c = 0; c < 3; c++) { uO[c] = own ? u[c] : ou[c]; uN[c] = own ? ou[c] : u[c]; }
-            const double thO = own ? th : oth, thN = own ? oth : th;
-            const double pO = own ? pp : opp, pN = own ? opp : pp;
-            const double fI = ghost ? 0.0 : 0.5;
-            const double magO = sqrt(dot3r(uO[0], uO[1], uO[2], uO[0], uO[1], uO[2]));
-            const double magN = sqrt(dot3r(uN[0], uN[1], uN[2], uN[0], uN[1], uN[2]));
-            const double lam = ((magO * fI + magN * (1 - fI)) +
-                                (sqrt(gRT * thO) * fI + sqrt(gRT * thN) * (1 - fI))) / 2;
-            const double mn = sqrt(dot3r(nv[0], nv[1], nv[2], nv[0], nv[1], nv[2]));
-            const double en[3] = {nv[0] / mn, nv[1] / mn, nv[2] / mn};
-            const double fcO[3] = {r0O * uO[0], r0O * uO[1], r0O * uO[2]};
-            const double fcN[3] = {r0N * uN[0], r0N * uN[1], r0N * uN[2]};
-            // viscous parts: mu = rho^n visc on each side, gradients G
-            double muO = 0, muN = 0;
-            const double* gO = nullptr;
-            const double* gN = nullptr;
-            if constexpr (VISC) {
-                muO = r0O * d.visc;
-                muN = r0N * d.visc;
-                gO = own ? gown : go;
-                gN = own ? go : gown;
-            }
-            for (int row = 0; row < 3; row++) {
-                double fF[3];
-                for (int c = 0; c < 3; c++) {
-                    double FO = fcO[row] * uO[c] + (row == c ? pO : 0.0 * pO);
-                    double FN = fcN[row] * uN[c] + (row == c ? pN : 0.0 * pN);
-                    if constexpr (VISC) {
-                        FO = FO - muO * gO[3 * row + c];
-                        FN = FN - muN * gN[3 * row + c];
-                    }
-                    const double dq = lam * (rN * uN[c] - rO * uO[c]);
-                    fF[c] = FO * fI + FN * (1 - fI) - en[row] * dq;
-                }
-                const double flux = fF[0] * nv[0] + fF[1] * nv[1] + fF[2] * nv[2];
-                if (own) res[row] += flux;
-                else res[row] -= flux;
-            }
-            {
-                const double dq = lam * (rN * thN - rO * thO);
-                double fF[3];
-                for (int c = 0; c < 3; c++) {
-                    double FO = fcO[c] * thO, FN = fcN[c] * thN;
-                    if constexpr (VISC) {
-                        FO = FO - (muO * d.iPr) * gO[9 + c];
-                        FN = FN - (muN * d.iPr) * gN[9 + c];
-                    }
-                    fF[c] = FO * fI + FN * (1 - fI) - en[c] * dq;
-                }
-                const double flux = dot3r(fF[0], fF[1], fF[2], nv[0], nv[1], nv[2]);
-                if (own) res[3] += flux;
-                else res[3] -= flux;
-            }
+            const double* sf = SF + 4 * (le * NFN + face_task_index(g, a, b, NX, NY, NZ));
+            for (int r = 0; r < 4; r++) res[r] += sf[r];
         }
-    }
+    __syncthreads();
     // volume terms in two passes of two flux rows through LDS
     if (active)
         for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[0][c]; F[(3 + c) * NP + q] = fr[1][c]; }
@@ -744,8 +823,10 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
     int threads = epb * NP;
     threads = (threads + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
-    size_t lds_a = (size_t)epb * 12 * NP * sizeof(double);
-    size_t lds_b = (size_t)epb * 15 * NP * sizeof(double);
+    const int NFN = split_nfn(d.npx, d.npy, d.npz);
+    size_t lds_a = split_lds_doubles(epb, NP, NFN, 3, 1) * sizeof(double);
+    size_t lds_b = split_lds_doubles(epb, NP, NFN, 6, 4) * sizeof(double);
+    if (lds_a > 65536 || lds_b > 65536) return hipErrorInvalidValue;
     size_t lds_g = (size_t)epb * 4 * NP * sizeof(double);
     hipError_t err;
     const int mthreads = (d.npfm + 63) / 64 * 64;

@@ -39,17 +39,31 @@ __device__ __forceinline__ void volume_reference_order(double* r, const double* 
     const int syz = NY * NZ;
     const int t = i * syz + j * NZ + k;
     for (int a = 0; a < i; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
-    for (int b = 0; b < j; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
-    for (int c = 0; c < k; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
     {
         const double d0 = D0[i * NX + i], d1 = D1[j * NY + j], d2 = D2[k * NZ + k];
-        const double w0 = J[0 * NP + t] * d0 + J[3 * NP + t] * d1 + J[5 * NP + t] * d2;
-        const double w1 = J[6 * NP + t] * d0 + J[1 * NP + t] * d1 + J[4 * NP + t] * d2;
-        const double w2 = J[8 * NP + t] * d0 + J[7 * NP + t] * d1 + J[2 * NP + t] * d2;
-        sub(t, w0, w1, w2);
+        const double s0 = J[0 * NP + t] * d0 + J[3 * NP + t] * d1 + J[5 * NP + t] * d2;
+        const double s1 = J[6 * NP + t] * d0 + J[1 * NP + t] * d1 + J[4 * NP + t] * d2;
+        const double s2 = J[8 * NP + t] * d0 + J[7 * NP + t] * d1 + J[2 * NP + t] * d2;
+        // y-line b < j, z-line c < k, the node itself, z-line c > k, y-line
+        // b > j: NY + NZ - 1 sources for every node, walked by one loop whose
+        // trip count does not depend on (j, k), so the lanes of a wavefront
+        // stay converged; each step selects its source, column and
+        // derivative value (at the node itself: c = k, the products above)
+        const int base = i * syz;
+#pragma unroll 1
+        for (int m = 0; m < NY + NZ - 1; m++) {
+            const bool ylo = m < j;
+            const bool isy = ylo || m >= j + NZ;
+            const int b = ylo ? m : m - NZ + 1;
+            const int c = m - j;
+            const int src = isy ? base + b * NZ + k : base + j * NZ + c;
+            const double D = isy ? D1[b * NY + j] : D2[c * NZ + k];
+            const int c0 = isy ? 3 : 5, c1 = isy ? 1 : 4, c2 = isy ? 7 : 2;
+            double w0 = J[c0 * NP + src] * D, w1 = J[c1 * NP + src] * D, w2 = J[c2 * NP + src] * D;
+            if (m == j + k) { w0 = s0; w1 = s1; w2 = s2; }
+            sub(src, w0, w1, w2);
+        }
     }
-    for (int c = k + 1; c < NZ; c++) col(i * syz + j * NZ + c, 2, D2[c * NZ + k]);
-    for (int b = j + 1; b < NY; b++) col(i * syz + b * NZ + k, 1, D1[b * NY + j]);
     for (int a = i + 1; a < NX; a++) col(a * syz + j * NZ + k, 0, D0[a * NX + i]);
 }
 
