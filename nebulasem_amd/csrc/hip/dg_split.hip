@@ -30,7 +30,8 @@ namespace nsem {
 // LDS of the density and momentum/theta kernels: per element FR flux rows
 // (3 x NP each), then per element Jin (9 x NP), then -- unless they fit in
 // the flux rows, which are written only after the face terms are summed --
-// the signed face fluxes of the face tasks (SFR values per task).
+// the signed face fluxes of the face tasks (SFR values per task), then the
+// three 1-D derivative matrices.
 __host__ __device__ __forceinline__ int split_nfn(int NX, int NY, int NZ) {
     return 2 * (NX * NY + NX * NZ + NY * NZ);
 }
@@ -39,6 +40,12 @@ __host__ __device__ __forceinline__ bool split_sf_overlaid(int epb, int NP, int 
 }
 __host__ __device__ __forceinline__ size_t split_lds_doubles(int epb, int NP, int NFN, int FR, int SFR) {
     return (size_t)epb * (FR + 9) * NP + (split_sf_overlaid(epb, NP, NFN, FR, SFR) ? 0 : (size_t)epb * NFN * SFR);
+}
+// Stage D0, D1, D2 (NX^2, NY^2, NZ^2 values) in LDS at Dl.
+__device__ __forceinline__ void split_stage_d(const DgDev& d, double* Dl, int NX, int NY, int NZ) {
+    const int n0 = NX * NX, n1 = NY * NY, n2 = NZ * NZ;
+    for (int t = threadIdx.x; t < n0 + n1 + n2; t += blockDim.x)
+        Dl[t] = t < n0 ? d.D[0][t] : (t < n0 + n1 ? d.D[1][t - n0] : d.D[2][t - n0 - n1]);
 }
 
 // Rusanov flux of the density pass at one face node (field.h:2927-2943 with
@@ -111,6 +118,8 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     double* F = lds + (size_t)le * 3 * NP;  // flux U rho (3 rows)
     double* J = lds + (size_t)epb * 3 * NP + (size_t)le * 9 * NP;
     double* SF = split_sf_overlaid(epb, NP, NFN, 3, 1) ? lds : lds + (size_t)epb * 12 * NP;
+    double* Dl = lds + split_lds_doubles(epb, NP, NFN, 3, 1);
+    split_stage_d(d, Dl, NX, NY, NZ);
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
@@ -118,7 +127,7 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     if (active) {
         rho = s.rho[n]; u0 = s.ux[n]; u1 = s.uy[n]; u2 = s.uz[n];
         th = s.T[n] + d.T0;
-        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
+        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)kJinRow[c] * d.nb + n];
     }
     // surface: one thread per face node of the work-group's elements (for
     // N = 7 one face per wavefront, so the face branch is uniform), each
@@ -153,7 +162,7 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     __syncthreads();
     if (!active) return;
     double r[1] = {surf};
-    volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
+    volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
     const double ap = (-1.0 / d.dt) * d.cV[n];
     const double rho1 = (r[0] + rho * ap) / ap;
     rho_out[n] = rho1;
@@ -411,6 +420,8 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     double* F = lds + (size_t)le * 6 * NP;  // two flux rows at a time
     double* J = lds + (size_t)epb * 6 * NP + (size_t)le * 9 * NP;
     double* SF = split_sf_overlaid(epb, NP, NFN, 6, 4) ? lds : lds + (size_t)epb * 15 * NP;
+    double* Dl = lds + split_lds_doubles(epb, NP, NFN, 6, 4);
+    split_stage_d(d, Dl, NX, NY, NZ);
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
@@ -433,7 +444,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
                 for (int c = 0; c < 3; c++) fr[row][c] = fr[row][c] - mu * gown[3 * row + c];
             for (int c = 0; c < 3; c++) fr[3][c] = fr[3][c] - imu * gown[9 + c];
         }
-        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)c * d.nb + n];
+        for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)kJinRow[c] * d.nb + n];
     }
     // surface: face tasks (one face per wavefront for N = 7), then each node
     // adds the faces through it in the cell's face order
@@ -465,13 +476,13 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
         for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[0][c]; F[(3 + c) * NP + q] = fr[1][c]; }
     __syncthreads();
     if (active)
-        volume_reference_order<2, 0>(res, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
+        volume_reference_order<2, 0>(res, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
     __syncthreads();
     if (active)
         for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[2][c]; F[(3 + c) * NP + q] = fr[3][c]; }
     __syncthreads();
     if (!active) return;
-    volume_reference_order<2, 2>(res + 2, F, J, NP, i, j, k, NX, NY, NZ, d.D[0], d.D[1], d.D[2]);
+    volume_reference_order<2, 2>(res + 2, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
 
     const double cv = d.cV[n];
     const double ap0 = (-1.0 / d.dt) * cv;
@@ -824,8 +835,9 @@ static hipError_t launch_split_t(const DgDev& d, const SplitState& s, double* rh
     threads = (threads + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
     const int NFN = split_nfn(d.npx, d.npy, d.npz);
-    size_t lds_a = split_lds_doubles(epb, NP, NFN, 3, 1) * sizeof(double);
-    size_t lds_b = split_lds_doubles(epb, NP, NFN, 6, 4) * sizeof(double);
+    const int ND = d.npx * d.npx + d.npy * d.npy + d.npz * d.npz;
+    size_t lds_a = (split_lds_doubles(epb, NP, NFN, 3, 1) + ND) * sizeof(double);
+    size_t lds_b = (split_lds_doubles(epb, NP, NFN, 6, 4) + ND) * sizeof(double);
     if (lds_a > 65536 || lds_b > 65536) return hipErrorInvalidValue;
     size_t lds_g = (size_t)epb * 4 * NP * sizeof(double);
     hipError_t err;

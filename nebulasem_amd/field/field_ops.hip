@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(512) divf_volume_kernel(DevTables t, const dou
     static constexpr int tmap[3][3] = {{0, 3, 5}, {6, 1, 4}, {8, 7, 2}};
     if (q < NP) {
         for (int c = 0; c < 3; c++) Fr[c * NP + q] = F[n * ncF + (ncF == 3 ? c : tmap[row][c])];
-        for (int c = 0; c < 9; c++) J[c * NP + q] = t.JcV[9 * n + c];
+        for (int c = 0; c < 9; c++) J[c * NP + q] = t.JcV[9 * n + nsem::kJinRow[c]];
     }
     __syncthreads();
     if (q >= NP) return;
