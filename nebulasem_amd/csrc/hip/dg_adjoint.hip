@@ -173,76 +173,41 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         const int s1 = NX * NX, s2 = s1 + NY * NY;
         Dl[t] = t < s1 ? d.D[0][t] : t < s2 ? d.D[1][t - s1] : d.D[2][t - s2];
     }
-    double Qn[5] = {0, 0, 0, 0, 0}, wn[5] = {0, 0, 0, 0, 0}, J[9];
-    double rref = 0, pref = 0;
+    // phase 1: w / cV and Q = q + din to LDS
     if (active) {
         const double cvn = d.cV[n];
         for (int v = 0; v < 5; v++) {
-            Qn[v] = a.q[v][n] + (DIN ? a.din[v][n] : 0.0);
-            wn[v] = a.w[v][n];
-            Wl[v * NP + q] = wn[v] / cvn;
-            Ql[v * NP + q] = Qn[v];
-        }
-        rref = d.rho_ref[n];
-        pref = d.p_ref[n];
-        if (d.affine) {
-            // the forward stage's per-element metric (JcV = Jinv_e (vol_e W))
-            const double* em = d.emet + (size_t)e * 10;
-            const double cv64 = em[9] * d.wnode[q];
-            for (int c = 0; c < 9; c++) J[c] = em[c] * cv64;
-        } else {
-            for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * nb + n];
+            Wl[v * NP + q] = a.w[v][n] / cvn;
+            Ql[v * NP + q] = a.q[v][n] + (DIN ? a.din[v][n] : 0.0);
         }
     }
     __syncthreads();
 
-    double qbar[5] = {0, 0, 0, 0, 0};
+    // phase 2: the face terms of this element's face nodes
     if (active) {
-        const double* D0 = Dl;
-        const double* Dy = Dl + NX * NX;
-        const double* Dz = Dy + NY * NY;
-        double fbar[5][3];
-        for (int row = 0; row < 5; row++) {
-            const double* W = Wl + row * NP;
-            double h0 = 0, h1 = 0, h2 = 0;
-            for (int t = 0; t < NX; t++) h0 += D0[i * NX + t] * W[t * NY * NZ + j * NZ + k];
-            for (int t = 0; t < NY; t++) h1 += Dy[j * NY + t] * W[i * NY * NZ + t * NZ + k];
-            for (int t = 0; t < NZ; t++) h2 += Dz[k * NZ + t] * W[i * NY * NZ + j * NZ + t];
-            fbar[row][0] = h0 * J[0] + h1 * J[3] + h2 * J[5];
-            fbar[row][1] = h0 * J[6] + h1 * J[1] + h2 * J[4];
-            fbar[row][2] = h0 * J[8] + h1 * J[7] + h2 * J[2];
-        }
-        {
-            Prim<double> P;
-            P.rp = Qn[0]; P.m[0] = Qn[1]; P.m[1] = Qn[2]; P.m[2] = Qn[3]; P.tp = Qn[4];
-            make_prim(P, rref, pref, T0, gam, gR);
-            flux_rows_vjp(P, rref, pref, T0, gam, fbar, qbar);
-        }
-        if (d.buoyancy) qbar[0] += wn[1] * d.g[0] + wn[2] * d.g[1] + wn[3] * d.g[2];
-
         for (int t = q; t < NFN; t += NP) {
             int g, fa, fb, fnode;
             face_task(t, NX, NY, NZ, g, fa, fb, fnode);
             const EFace ef = d.eface[(size_t)e * 6 + g];
             double contrib[5] = {0, 0, 0, 0, 0};
-            // 2:1 mortar sides: their share comes from mortar_vjp_kernel (madj)
+            // 2:1 mortar sides: their share comes from mortar_vjp_kernel
+            // (madj); ghost sides: from ghost_vjp_kernel (gadj)
             if (((ef.meta >> 4) & 1) && !((ef.meta >> 6) & 1)) {
                 const bool own = (ef.meta >> 3) & 1;
                 const bool ghost = (ef.meta >> 5) & 1;
-                const int og = ef.meta & 7;
-                const double wgt = face_weight(d, own ? g : og, fa, fb);
-                const uint64_t na = (uint64_t)e * NP + fnode;
-                const double rrA = d.rho_ref[na], prA = d.p_ref[na];
-                double QA[5], coef[5];
-                for (int v = 0; v < 5; v++) QA[v] = Ql[v * NP + fnode];
-                const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
-                                       d.gFN[3 * (size_t)ef.facet + 2] * wgt};
                 if (!ghost) {
                     // interior face: the hand-written Rusanov VJP
+                    const int og = ef.meta & 7;
+                    const double wgt = face_weight(d, own ? g : og, fa, fb);
+                    const uint64_t na = (uint64_t)e * NP + fnode;
+                    const double rrA = d.rho_ref[na], prA = d.p_ref[na];
+                    const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                                           d.gFN[3 * (size_t)ef.facet + 2] * wgt};
                     const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
                     const double cvm = d.cV[m];
                     Prim<double> Ap, Bp;
-                    Ap.rp = QA[0]; Ap.m[0] = QA[1]; Ap.m[1] = QA[2]; Ap.m[2] = QA[3]; Ap.tp = QA[4];
+                    Ap.rp = Ql[fnode]; Ap.m[0] = Ql[NP + fnode]; Ap.m[1] = Ql[2 * NP + fnode];
+                    Ap.m[2] = Ql[3 * NP + fnode]; Ap.tp = Ql[4 * NP + fnode];
                     make_prim(Ap, rrA, prA, T0, gam, gR);
                     Bp.rp = a.q[0][m] + (DIN ? a.din[0][m] : 0.0);
                     Bp.m[0] = a.q[1][m] + (DIN ? a.din[1][m] : 0.0);
@@ -251,15 +216,14 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                     Bp.tp = a.q[4][m] + (DIN ? a.din[4][m] : 0.0);
                     make_prim(Bp, d.rho_ref[m], d.p_ref[m], T0, gam, gR);
                     const double s = own ? -1.0 : 1.0;
+                    double coef[5];
                     for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - a.w[v][m] / cvm);
                     if (own) rusanov_vjp_side(Ap, Bp, nvd, 0.5, T0, gam, gR, coef, true, rrA, prA, contrib);
                     else rusanov_vjp_side(Bp, Ap, nvd, 0.5, T0, gam, gR, coef, false, rrA, prA, contrib);
                 } else {
-                    const BcTable& bt = *d.bc;
-                    const int pi = ef.meta >> 8;
-                    const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
-                    for (int v = 0; v < 5; v++) coef[v] = -Wl[v * NP + fnode];
-                    ghost_face_vjp(QA, nvd, bt.kind[1][pi], uv, rrA, prA, T0, gam, gR, coef, contrib);
+                    const double* ga = d.gadj + (size_t)d.ghost_of[(size_t)e * 6 + g] * 5 * d.npfm +
+                                       fa * (g < 2 ? NY : NZ) + fb;
+                    for (int v = 0; v < 5; v++) contrib[v] = ga[v * d.npfm];
                 }
             }
             for (int v = 0; v < 5; v++) FL[v * NFN + t] = contrib[v];
@@ -267,6 +231,60 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     }
     __syncthreads();
     if (!active) return;
+
+    // phase 3: volume term, its flux Jacobian, buoyancy, then the face terms
+    // of the faces through this node; the flux-row cotangents are consumed
+    // row by row (rows rho, Theta, then momentum, the order in which each
+    // accumulator of flux_rows_vjp sums them)
+    double qbar[5] = {0, 0, 0, 0, 0};
+    {
+        double J[9];
+        if (d.affine) {
+            // the forward stage's per-element metric (JcV = Jinv_e (vol_e W))
+            const double* em = d.emet + (size_t)e * 10;
+            const double cv64 = em[9] * d.wnode[q];
+            for (int c = 0; c < 9; c++) J[c] = em[c] * cv64;
+        } else {
+            for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * nb + n];
+        }
+        const double rref = d.rho_ref[n], pref = d.p_ref[n];
+        Prim<double> P;
+        P.rp = Ql[q]; P.m[0] = Ql[NP + q]; P.m[1] = Ql[2 * NP + q]; P.m[2] = Ql[3 * NP + q]; P.tp = Ql[4 * NP + q];
+        make_prim(P, rref, pref, T0, gam, gR);
+        const double* D0 = Dl;
+        const double* Dy = Dl + NX * NX;
+        const double* Dz = Dy + NY * NY;
+        double ub[3] = {0, 0, 0}, thb = 0, ppb = 0;
+#pragma unroll 1
+        for (int oi = 0; oi < 5; oi++) {
+            const int row = oi == 0 ? 0 : (oi == 1 ? 4 : oi - 1);
+            const double* W = Wl + row * NP;
+            double h0 = 0, h1 = 0, h2 = 0;
+            for (int t = 0; t < NX; t++) h0 += D0[i * NX + t] * W[t * NY * NZ + j * NZ + k];
+            for (int t = 0; t < NY; t++) h1 += Dy[j * NY + t] * W[i * NY * NZ + t * NZ + k];
+            for (int t = 0; t < NZ; t++) h2 += Dz[k * NZ + t] * W[i * NY * NZ + j * NZ + t];
+            const double fb[3] = {h0 * J[0] + h1 * J[3] + h2 * J[5], h0 * J[6] + h1 * J[1] + h2 * J[4],
+                                  h0 * J[8] + h1 * J[7] + h2 * J[2]};
+            if (row == 0) {                                // F_rho = m
+                for (int c = 0; c < 3; c++) qbar[1 + c] += fb[c];
+            } else if (row == 4) {                         // F_Theta = m theta
+                for (int c = 0; c < 3; c++) {
+                    qbar[1 + c] += fb[c] * (T0 + P.thp);
+                    thb += fb[c] * P.m[c];
+                }
+            } else {                                       // F_m_r = m_r u + p' e_r
+                const int r = row - 1;
+                ppb += fb[r];
+                for (int c = 0; c < 3; c++) {
+                    qbar[1 + r] += fb[c] * P.u[c];
+                    ub[c] += fb[c] * P.m[r];
+                }
+            }
+        }
+        prim_to_conserved(P, rref, pref, T0, gam, ub, thb, ppb, qbar);
+    }
+    const double wn[5] = {a.w[0][n], a.w[1][n], a.w[2][n], a.w[3][n], a.w[4][n]};
+    if (d.buoyancy) qbar[0] += wn[1] * d.g[0] + wn[2] * d.g[1] + wn[3] * d.g[2];
     for (int g = 0; g < 6; g++) {
         if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
         if (!((d.eface[(size_t)e * 6 + g].meta >> 4) & 1)) continue;
@@ -281,6 +299,48 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         if (a.t[0]) a.t[v][n] = qbar[v];
         if (a.ad_out[0]) a.ad_out[v][n] = a.c1 * (wn[v] + a.dt * qbar[v]);
         if (a.acc_out[0]) a.acc_out[v][n] = a.acc_in[v][n] + a.c2 * a.dt * qbar[v];
+    }
+}
+
+// Boundary (ghost) sides of the vector-Jacobian product, ahead of the
+// element kernel: one work-group per ghost side, one thread per face node,
+// the flux through the ghost state differentiated by forward-mode sweeps
+// (ghost_face_vjp) with the element kernel's coefficient -w/cV, written to
+// gadj for the element kernel to add in its face order.  Kept out of the
+// element kernel, whose registers the sweeps' call would otherwise hold.
+template <bool DIN>
+__global__ void __launch_bounds__(64) ghost_vjp_kernel(DgDev d, VjpArgs a) {
+    const int NX = d.npx, NY = d.npy, NZ = d.npz, NP = d.np;
+    const uint32_t rec = d.ghost_rec[blockIdx.x];
+    const uint32_t e = rec / 6;
+    const int g = rec - e * 6;
+    const int nbf = g < 2 ? NY : NZ, naf = g < 4 ? NX : NY;
+    const double T0 = d.T0, gam = d.gamma, gR = d.gamma * d.R;
+    for (int t = threadIdx.x; t < naf * nbf; t += blockDim.x) {
+        const int fa = t / nbf, fb = t - fa * nbf;
+        int i, j, k;
+        if (g < 2) { i = fa; j = fb; k = (g & 1) ? NZ - 1 : 0; }
+        else if (g < 4) { i = fa; j = (g & 1) ? NY - 1 : 0; k = fb; }
+        else { i = (g & 1) ? NX - 1 : 0; j = fa; k = fb; }
+        const uint64_t na = (uint64_t)e * NP + (uint64_t)(i * NY * NZ + j * NZ + k);
+        const EFace ef = d.eface[rec];
+        const bool own = (ef.meta >> 3) & 1;
+        const int og = ef.meta & 7;
+        const double wgt = face_weight(d, own ? g : og, fa, fb);
+        const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                               d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+        const double cvn = d.cV[na];
+        double QA[5], coef[5], contrib[5];
+        for (int v = 0; v < 5; v++) {
+            QA[v] = a.q[v][na] + (DIN ? a.din[v][na] : 0.0);
+            coef[v] = -(a.w[v][na] / cvn);
+        }
+        const BcTable& bt = *d.bc;
+        const int pi = ef.meta >> 8;
+        const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
+        ghost_face_vjp(QA, nvd, bt.kind[1][pi], uv, d.rho_ref[na], d.p_ref[na], T0, gam, gR, coef, contrib);
+        double* ga = d.gadj + (size_t)blockIdx.x * 5 * d.npfm + t;
+        for (int v = 0; v < 5; v++) ga[v * d.npfm] = contrib[v];
     }
 }
 
@@ -451,6 +511,12 @@ hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), (size_t)29 * d.npfm * sizeof(double),
                            st, d, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (d.nghost) {
+        if (a.din[0]) hipLaunchKernelGGL(ghost_vjp_kernel<true>, dim3(d.nghost), dim3(64), 0, st, d, a);
+        else hipLaunchKernelGGL(ghost_vjp_kernel<false>, dim3(d.nghost), dim3(64), 0, st, d, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
     if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_vjp_t<8, 8, 8>(d, a, st);
     if (d.npx == 6 && d.npy == 6 && d.npz == 6) return launch_vjp_t<6, 6, 6>(d, a, st);

@@ -109,6 +109,14 @@ struct DgDev {
     double* madj;
     // the split step's strong-gradient lift of the mortar sides [slot][npfm][12]
     double* mgrad;
+    // boundary (ghost) sides of the vector-Jacobian product: ghost_rec
+    // [nghost] = element * 6 + face group, ghost_of [nelem * 6] = the
+    // record of a ghost side (or ~0u), gadj [nghost][5][npfm] the face-node
+    // terms a pre-pass writes for the element kernel to add in face order
+    int nghost;
+    const uint32_t* ghost_rec;
+    const uint32_t* ghost_of;
+    double* gadj;
     // set to 1 by a kernel that produced a NaN or Inf in the state (the
     // split step's update; the RK stage kernels take it in RkArgs)
     int* flag;

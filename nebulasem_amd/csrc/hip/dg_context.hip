@@ -345,6 +345,15 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
             mbg[f * npfm + t] = sum - rref[mfine[f * npfm + t]];
         }
     }
+    // ghost sides (boundary faces that are not 2:1 mortar sides)
+    std::vector<uint32_t> ghost_rec, ghost_of((size_t)g.nbcs * 6, ~0u);
+    for (size_t r = 0; r < ef.size(); r++) {
+        const uint32_t m = ef[r].meta;
+        if (((m >> 4) & 1) && ((m >> 5) & 1) && !((m >> 6) & 1)) {
+            ghost_of[r] = (uint32_t)ghost_rec.size();
+            ghost_rec.push_back((uint32_t)r);
+        }
+    }
     std::vector<double> gfn(3 * (size_t)g.nfacets);
     for (Int f = 0; f < g.nfacets; f++)
         for (int c = 0; c < 3; c++) gfn[3 * (size_t)f + c] = lm.fn[f][c];
@@ -354,7 +363,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         (e = upload(d->p_ref, pref, s)) != hipSuccess || (e = upload(d->rho_ref, rref, s)) != hipSuccess ||
         (e = upload(d->eface, ef, s)) != hipSuccess || (e = upload(d->gFN, gfn, s)) != hipSuccess ||
         (e = upload(d->gh, ghv, s)) != hipSuccess || (e = upload(d->emet, emet, s)) != hipSuccess ||
-        (e = upload(d->wnode, wnode, s)) != hipSuccess)
+        (e = upload(d->wnode, wnode, s)) != hipSuccess || (e = upload(d->ghost_rec, ghost_rec, s)) != hipSuccess ||
+        (e = upload(d->ghost_of, ghost_of, s)) != hipSuccess)
         return nsem_fail(NSEM_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
     for (int dir = 0; dir < 3; dir++) {
         if ((e = upload(d->D[dir], g.basis[dir].dpsi, s)) != hipSuccess ||
@@ -404,6 +414,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     HIP_TRY(d->diag_out.alloc(6 * sizeof(double)));
     HIP_TRY(d->flag.alloc(sizeof(int)));
     HIP_TRY(hipMemsetAsync(d->flag.p, 0, sizeof(int), s));
+    HIP_TRY(d->gadj.alloc(ghost_rec.size() * 5 * npfm * sizeof(double)));
     HIP_TRY(hipStreamSynchronize(s));
     d->host_p_ref = pref;
     d->host_rho_ref = rref;
@@ -447,6 +458,10 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.madj = d->madj.as<double>();
     dv.mgrad = d->mgrad.as<double>();
     dv.flag = d->flag.as<int>();
+    dv.nghost = (int)ghost_rec.size();
+    dv.ghost_rec = d->ghost_rec.as<uint32_t>();
+    dv.ghost_of = d->ghost_of.as<uint32_t>();
+    dv.gadj = d->gadj.as<double>();
     *out = d.release();
     return NSEM_OK;
 }

@@ -44,6 +44,8 @@ struct nsem_dg {
     // 2:1 non-conforming faces (empty on conforming meshes)
     DevBuf mort, mort_fine, mort_fms, mort_w, mort_bg, psiRef[3], psiCor[3], cslot, mphi, msurf, mphi_t, msurf_t,
         madj, mgrad;
+    // ghost sides of the vector-Jacobian product
+    DevBuf ghost_rec, ghost_of, gadj;
     // double-buffered state: rho ux uy uz T p
     DevBuf st[2][6];
     int cur = 0;
