@@ -149,7 +149,8 @@ __device__ __noinline__ void ghost_face_vjp(const double QA[5], const double nvd
 
 // DIN: the launch reads a stage increment (compile-time, so the state loads
 // carry no null-pointer branches and can be issued together).
-template <int NXc, int NYc, int NZc, bool DIN>
+// O32: node byte offsets formed in 32 bits (nb * 8 below 4 GiB).
+template <int NXc, int NYc, int NZc, bool DIN, bool O32>
 __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
@@ -175,10 +176,10 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     }
     // phase 1: w / cV and Q = q + din to LDS
     if (active) {
-        const double cvn = d.cV[n];
+        const double cvn = ldn<O32>(d.cV, n);
         for (int v = 0; v < 5; v++) {
-            Wl[v * NP + q] = a.w[v][n] / cvn;
-            Ql[v * NP + q] = a.q[v][n] + (DIN ? a.din[v][n] : 0.0);
+            Wl[v * NP + q] = ldn<O32>(a.w[v], n) / cvn;
+            Ql[v * NP + q] = ldn<O32>(a.q[v], n) + (DIN ? ldn<O32>(a.din[v], n) : 0.0);
         }
     }
     __syncthreads();
@@ -200,24 +201,24 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                     const int og = ef.meta & 7;
                     const double wgt = face_weight(d, own ? g : og, fa, fb);
                     const uint64_t na = (uint64_t)e * NP + fnode;
-                    const double rrA = d.rho_ref[na], prA = d.p_ref[na];
+                    const double rrA = ldn<O32>(d.rho_ref, na), prA = ldn<O32>(d.p_ref, na);
                     const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
                                            d.gFN[3 * (size_t)ef.facet + 2] * wgt};
                     const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
-                    const double cvm = d.cV[m];
+                    const double cvm = ldn<O32>(d.cV, m);
                     Prim<double> Ap, Bp;
                     Ap.rp = Ql[fnode]; Ap.m[0] = Ql[NP + fnode]; Ap.m[1] = Ql[2 * NP + fnode];
                     Ap.m[2] = Ql[3 * NP + fnode]; Ap.tp = Ql[4 * NP + fnode];
                     make_prim(Ap, rrA, prA, T0, gam, gR);
-                    Bp.rp = a.q[0][m] + (DIN ? a.din[0][m] : 0.0);
-                    Bp.m[0] = a.q[1][m] + (DIN ? a.din[1][m] : 0.0);
-                    Bp.m[1] = a.q[2][m] + (DIN ? a.din[2][m] : 0.0);
-                    Bp.m[2] = a.q[3][m] + (DIN ? a.din[3][m] : 0.0);
-                    Bp.tp = a.q[4][m] + (DIN ? a.din[4][m] : 0.0);
-                    make_prim(Bp, d.rho_ref[m], d.p_ref[m], T0, gam, gR);
+                    Bp.rp = ldn<O32>(a.q[0], m) + (DIN ? ldn<O32>(a.din[0], m) : 0.0);
+                    Bp.m[0] = ldn<O32>(a.q[1], m) + (DIN ? ldn<O32>(a.din[1], m) : 0.0);
+                    Bp.m[1] = ldn<O32>(a.q[2], m) + (DIN ? ldn<O32>(a.din[2], m) : 0.0);
+                    Bp.m[2] = ldn<O32>(a.q[3], m) + (DIN ? ldn<O32>(a.din[3], m) : 0.0);
+                    Bp.tp = ldn<O32>(a.q[4], m) + (DIN ? ldn<O32>(a.din[4], m) : 0.0);
+                    make_prim(Bp, ldn<O32>(d.rho_ref, m), ldn<O32>(d.p_ref, m), T0, gam, gR);
                     const double s = own ? -1.0 : 1.0;
                     double coef[5];
-                    for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - a.w[v][m] / cvm);
+                    for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - ldn<O32>(a.w[v], m) / cvm);
                     if (own) rusanov_vjp_side(Ap, Bp, nvd, 0.5, T0, gam, gR, coef, true, rrA, prA, contrib);
                     else rusanov_vjp_side(Bp, Ap, nvd, 0.5, T0, gam, gR, coef, false, rrA, prA, contrib);
                 } else {
@@ -245,9 +246,9 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
             const double cv64 = em[9] * d.wnode[q];
             for (int c = 0; c < 9; c++) J[c] = em[c] * cv64;
         } else {
-            for (int c = 0; c < 9; c++) J[c] = d.JcV[(size_t)c * nb + n];
+            for (int c = 0; c < 9; c++) J[c] = ldn<O32>(d.JcV + (size_t)c * nb, n);
         }
-        const double rref = d.rho_ref[n], pref = d.p_ref[n];
+        const double rref = ldn<O32>(d.rho_ref, n), pref = ldn<O32>(d.p_ref, n);
         Prim<double> P;
         P.rp = Ql[q]; P.m[0] = Ql[NP + q]; P.m[1] = Ql[2 * NP + q]; P.m[2] = Ql[3 * NP + q]; P.tp = Ql[4 * NP + q];
         make_prim(P, rref, pref, T0, gam, gR);
@@ -283,7 +284,8 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         }
         prim_to_conserved(P, rref, pref, T0, gam, ub, thb, ppb, qbar);
     }
-    const double wn[5] = {a.w[0][n], a.w[1][n], a.w[2][n], a.w[3][n], a.w[4][n]};
+    const double wn[5] = {ldn<O32>(a.w[0], n), ldn<O32>(a.w[1], n), ldn<O32>(a.w[2], n), ldn<O32>(a.w[3], n),
+                          ldn<O32>(a.w[4], n)};
     if (d.buoyancy) qbar[0] += wn[1] * d.g[0] + wn[2] * d.g[1] + wn[3] * d.g[2];
     for (int g = 0; g < 6; g++) {
         if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
@@ -294,11 +296,11 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         for (int v = 0; v < 5; v++) qbar[v] += FL[v * NFN + t];
     }
     if (d.nmort)
-        for (int v = 0; v < 5; v++) qbar[v] += d.madj[(size_t)v * nb + n];
+        for (int v = 0; v < 5; v++) qbar[v] += ldn<O32>(d.madj + (size_t)v * nb, n);
     for (int v = 0; v < 5; v++) {
-        if (a.t[0]) a.t[v][n] = qbar[v];
-        if (a.ad_out[0]) a.ad_out[v][n] = a.c1 * (wn[v] + a.dt * qbar[v]);
-        if (a.acc_out[0]) a.acc_out[v][n] = a.acc_in[v][n] + a.c2 * a.dt * qbar[v];
+        if (a.t[0]) stv<O32>(a.t[v], n, qbar[v]);
+        if (a.ad_out[0]) stv<O32>(a.ad_out[v], n, a.c1 * (wn[v] + a.dt * qbar[v]));
+        if (a.acc_out[0]) stv<O32>(a.acc_out[v], n, ldn<O32>(a.acc_in[v], n) + a.c2 * a.dt * qbar[v]);
     }
 }
 
@@ -490,8 +492,16 @@ hipError_t launch_vjp_t(const DgDev& d, const VjpArgs& a_in, hipStream_t st) {
     const size_t lds = rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb);
     if (NP > 512 || lds > 65536) return hipErrorInvalidValue;
     a.epb = epb;
-    if (a.din[0]) hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, true>), dim3(blocks), dim3(threads), lds, st, d, a);
-    else hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, false>), dim3(blocks), dim3(threads), lds, st, d, a);
+    // node byte offsets below 4 GiB (each array is addressed from its own
+    // base, JcV and madj from the base of each component's slice)
+    const bool o32 = d.nb * sizeof(double) < (1ull << 32);
+    if (a.din[0]) {
+        if (o32) hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, true, true>), dim3(blocks), dim3(threads), lds, st, d, a);
+        else hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, true, false>), dim3(blocks), dim3(threads), lds, st, d, a);
+    } else {
+        if (o32) hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, false, true>), dim3(blocks), dim3(threads), lds, st, d, a);
+        else hipLaunchKernelGGL((rk_vjp_kernel<X, Y, Z, false, false>), dim3(blocks), dim3(threads), lds, st, d, a);
+    }
     return hipGetLastError();
 }
 

@@ -122,6 +122,31 @@ struct DgDev {
     int* flag;
 };
 
+// Node-array access.  With O32 the byte offset is formed in 32 bits (node
+// arrays below 4 GiB each), so the load takes the array base from scalar
+// registers and one shared 32-bit offset instead of a 64-bit address per
+// array.
+// (The empty asm keeps the 32-bit offset opaque, so it is not re-widened to
+// a 64-bit address computation.)
+__device__ __forceinline__ uint32_t byte_off(uint64_t i, uint32_t size) {
+    uint32_t off = (uint32_t)i * size;
+    __asm__("" : "+v"(off));
+    return off;
+}
+template <bool O32, typename T>
+__device__ __forceinline__ T ldn(const T* base, uint64_t i) {
+    if constexpr (O32)
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off(i, sizeof(T)));
+    else
+        return base[i];
+}
+template <bool O32, typename T>
+__device__ __forceinline__ void stv(T* base, uint64_t i, T v) {
+    if constexpr (O32)
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off(i, sizeof(T))) = v;
+    else
+        base[i] = v;
+}
 __device__ __forceinline__ double dot3r(double a0, double a1, double a2, double b0, double b1,
                                         double b2) {
     // right-nested like the reference's unrolled vector dot product
