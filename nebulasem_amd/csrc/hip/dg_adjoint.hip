@@ -174,11 +174,14 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         const int s1 = NX * NX, s2 = s1 + NY * NY;
         Dl[t] = t < s1 ? d.D[0][t] : t < s2 ? d.D[1][t - s1] : d.D[2][t - s2];
     }
-    // phase 1: w / cV and Q = q + din to LDS
+    // phase 1: w / cV and Q = q + din to LDS (w itself kept for the
+    // epilogue)
+    double wn[5] = {0, 0, 0, 0, 0};
     if (active) {
         const double cvn = ldn<O32>(d.cV, n);
         for (int v = 0; v < 5; v++) {
-            Wl[v * NP + q] = ldn<O32>(a.w[v], n) / cvn;
+            wn[v] = ldn<O32>(a.w[v], n);
+            Wl[v * NP + q] = wn[v] / cvn;
             Ql[v * NP + q] = ldn<O32>(a.q[v], n) + (DIN ? ldn<O32>(a.din[v], n) : 0.0);
         }
     }
@@ -284,8 +287,6 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         }
         prim_to_conserved(P, rref, pref, T0, gam, ub, thb, ppb, qbar);
     }
-    const double wn[5] = {ldn<O32>(a.w[0], n), ldn<O32>(a.w[1], n), ldn<O32>(a.w[2], n), ldn<O32>(a.w[3], n),
-                          ldn<O32>(a.w[4], n)};
     if (d.buoyancy) qbar[0] += wn[1] * d.g[0] + wn[2] * d.g[1] + wn[3] * d.g[2];
     for (int g = 0; g < 6; g++) {
         if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
@@ -300,7 +301,11 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     for (int v = 0; v < 5; v++) {
         if (a.t[0]) stv<O32>(a.t[v], n, qbar[v]);
         if (a.ad_out[0]) stv<O32>(a.ad_out[v], n, a.c1 * (wn[v] + a.dt * qbar[v]));
-        if (a.acc_out[0]) stv<O32>(a.acc_out[v], n, ldn<O32>(a.acc_in[v], n) + a.c2 * a.dt * qbar[v]);
+        if (a.acc_out[0]) {
+            // the last RK3 stage's accumulator starts from w itself
+            const double ain = a.acc_in[0] == a.w[0] ? wn[v] : ldn<O32>(a.acc_in[v], n);
+            stv<O32>(a.acc_out[v], n, ain + a.c2 * a.dt * qbar[v]);
+        }
     }
 }
 
