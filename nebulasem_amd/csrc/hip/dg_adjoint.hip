@@ -365,16 +365,20 @@ __global__ void __launch_bounds__(64) ghost_vjp_kernel(DgDev d, VjpArgs a) {
 // back to the coarse nodes through psiRef transposed, and through the node
 // values' derivatives to the coarse states.  Contributions are added into
 // d.madj [5][nb] (a node can sit on several mortar faces), which the VJP
-// kernel adds to its own.  One work-group per mortar face, one thread per
-// face node (also, in the last phase, per coarse side node).
-__global__ void __launch_bounds__(256) mortar_vjp_kernel(DgDev d, VjpArgs a) {
+// kernel adds to its own.  One work-group per mortar face; per face node
+// (refined, and in the last phase coarse) kMortarSub threads, thread s of a
+// node taking state component k = s and interpolated values j = s, s + 5,
+// s + 10, so the 17 (refined) and 5 (coarse) forward-mode sweeps of a node
+// run side by side.
+constexpr int kMortarSub = 5;
+__global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     const uint32_t f = blockIdx.x;
-    const int t = threadIdx.x;
+    const int np = d.npfm;
+    const int sub = threadIdx.x / np, t = threadIdx.x - sub * np;
     const MortarFace mf = d.mort[f];
-    const bool act = t < mf.n0 * mf.n1;
+    const bool act = sub < kMortarSub && t < mf.n0 * mf.n1;
     const int NX = d.npx, NY = d.npy, NZ = d.npz;
     const int ao = t / mf.n1, bo = t - ao * mf.n1;
-    const int np = d.npfm;
     const size_t ft = (size_t)f * np + t;
     const uint64_t nb = d.nb;
     const double T0 = d.T0, gam = d.gamma, gR = d.gamma * d.R;
@@ -406,9 +410,17 @@ __global__ void __launch_bounds__(256) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     double* Lv = mortar_adj_lds;   // [12][np] coarse node values
     double* Lw = Lv + 12 * np;     // [5][np]  sc w_t' w^(coarse node t')
     double* Lp = Lw + 5 * np;      // [12][np] cotangents of the interpolated values
-    uint64_t nc = 0;
-    if (act) {
-        nc = face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ);
+    double* Lf = Lp + 12 * np;     // [12][np] refined node values
+    double* Lc = Lf + 12 * np;     // [12][np] interpolated coarse values; then gbar
+    double* Lb = Lc + 12 * np;     // [5][np]  cotangent of phi
+    double* LR0 = Lb + 5 * np;     // psiRef and psiCor of the face's half, n0^2 and n1^2 each
+    double* LR1 = LR0 + mf.n0 * mf.n0;
+    double* LC0 = LR1 + mf.n1 * mf.n1;
+    double* LC1 = LC0 + mf.n0 * mf.n0;
+    for (int r = threadIdx.x; r < mf.n0 * mf.n0; r += blockDim.x) { LR0[r] = R0[r]; LC0[r] = C0[r]; }
+    for (int r = threadIdx.x; r < mf.n1 * mf.n1; r += blockDim.x) { LR1[r] = R1[r]; LC1[r] = C1[r]; }
+    const uint64_t nc = face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ);
+    if (act && sub == 0) {
         double v[12];
         values(nc, v);
         for (int q = 0; q < 12; q++) Lv[q * np + t] = v[q];
@@ -417,45 +429,57 @@ __global__ void __launch_bounds__(256) mortar_vjp_kernel(DgDev d, VjpArgs a) {
         for (int v5 = 0; v5 < 5; v5++) Lw[v5 * np + t] = sc * (a.w[v5][nc] * icv);
     }
     __syncthreads();
+    const uint64_t nf = act ? d.mort_fine[ft] : 0;
     if (act) {
-        const uint64_t nf = d.mort_fine[ft];
-        double vf[12], vc[12];
-        values(nf, vf);
-        for (int q = 0; q < 12; q++) vc[q] = 0.0;
+        if (sub == 0) {
+            double vf[12];
+            values(nf, vf);
+            for (int q = 0; q < 12; q++) Lf[q * np + t] = vf[q];
+        }
+        // interpolated values q = sub, sub + 5, sub + 10
+        double vc[3] = {0.0, 0.0, 0.0};
         for (int an = 0; an < mf.n0; an++)
             for (int bn = 0; bn < mf.n1; bn++) {
-                const double w = R0[an * mf.n0 + ao] * R1[bn * mf.n1 + bo];
-                for (int q = 0; q < 12; q++) vc[q] += w * Lv[q * np + an * mf.n1 + bn];
+                const double w = LR0[an * mf.n0 + ao] * LR1[bn * mf.n1 + bo];
+                for (int i = 0; i < 3; i++)
+                    if (sub + kMortarSub * i < 12) vc[i] += w * Lv[(sub + kMortarSub * i) * np + an * mf.n1 + bn];
             }
-        // cotangent of phi_t
+        for (int i = 0; i < 3; i++)
+            if (sub + kMortarSub * i < 12) Lc[(sub + kMortarSub * i) * np + t] = vc[i];
+        // cotangent of phi_t, component sub
         const double sf = fo ? -d.mort_w[ft] : d.mort_w[ft];
         const double icvf = 1.0 / d.cV[nf];
-        double phib[5];
-        for (int v5 = 0; v5 < 5; v5++) phib[v5] = sf * (a.w[v5][nf] * icvf);
+        double phib = sf * (a.w[sub][nf] * icvf);
         for (int an = 0; an < mf.n0; an++)
             for (int bn = 0; bn < mf.n1; bn++) {
-                const double c = C0[ao * mf.n0 + an] * C1[bo * mf.n1 + bn];
-                for (int v5 = 0; v5 < 5; v5++) phib[v5] += c * Lw[v5 * np + an * mf.n1 + bn];
+                const double c = LC0[ao * mf.n0 + an] * LC1[bo * mf.n1 + bn];
+                phib += c * Lw[sub * np + an * mf.n1 + bn];
             }
+        Lb[sub * np + t] = phib;
+    }
+    __syncthreads();
+    if (act) {
         const D1 bg(d.mort_bg[ft]);
-        // the refined node's state
-#pragma nounroll
-        for (int k = 0; k < 5; k++) {
+        double phib[5];
+        for (int v5 = 0; v5 < 5; v5++) phib[v5] = Lb[v5 * np + t];
+        {
+            // the refined node's state component k = sub
+            const int k = sub;
             D1 vfD[12], vcD[12], phi[5];
             values_d(nf, k, vfD);
-            for (int q = 0; q < 12; q++) vcD[q] = D1(vc[q]);
+            for (int q = 0; q < 12; q++) vcD[q] = D1(Lc[q * np + t]);
             mortar_rusanov(vfD, vcD, bg, nvD, fo, D1(T0), phi);
             double acc = 0;
             for (int v5 = 0; v5 < 5; v5++) acc += phib[v5] * phi[v5].d[0];
             atomicAdd(d.madj + (size_t)k * nb + nf, acc);
         }
-        // the interpolated values
+        // the interpolated values j = sub, sub + 5, sub + 10
 #pragma nounroll
-        for (int j = 0; j < 12; j++) {
+        for (int j = sub; j < 12; j += kMortarSub) {
             D1 vfD[12], vcD[12], phi[5];
             for (int q = 0; q < 12; q++) {
-                vfD[q] = D1(vf[q]);
-                vcD[q] = seeded(vc[q], q == j);
+                vfD[q] = D1(Lf[q * np + t]);
+                vcD[q] = seeded(Lc[q * np + t], q == j);
             }
             mortar_rusanov(vfD, vcD, bg, nvD, fo, D1(T0), phi);
             double acc = 0;
@@ -464,22 +488,27 @@ __global__ void __launch_bounds__(256) mortar_vjp_kernel(DgDev d, VjpArgs a) {
         }
     }
     __syncthreads();
-    if (!act) return;
     // coarse side node (ao, bo): psiRef transposed, then the node values'
     // derivatives
-    double gbar[12];
-    for (int q = 0; q < 12; q++) gbar[q] = 0.0;
-    for (int a2 = 0; a2 < mf.n0; a2++)
-        for (int b2 = 0; b2 < mf.n1; b2++) {
-            const double r = R0[ao * mf.n0 + a2] * R1[bo * mf.n1 + b2];
-            for (int q = 0; q < 12; q++) gbar[q] += r * Lp[q * np + a2 * mf.n1 + b2];
-        }
-#pragma nounroll
-    for (int k = 0; k < 5; k++) {
+    if (act) {
+        double gb[3] = {0.0, 0.0, 0.0};
+        for (int a2 = 0; a2 < mf.n0; a2++)
+            for (int b2 = 0; b2 < mf.n1; b2++) {
+                const double r = LR0[ao * mf.n0 + a2] * LR1[bo * mf.n1 + b2];
+                for (int i = 0; i < 3; i++)
+                    if (sub + kMortarSub * i < 12) gb[i] += r * Lp[(sub + kMortarSub * i) * np + a2 * mf.n1 + b2];
+            }
+        for (int i = 0; i < 3; i++)
+            if (sub + kMortarSub * i < 12) Lc[(sub + kMortarSub * i) * np + t] = gb[i];
+    }
+    __syncthreads();
+    if (!act) return;
+    {
+        const int k = sub;
         D1 v[12];
         values_d(nc, k, v);
         double acc = 0;
-        for (int q = 0; q < 12; q++) acc += gbar[q] * v[q].d[0];
+        for (int q = 0; q < 12; q++) acc += Lc[q * np + t] * v[q].d[0];
         atomicAdd(d.madj + (size_t)k * nb + nc, acc);
     }
 }
@@ -519,12 +548,14 @@ size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb) {
 
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
     if (d.nmort) {
-        if (d.npfm > 256) return hipErrorInvalidValue;
+        if (kMortarSub * d.npfm > 512) return hipErrorInvalidValue;
         hipError_t e = hipMemsetAsync(d.madj, 0, 5 * d.nb * sizeof(double), st);
         if (e != hipSuccess) return e;
-        const int threads = (d.npfm + 63) / 64 * 64;
-        hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), (size_t)29 * d.npfm * sizeof(double),
-                           st, d, a);
+        const int threads = (kMortarSub * d.npfm + 63) / 64 * 64;
+        // LDS: 58 rows of npfm values and the four 1-D matrices (at most
+        // 4 x 16 x 16 for the largest supported element)
+        const size_t lds = ((size_t)58 * d.npfm + 2 * (d.npx * d.npx + d.npy * d.npy + d.npz * d.npz)) * sizeof(double);
+        hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), lds, st, d, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (d.nghost) {
