@@ -277,12 +277,20 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                     thb += fb[c] * P.m[c];
                 }
             } else {                                       // F_m_r = m_r u + p' e_r
+                // (component r selected, not indexed: a run-time index
+                // would put qbar in scratch memory)
                 const int r = row - 1;
-                ppb += fb[r];
+                const double fr = r == 0 ? fb[0] : (r == 1 ? fb[1] : fb[2]);
+                const double mr = r == 0 ? P.m[0] : (r == 1 ? P.m[1] : P.m[2]);
+                double qr = r == 0 ? qbar[1] : (r == 1 ? qbar[2] : qbar[3]);
+                ppb += fr;
                 for (int c = 0; c < 3; c++) {
-                    qbar[1 + r] += fb[c] * P.u[c];
-                    ub[c] += fb[c] * P.m[r];
+                    qr += fb[c] * P.u[c];
+                    ub[c] += fb[c] * mr;
                 }
+                if (r == 0) qbar[1] = qr;
+                else if (r == 1) qbar[2] = qr;
+                else qbar[3] = qr;
             }
         }
         prim_to_conserved(P, rref, pref, T0, gam, ub, thb, ppb, qbar);
