@@ -304,8 +304,18 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         const int t = face_task_index(g, fa, fb, NX, NY, NZ);
         for (int v = 0; v < 5; v++) qbar[v] += FL[v * NFN + t];
     }
-    if (d.nmort)
-        for (int v = 0; v < 5; v++) qbar[v] += ldn<O32>(d.madj + (size_t)v * nb, n);
+    if (d.nmort) {
+        // the mortar faces' share (mortar_vjp_kernel) sits only on nodes of
+        // elements with a 2:1 side; read it there and clear it for the next
+        // launch (madj is zero between launches)
+        bool mside = false;
+        for (int g = 0; g < 6; g++) mside = mside || ((d.eface[(size_t)e * 6 + g].meta >> 6) & 1);
+        if (mside)
+            for (int v = 0; v < 5; v++) {
+                qbar[v] += ldn<O32>(d.madj + (size_t)v * nb, n);
+                stv<O32>(d.madj + (size_t)v * nb, n, 0.0);
+            }
+    }
     for (int v = 0; v < 5; v++) {
         if (a.t[0]) stv<O32>(a.t[v], n, qbar[v]);
         if (a.ad_out[0]) stv<O32>(a.ad_out[v], n, a.c1 * (wn[v] + a.dt * qbar[v]));
@@ -555,28 +565,35 @@ size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb) {
 }
 
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
+    // madj is zero on entry (cleared when the context is made, and by each
+    // element VJP launch after reading it); a failed launch clears it
+    auto fail = [&](hipError_t e) {
+        if (d.nmort) (void)hipMemsetAsync(d.madj, 0, 5 * d.nb * sizeof(double), st);
+        return e;
+    };
     if (d.nmort) {
         if (kMortarSub * d.npfm > 512) return hipErrorInvalidValue;
-        hipError_t e = hipMemsetAsync(d.madj, 0, 5 * d.nb * sizeof(double), st);
-        if (e != hipSuccess) return e;
+        hipError_t e;
         const int threads = (kMortarSub * d.npfm + 63) / 64 * 64;
         // LDS: 58 rows of npfm values and the four 1-D matrices (at most
         // 4 x 16 x 16 for the largest supported element)
         const size_t lds = ((size_t)58 * d.npfm + 2 * (d.npx * d.npx + d.npy * d.npy + d.npz * d.npz)) * sizeof(double);
         hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), lds, st, d, a);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     }
     if (d.nghost) {
         if (a.din[0]) hipLaunchKernelGGL(ghost_vjp_kernel<true>, dim3(d.nghost), dim3(64), 0, st, d, a);
         else hipLaunchKernelGGL(ghost_vjp_kernel<false>, dim3(d.nghost), dim3(64), 0, st, d, a);
         hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) return fail(e);
     }
-    if (d.npx == 8 && d.npy == 8 && d.npz == 8) return launch_vjp_t<8, 8, 8>(d, a, st);
-    if (d.npx == 6 && d.npy == 6 && d.npz == 6) return launch_vjp_t<6, 6, 6>(d, a, st);
-    if (d.npx == 6 && d.npy == 1 && d.npz == 6) return launch_vjp_t<6, 1, 6>(d, a, st);
-    if (d.npx == 5 && d.npy == 1 && d.npz == 5) return launch_vjp_t<5, 1, 5>(d, a, st);
-    return launch_vjp_t<0, 0, 0>(d, a, st);
+    hipError_t e;
+    if (d.npx == 8 && d.npy == 8 && d.npz == 8) e = launch_vjp_t<8, 8, 8>(d, a, st);
+    else if (d.npx == 6 && d.npy == 6 && d.npz == 6) e = launch_vjp_t<6, 6, 6>(d, a, st);
+    else if (d.npx == 6 && d.npy == 1 && d.npz == 6) e = launch_vjp_t<6, 1, 6>(d, a, st);
+    else if (d.npx == 5 && d.npy == 1 && d.npz == 5) e = launch_vjp_t<5, 1, 5>(d, a, st);
+    else e = launch_vjp_t<0, 0, 0>(d, a, st);
+    return e == hipSuccess ? e : fail(e);
 }
 
 }  // namespace nsem

@@ -394,6 +394,7 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         HIP_TRY(d->msurf_t.alloc((size_t)nslots * npfm * 5 * sizeof(double)));
         HIP_TRY(hipMemsetAsync(d->msurf_t.p, 0, (size_t)nslots * npfm * 5 * sizeof(double), s));
         HIP_TRY(d->madj.alloc(5 * nb * sizeof(double)));
+        HIP_TRY(hipMemsetAsync(d->madj.p, 0, 5 * nb * sizeof(double), s));
         if (prm->diffusion) HIP_TRY(d->mgrad.alloc((size_t)nslots * npfm * 12 * sizeof(double)));
     }
     HIP_TRY(d->bc.alloc(sizeof(BcTable)));
