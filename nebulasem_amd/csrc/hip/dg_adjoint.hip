@@ -124,27 +124,25 @@ __device__ __forceinline__ void rusanov_vjp_side(const Prim<double>& O, const Pr
     prim_to_conserved(S, rrS, prS, T0, gam, ub, thb, fS * phn, qb);
 }
 
-// Boundary face: the flux through the ghost state built from the owner
-// side, differentiated by forward-mode sweeps (kept out of line: rare).
-__device__ __noinline__ void ghost_face_vjp(const double QA[5], const double nvd[3], int kind, const double uv[3],
-                                            double rrA, double prA, double T0, double gam, double gR,
-                                            const double coef[5], double contrib[5]) {
+// Boundary face: the derivative of the flux through the ghost state built
+// from the owner side with respect to the owner's state component kk,
+// contracted with coef, by one forward-mode sweep.
+__device__ __forceinline__ double ghost_face_vjp(const double QA[5], const double nvd[3], int kind, const double uv[3],
+                                                 double rrA, double prA, double T0, double gam, double gR,
+                                                 const double coef[5], int kk) {
     const D1 nv[3] = {D1(nvd[0]), D1(nvd[1]), D1(nvd[2])};
-#pragma nounroll
-    for (int kk = 0; kk < 5; kk++) {
-        Prim<D1> A;
-        A.rp = seeded(QA[0], kk == 0);
-        A.m[0] = seeded(QA[1], kk == 1); A.m[1] = seeded(QA[2], kk == 2); A.m[2] = seeded(QA[3], kk == 3);
-        A.tp = seeded(QA[4], kk == 4);
-        make_prim(A, D1(rrA), D1(prA), D1(T0), D1(gam), D1(gR));
-        Prim<D1> N;
-        ghost_state(A, nv, kind, uv, D1(rrA), N);
-        D1 flux[5];
-        rusanov(A, N, nv, D1(0.0), D1(T0), flux);
-        double acc = 0;
-        for (int v = 0; v < 5; v++) acc += coef[v] * flux[v].d[0];
-        contrib[kk] = acc;
-    }
+    Prim<D1> A;
+    A.rp = seeded(QA[0], kk == 0);
+    A.m[0] = seeded(QA[1], kk == 1); A.m[1] = seeded(QA[2], kk == 2); A.m[2] = seeded(QA[3], kk == 3);
+    A.tp = seeded(QA[4], kk == 4);
+    make_prim(A, D1(rrA), D1(prA), D1(T0), D1(gam), D1(gR));
+    Prim<D1> N;
+    ghost_state(A, nv, kind, uv, D1(rrA), N);
+    D1 flux[5];
+    rusanov(A, N, nv, D1(0.0), D1(T0), flux);
+    double acc = 0;
+    for (int v = 0; v < 5; v++) acc += coef[v] * flux[v].d[0];
+    return acc;
 }
 
 // DIN: the launch reads a stage increment (compile-time, so the state loads
@@ -328,45 +326,46 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
 }
 
 // Boundary (ghost) sides of the vector-Jacobian product, ahead of the
-// element kernel: one work-group per ghost side, one thread per face node,
-// the flux through the ghost state differentiated by forward-mode sweeps
-// (ghost_face_vjp) with the element kernel's coefficient -w/cV, written to
-// gadj for the element kernel to add in its face order.  Kept out of the
-// element kernel, whose registers the sweeps' call would otherwise hold.
+// element kernel: one work-group per ghost side, five threads per face node
+// (one per state component), the flux through the ghost state
+// differentiated by a forward-mode sweep (ghost_face_vjp) and contracted
+// with the element kernel's coefficient -w/cV, written to gadj for the
+// element kernel to add in its face order.  Kept out of the element kernel,
+// whose registers the sweeps would otherwise hold.
 template <bool DIN>
-__global__ void __launch_bounds__(64) ghost_vjp_kernel(DgDev d, VjpArgs a) {
+__global__ void __launch_bounds__(512) ghost_vjp_kernel(DgDev d, VjpArgs a) {
     const int NX = d.npx, NY = d.npy, NZ = d.npz, NP = d.np;
+    const int np = d.npfm;
+    const int kk = threadIdx.x / np, t = threadIdx.x - kk * np;
     const uint32_t rec = d.ghost_rec[blockIdx.x];
     const uint32_t e = rec / 6;
     const int g = rec - e * 6;
     const int nbf = g < 2 ? NY : NZ, naf = g < 4 ? NX : NY;
+    if (kk >= 5 || t >= naf * nbf) return;
     const double T0 = d.T0, gam = d.gamma, gR = d.gamma * d.R;
-    for (int t = threadIdx.x; t < naf * nbf; t += blockDim.x) {
-        const int fa = t / nbf, fb = t - fa * nbf;
-        int i, j, k;
-        if (g < 2) { i = fa; j = fb; k = (g & 1) ? NZ - 1 : 0; }
-        else if (g < 4) { i = fa; j = (g & 1) ? NY - 1 : 0; k = fb; }
-        else { i = (g & 1) ? NX - 1 : 0; j = fa; k = fb; }
-        const uint64_t na = (uint64_t)e * NP + (uint64_t)(i * NY * NZ + j * NZ + k);
-        const EFace ef = d.eface[rec];
-        const bool own = (ef.meta >> 3) & 1;
-        const int og = ef.meta & 7;
-        const double wgt = face_weight(d, own ? g : og, fa, fb);
-        const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
-                               d.gFN[3 * (size_t)ef.facet + 2] * wgt};
-        const double cvn = d.cV[na];
-        double QA[5], coef[5], contrib[5];
-        for (int v = 0; v < 5; v++) {
-            QA[v] = a.q[v][na] + (DIN ? a.din[v][na] : 0.0);
-            coef[v] = -(a.w[v][na] / cvn);
-        }
-        const BcTable& bt = *d.bc;
-        const int pi = ef.meta >> 8;
-        const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
-        ghost_face_vjp(QA, nvd, bt.kind[1][pi], uv, d.rho_ref[na], d.p_ref[na], T0, gam, gR, coef, contrib);
-        double* ga = d.gadj + (size_t)blockIdx.x * 5 * d.npfm + t;
-        for (int v = 0; v < 5; v++) ga[v * d.npfm] = contrib[v];
+    const int fa = t / nbf, fb = t - fa * nbf;
+    int i, j, k;
+    if (g < 2) { i = fa; j = fb; k = (g & 1) ? NZ - 1 : 0; }
+    else if (g < 4) { i = fa; j = (g & 1) ? NY - 1 : 0; k = fb; }
+    else { i = (g & 1) ? NX - 1 : 0; j = fa; k = fb; }
+    const uint64_t na = (uint64_t)e * NP + (uint64_t)(i * NY * NZ + j * NZ + k);
+    const EFace ef = d.eface[rec];
+    const bool own = (ef.meta >> 3) & 1;
+    const int og = ef.meta & 7;
+    const double wgt = face_weight(d, own ? g : og, fa, fb);
+    const double nvd[3] = {d.gFN[3 * (size_t)ef.facet] * wgt, d.gFN[3 * (size_t)ef.facet + 1] * wgt,
+                           d.gFN[3 * (size_t)ef.facet + 2] * wgt};
+    const double cvn = d.cV[na];
+    double QA[5], coef[5];
+    for (int v = 0; v < 5; v++) {
+        QA[v] = a.q[v][na] + (DIN ? a.din[v][na] : 0.0);
+        coef[v] = -(a.w[v][na] / cvn);
     }
+    const BcTable& bt = *d.bc;
+    const int pi = ef.meta >> 8;
+    const double uv[3] = {bt.value[1][pi], bt.value[2][pi], bt.value[3][pi]};
+    d.gadj[((size_t)blockIdx.x * 5 + kk) * np + t] =
+        ghost_face_vjp(QA, nvd, bt.kind[1][pi], uv, d.rho_ref[na], d.p_ref[na], T0, gam, gR, coef, kk);
 }
 
 // Vector-Jacobian product of the mortar faces' share of the residual (the
@@ -582,8 +581,10 @@ hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
         if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     }
     if (d.nghost) {
-        if (a.din[0]) hipLaunchKernelGGL(ghost_vjp_kernel<true>, dim3(d.nghost), dim3(64), 0, st, d, a);
-        else hipLaunchKernelGGL(ghost_vjp_kernel<false>, dim3(d.nghost), dim3(64), 0, st, d, a);
+        if (5 * d.npfm > 512) return fail(hipErrorInvalidValue);
+        const int gthreads = (5 * d.npfm + 63) / 64 * 64;
+        if (a.din[0]) hipLaunchKernelGGL(ghost_vjp_kernel<true>, dim3(d.nghost), dim3(gthreads), 0, st, d, a);
+        else hipLaunchKernelGGL(ghost_vjp_kernel<false>, dim3(d.nghost), dim3(gthreads), 0, st, d, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return fail(e);
     }
