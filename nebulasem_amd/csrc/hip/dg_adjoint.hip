@@ -147,6 +147,32 @@ __device__ __forceinline__ double ghost_face_vjp(const double QA[5], const doubl
 
 // DIN: the launch reads a stage increment (compile-time, so the state loads
 // carry no null-pointer branches and can be issued together).
+// The primitives of element node t from LDS: Q rows, and with PL the
+// derived rows too; without PL they are recomputed.
+template <bool PL>
+__device__ __forceinline__ void load_prim(Prim<double>& P, const double* Ql, const double* Xl, int NP, int t,
+                                          double rref, double pref, double T0, double gam, double gR) {
+    P.rp = Ql[t]; P.m[0] = Ql[NP + t]; P.m[1] = Ql[2 * NP + t]; P.m[2] = Ql[3 * NP + t]; P.tp = Ql[4 * NP + t];
+    if constexpr (PL) {
+        P.u[0] = Xl[t]; P.u[1] = Xl[NP + t]; P.u[2] = Xl[2 * NP + t]; P.thp = Xl[3 * NP + t];
+        P.pp = Xl[4 * NP + t]; P.c = Xl[5 * NP + t]; P.umag = Xl[6 * NP + t];
+    } else {
+        make_prim(P, rref, pref, T0, gam, gR);
+    }
+}
+
+// Whether the VJP kernel for a compile-time element shape keeps the
+// primitives (u, theta', p', c, |U|) of its nodes in LDS beside Q and w/cV,
+// so the face tasks take their own node's primitives from there instead of
+// recomputing them: when a work-group's LDS (vjp_epb elements) still lets
+// three work-groups share a CU's 160 KiB.
+__host__ __device__ constexpr bool vjp_prims_fit(int nx, int ny, int nz) {
+    return nx && ny && nz &&
+           ((nx * ny * nz >= 256 ? 1 : 256 / (nx * ny * nz)) *
+                (17 * nx * ny * nz + 10 * (nx * ny + nx * nz + ny * nz)) +
+            nx * nx + ny * ny + nz * nz) * 8 * 3 <= 160 * 1024;
+}
+
 // O32: node byte offsets formed in 32 bits (nb * 8 below 4 GiB).
 template <int NXc, int NYc, int NZc, bool DIN, bool O32>
 __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
@@ -158,10 +184,12 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds_v[];
-    const int per_elem = 10 * NP + 5 * NFN;
+    constexpr bool PL = vjp_prims_fit(NXc, NYc, NZc);
+    const int per_elem = (PL ? 17 : 10) * NP + 5 * NFN;
     double* Wl = lds_v + (size_t)le * per_elem;  // w / cV      [5][NP]
     double* Ql = Wl + 5 * NP;                    // Q = q + din [5][NP]
-    double* FL = Ql + 5 * NP;                    // face terms  [5][NFN]
+    double* Xl = Ql + 5 * NP;                    // PL: u, theta', p', c, |U| [7][NP]
+    double* FL = Xl + (PL ? 7 : 0) * NP;         // face terms  [5][NFN]
     double* Dl = lds_v + (size_t)epb * per_elem;
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
@@ -177,10 +205,19 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     double wn[5] = {0, 0, 0, 0, 0};
     if (active) {
         const double cvn = ldn<O32>(d.cV, n);
+        double Qn[5];
         for (int v = 0; v < 5; v++) {
             wn[v] = ldn<O32>(a.w[v], n);
             Wl[v * NP + q] = wn[v] / cvn;
-            Ql[v * NP + q] = ldn<O32>(a.q[v], n) + (DIN ? ldn<O32>(a.din[v], n) : 0.0);
+            Qn[v] = ldn<O32>(a.q[v], n) + (DIN ? ldn<O32>(a.din[v], n) : 0.0);
+            Ql[v * NP + q] = Qn[v];
+        }
+        if constexpr (PL) {
+            Prim<double> P;
+            P.rp = Qn[0]; P.m[0] = Qn[1]; P.m[1] = Qn[2]; P.m[2] = Qn[3]; P.tp = Qn[4];
+            make_prim(P, ldn<O32>(d.rho_ref, n), ldn<O32>(d.p_ref, n), T0, gam, gR);
+            const double x[7] = {P.u[0], P.u[1], P.u[2], P.thp, P.pp, P.c, P.umag};
+            for (int c = 0; c < 7; c++) Xl[c * NP + q] = x[c];
         }
     }
     __syncthreads();
@@ -208,9 +245,7 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
                     const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
                     const double cvm = ldn<O32>(d.cV, m);
                     Prim<double> Ap, Bp;
-                    Ap.rp = Ql[fnode]; Ap.m[0] = Ql[NP + fnode]; Ap.m[1] = Ql[2 * NP + fnode];
-                    Ap.m[2] = Ql[3 * NP + fnode]; Ap.tp = Ql[4 * NP + fnode];
-                    make_prim(Ap, rrA, prA, T0, gam, gR);
+                    load_prim<PL>(Ap, Ql, Xl, NP, fnode, rrA, prA, T0, gam, gR);
                     Bp.rp = ldn<O32>(a.q[0], m) + (DIN ? ldn<O32>(a.din[0], m) : 0.0);
                     Bp.m[0] = ldn<O32>(a.q[1], m) + (DIN ? ldn<O32>(a.din[1], m) : 0.0);
                     Bp.m[1] = ldn<O32>(a.q[2], m) + (DIN ? ldn<O32>(a.din[2], m) : 0.0);
@@ -251,8 +286,7 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
         }
         const double rref = ldn<O32>(d.rho_ref, n), pref = ldn<O32>(d.p_ref, n);
         Prim<double> P;
-        P.rp = Ql[q]; P.m[0] = Ql[NP + q]; P.m[1] = Ql[2 * NP + q]; P.m[2] = Ql[3 * NP + q]; P.tp = Ql[4 * NP + q];
-        make_prim(P, rref, pref, T0, gam, gR);
+        load_prim<PL>(P, Ql, Xl, NP, q, rref, pref, T0, gam, gR);
         const double* D0 = Dl;
         const double* Dy = Dl + NX * NX;
         const double* Dz = Dy + NY * NY;
@@ -536,11 +570,12 @@ template <int X, int Y, int Z>
 hipError_t launch_vjp_t(const DgDev& d, const VjpArgs& a_in, hipStream_t st) {
     VjpArgs a = a_in;
     const int NP = d.np;
+    constexpr bool pl = vjp_prims_fit(X, Y, Z);
     int epb = vjp_epb(NP);
-    while (epb > 1 && rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb) > 65536) epb--;
+    while (epb > 1 && rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb, pl) > 65536) epb--;
     const int threads = (epb * NP + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
-    const size_t lds = rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb);
+    const size_t lds = rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb, pl);
     if (NP > 512 || lds > 65536) return hipErrorInvalidValue;
     a.epb = epb;
     // node byte offsets below 4 GiB (each array is addressed from its own
@@ -558,9 +593,9 @@ hipError_t launch_vjp_t(const DgDev& d, const VjpArgs& a_in, hipStream_t st) {
 
 }  // namespace
 
-size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb) {
+size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb, bool pl) {
     const int np = nx * ny * nz, nfn = 2 * (nx * ny + nx * nz + ny * nz);
-    return ((size_t)epb * (10 * np + 5 * nfn) + nx * nx + ny * ny + nz * nz) * sizeof(double);
+    return ((size_t)epb * ((pl ? 17 : 10) * np + 5 * nfn) + nx * nx + ny * ny + nz * nz) * sizeof(double);
 }
 
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
@@ -716,7 +751,7 @@ int adjoint_step(nsem_dg* d, const double* const q[5], const double* const d1[5]
 int check_adjoint_support(const nsem_dg* d) {
     if (int rc = nsem_mol_supported(d)) return rc;
     const DgDev& v = d->dev;
-    if (v.np > 512 || rk_vjp_lds_bytes(v.npx, v.npy, v.npz, 1) > 65536)
+    if (v.np > 512 || rk_vjp_lds_bytes(v.npx, v.npy, v.npz, 1, false) > 65536)
         return nsem_fail(NSEM_EUNSUPPORTED, "element too large for the adjoint kernel");
     return NSEM_OK;
 }

@@ -73,7 +73,7 @@ struct VjpArgs {
     double dt, c1, c2;
 };
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st);
-size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb);
+size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb, bool prims_in_lds);
 
 
 // LDS of the RK stage kernel, in values of the residual type: per element
