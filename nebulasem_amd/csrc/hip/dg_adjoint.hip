@@ -58,6 +58,31 @@ __device__ __forceinline__ double dpp_dtp(const Prim<double>& P, double rref, do
     return gam * (pref + P.pp) / (rref * T0 + P.tp);
 }
 
+// The primitives of one node as dual numbers, seeded in conserved
+// component k (rho', m_x, m_y, m_z, Theta'), from their values P: the
+// derivatives of make_prim written out (d(1/rho), the velocity, theta', p'
+// through dpp_dtp, c and |U| with the zero-at-zero convention of r_sqrt).
+__device__ __forceinline__ void prim_tangent(const Prim<double>& P, double rref, double pref, double T0, double gam,
+                                             double gR, int k, Prim<Dual<1>>& T) {
+    const double ir = 1.0 / (rref + P.rp);
+    const double drp = k == 0 ? 1.0 : 0.0, dtp = k == 4 ? 1.0 : 0.0;
+    const double dm[3] = {k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0, k == 3 ? 1.0 : 0.0};
+    const double dir = -ir * ir * drp;
+    T.rp.v = P.rp; T.rp.d[0] = drp;
+    T.tp.v = P.tp; T.tp.d[0] = dtp;
+    double udu = 0.0;
+    for (int c = 0; c < 3; c++) {
+        T.m[c].v = P.m[c]; T.m[c].d[0] = dm[c];
+        T.u[c].v = P.u[c]; T.u[c].d[0] = dm[c] * ir + P.m[c] * dir;
+        udu += P.u[c] * T.u[c].d[0];
+    }
+    const double dthp = (dtp - T0 * drp) * ir + (P.tp - T0 * P.rp) * dir;
+    T.thp.v = P.thp; T.thp.d[0] = dthp;
+    T.pp.v = P.pp; T.pp.d[0] = dpp_dtp(P, rref, pref, T0, gam) * dtp;
+    T.c.v = P.c; T.c.d[0] = P.c > 0.0 ? 0.5 * gR * dthp / P.c : 0.0;
+    T.umag.v = P.umag; T.umag.d[0] = P.umag > 0.0 ? udu / P.umag : 0.0;
+}
+
 // Cotangent of the conserved state (rho', m, Theta') from cotangents of the
 // primitives u, theta and p' (u = m/rho, theta = (Theta' + rho_ref T0)/rho).
 __device__ __forceinline__ void prim_to_conserved(const Prim<double>& P, double rref, double pref, double T0,
@@ -442,20 +467,25 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     const double* C0 = d.psiCor[d0] + (size_t)mf.h0 * mf.n0 * mf.n0;
     const double* C1 = d.psiCor[d1] + (size_t)mf.h1 * mf.n1 * mf.n1;
     auto Q = [&](uint64_t n, int v) { return a.q[v][n] + (a.din[v] ? a.din[v][n] : 0.0); };
-    auto values = [&](uint64_t n, double* v) {
+    // a node's primitives (kept in LDS rows PL[12][np]) and its 12 values
+    auto values = [&](uint64_t n, double* v, double* PL) {
         Prim<double> P;
         P.rp = Q(n, 0); P.m[0] = Q(n, 1); P.m[1] = Q(n, 2); P.m[2] = Q(n, 3); P.tp = Q(n, 4);
         make_prim(P, d.rho_ref[n], d.p_ref[n], T0, gam, gR);
         mortar_node_values(P, nv, T0, v);
+        const double x[12] = {P.rp, P.m[0], P.m[1], P.m[2], P.tp, P.u[0], P.u[1], P.u[2], P.thp, P.pp, P.c, P.umag};
+        for (int c = 0; c < 12; c++) PL[c * np + t] = x[c];
     };
-    // the node values' derivatives with respect to the node's state k
-    auto values_d = [&](uint64_t n, int k, D1* v) {
-        Prim<D1> P;
-        P.rp = seeded(Q(n, 0), k == 0);
-        P.m[0] = seeded(Q(n, 1), k == 1); P.m[1] = seeded(Q(n, 2), k == 2); P.m[2] = seeded(Q(n, 3), k == 3);
-        P.tp = seeded(Q(n, 4), k == 4);
-        make_prim(P, D1(d.rho_ref[n]), D1(d.p_ref[n]), D1(T0), D1(gam), D1(gR));
-        mortar_node_values(P, nvD, D1(T0), v);
+    // the node values' derivatives with respect to the node's state k, from
+    // the primitives in PL
+    auto values_d = [&](uint64_t n, int k, D1* v, const double* PL) {
+        Prim<double> P;
+        P.rp = PL[t]; P.m[0] = PL[np + t]; P.m[1] = PL[2 * np + t]; P.m[2] = PL[3 * np + t]; P.tp = PL[4 * np + t];
+        P.u[0] = PL[5 * np + t]; P.u[1] = PL[6 * np + t]; P.u[2] = PL[7 * np + t]; P.thp = PL[8 * np + t];
+        P.pp = PL[9 * np + t]; P.c = PL[10 * np + t]; P.umag = PL[11 * np + t];
+        Prim<D1> T;
+        prim_tangent(P, d.rho_ref[n], d.p_ref[n], T0, gam, gR, k, T);
+        mortar_node_values(T, nvD, D1(T0), v);
     };
     extern __shared__ double mortar_adj_lds[];
     double* Lv = mortar_adj_lds;   // [12][np] coarse node values
@@ -464,7 +494,9 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     double* Lf = Lp + 12 * np;     // [12][np] refined node values
     double* Lc = Lf + 12 * np;     // [12][np] interpolated coarse values; then gbar
     double* Lb = Lc + 12 * np;     // [5][np]  cotangent of phi
-    double* LR0 = Lb + 5 * np;     // psiRef and psiCor of the face's half, n0^2 and n1^2 each
+    double* Pc = Lb + 5 * np;      // [12][np] primitives of the coarse side nodes
+    double* Pf = Pc + 12 * np;     // [12][np] primitives of the refined nodes
+    double* LR0 = Pf + 12 * np;    // psiRef and psiCor of the face's half, n0^2 and n1^2 each
     double* LR1 = LR0 + mf.n0 * mf.n0;
     double* LC0 = LR1 + mf.n1 * mf.n1;
     double* LC1 = LC0 + mf.n0 * mf.n0;
@@ -473,7 +505,7 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     const uint64_t nc = face_node(mf.coarse, mf.side, ao, bo, NX, NY, NZ);
     if (act && sub == 0) {
         double v[12];
-        values(nc, v);
+        values(nc, v, Pc);
         for (int q = 0; q < 12; q++) Lv[q * np + t] = v[q];
         const double sc = fo ? d.mort_w[ft] : -d.mort_w[ft];
         const double icv = 1.0 / d.cV[nc];
@@ -484,7 +516,7 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     if (act) {
         if (sub == 0) {
             double vf[12];
-            values(nf, vf);
+            values(nf, vf, Pf);
             for (int q = 0; q < 12; q++) Lf[q * np + t] = vf[q];
         }
         // interpolated values q = sub, sub + 5, sub + 10
@@ -517,7 +549,7 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
             // the refined node's state component k = sub
             const int k = sub;
             D1 vfD[12], vcD[12], phi[5];
-            values_d(nf, k, vfD);
+            values_d(nf, k, vfD, Pf);
             for (int q = 0; q < 12; q++) vcD[q] = D1(Lc[q * np + t]);
             mortar_rusanov(vfD, vcD, bg, nvD, fo, D1(T0), phi);
             double acc = 0;
@@ -557,7 +589,7 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     {
         const int k = sub;
         D1 v[12];
-        values_d(nc, k, v);
+        values_d(nc, k, v, Pc);
         double acc = 0;
         for (int q = 0; q < 12; q++) acc += Lc[q * np + t] * v[q].d[0];
         atomicAdd(d.madj + (size_t)k * nb + nc, acc);
@@ -609,9 +641,8 @@ hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st) {
         if (kMortarSub * d.npfm > 512) return hipErrorInvalidValue;
         hipError_t e;
         const int threads = (kMortarSub * d.npfm + 63) / 64 * 64;
-        // LDS: 58 rows of npfm values and the four 1-D matrices (at most
-        // 4 x 16 x 16 for the largest supported element)
-        const size_t lds = ((size_t)58 * d.npfm + 2 * (d.npx * d.npx + d.npy * d.npy + d.npz * d.npz)) * sizeof(double);
+        // LDS: 82 rows of npfm values and the four 1-D matrices
+        const size_t lds = ((size_t)82 * d.npfm + 2 * (d.npx * d.npx + d.npy * d.npy + d.npz * d.npz)) * sizeof(double);
         hipLaunchKernelGGL(mortar_vjp_kernel, dim3(d.nmort), dim3(threads), lds, st, d, a);
         if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     }
