@@ -117,10 +117,18 @@ template <int K>
 __device__ __forceinline__ Dual<K> r_div(const Dual<K>& a, const Dual<K>& b) { return a / b; }
 
 // p' / p_ref = (1 + x)^gamma - 1 with x = Theta' / (rho_ref T0), computed as
-// expm1(gamma log1p(x)) (no cancellation).  In FP32, for |x| <= 1/32 the
-// binomial series to x^6 is used instead (truncation below 1e-10 relative,
-// under FP32 rounding), which is a few FMAs instead of two libm calls.
-__device__ __forceinline__ double pow1p_m1(double x, double gam) { return expm1(gam * log1p(x)); }
+// expm1(gamma log1p(x)) (no cancellation).  For small x the binomial series
+// sum_k binom(gamma, k) x^k is used instead, a few multiply-adds in place of
+// two libm calls: in FP64 to x^8 for |x| <= 1/64 (for gamma in (1, 2) the
+// first omitted term is below 1e-17 of the sum), in FP32 to x^6 for
+// |x| <= 1/32 (below 1e-10, under FP32 rounding).
+__device__ __forceinline__ double pow1p_m1(double x, double gam) {
+    if (fabs(x) > 0.015625) return expm1(gam * log1p(x));
+    const double c1 = gam, c2 = c1 * (gam - 1.0) * 0.5, c3 = c2 * (gam - 2.0) * (1.0 / 3.0);
+    const double c4 = c3 * (gam - 3.0) * 0.25, c5 = c4 * (gam - 4.0) * 0.2, c6 = c5 * (gam - 5.0) * (1.0 / 6.0);
+    const double c7 = c6 * (gam - 6.0) * (1.0 / 7.0), c8 = c7 * (gam - 7.0) * 0.125;
+    return x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * (c6 + x * (c7 + x * c8)))))));
+}
 __device__ __forceinline__ float pow1p_m1(float x, float gam) {
     if (fabsf(x) > 0.03125f) return expm1f(gam * log1pf(x));
     const float c1 = gam, c2 = c1 * (gam - 1.0f) * 0.5f, c3 = c2 * (gam - 2.0f) * (1.0f / 3.0f);
