@@ -23,9 +23,11 @@
 // Step level (SSP-RK3 in increment form, d1 = dt L(q), d2 = (d1 + dt L(q+d1))/4,
 // q+ = q + 2(d2 + dt L(q+d2))/3), backwards, each line one VJP launch with
 // the algebra fused into its epilogue:
-//   t3 = J(q+d2)^T lam;  ad2 = 2/3 (lam + dt t3);  acc = lam + 2/3 dt t3
-//   t2 = J(q+d1)^T ad2;  ad1 = 1/4 (ad2 + dt t2);  acc += 1/4 dt t2
-//   t1 = J(q)^T ad1;                               lam = acc + dt t1
+//   t3 = J(q+d2)^T lam;  ad2 = 2/3 (lam + dt t3)
+//   t2 = J(q+d1)^T ad2;  ad1 = 1/4 (ad2 + dt t2)
+//   t1 = J(q)^T ad1;     lam <- lam + 2/3 dt t3 + 1/4 dt t2 + dt t1
+//                             = 1/3 lam + 3/4 ad2 + ad1 + dt t1
+// (the last form needs no accumulator array between the launches)
 // with the forward states recomputed from checkpoints stored every
 // `checkpoint` steps, one segment at a time (its states and stage increments
 // held while the segment is swept backwards).
@@ -376,11 +378,9 @@ __global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
     for (int v = 0; v < 5; v++) {
         if (a.t[0]) stv<O32>(a.t[v], n, qbar[v]);
         if (a.ad_out[0]) stv<O32>(a.ad_out[v], n, a.c1 * (wn[v] + a.dt * qbar[v]));
-        if (a.acc_out[0]) {
-            // the last RK3 stage's accumulator starts from w itself
-            const double ain = a.acc_in[0] == a.w[0] ? wn[v] : ldn<O32>(a.acc_in[v], n);
-            stv<O32>(a.acc_out[v], n, ain + a.c2 * a.dt * qbar[v]);
-        }
+        if (a.comb_out[0])
+            stv<O32>(a.comb_out[v], n,
+                     a.k0 * ldn<O32>(a.y0[v], n) + a.k1 * ldn<O32>(a.y1[v], n) + wn[v] + a.dt * qbar[v]);
     }
 }
 
@@ -746,7 +746,7 @@ bool all_finite(const double* v, uint64_t n) {
 }
 
 struct AdjointWork {
-    Field5 ad1, ad2, acc;
+    Field5 ad1, ad2;
 };
 
 void ptrs(const Field5& f, double* out[5]) { for (int v = 0; v < 5; v++) out[v] = f.p(v); }
@@ -756,26 +756,28 @@ void cptrs(const Field5& f, const double* out[5]) { for (int v = 0; v < 5; v++) 
 // d1, d2 were stored by the forward recomputation.
 int adjoint_step(nsem_dg* d, const double* const q[5], const double* const d1[5], const double* const d2[5],
                  Field5& lam, AdjointWork& w) {
-    double *ad1[5], *ad2[5], *acc[5], *lp[5];
-    ptrs(w.ad1, ad1); ptrs(w.ad2, ad2); ptrs(w.acc, acc); ptrs(lam, lp);
+    double *ad1[5], *ad2[5], *lp[5];
+    ptrs(w.ad1, ad1); ptrs(w.ad2, ad2); ptrs(lam, lp);
     const double dt = d->dev.dt;
-    auto vjp = [&](const double* const din[5], double* const wv[5], double* const ad_out[5],
-                   double* const acc_in[5], double* const acc_out[5], double c1, double c2) -> hipError_t {
+    auto vjp = [&](const double* const din[5], double* const wv[5], double* const ad_out[5], double c1,
+                   bool last) -> hipError_t {
         VjpArgs a{};
         for (int v = 0; v < 5; v++) {
             a.q[v] = q[v]; a.din[v] = din ? din[v] : nullptr; a.w[v] = wv[v];
             a.ad_out[v] = ad_out ? ad_out[v] : nullptr;
-            a.acc_in[v] = acc_in[v]; a.acc_out[v] = acc_out[v];
+            if (last) { a.y0[v] = lp[v]; a.y1[v] = ad2[v]; a.comb_out[v] = lp[v]; }
         }
-        a.dt = dt; a.c1 = c1; a.c2 = c2;
+        a.dt = dt; a.c1 = c1;
+        a.k0 = 1.0 / 3.0; a.k1 = 0.75;
         return launch_rk_vjp(d->dev, a, d->stream);
     };
-    // t3 = J(q+d2)^T lam: ad2 = 2/3 (lam + dt t3), acc = lam + 2/3 dt t3
-    HIP_TRY(vjp(d2, lp, ad2, lp, acc, 2.0 / 3.0, 2.0 / 3.0));
-    // t2 = J(q+d1)^T ad2: ad1 = 1/4 (ad2 + dt t2), acc += 1/4 dt t2
-    HIP_TRY(vjp(d1, ad2, ad1, acc, acc, 0.25, 0.25));
-    // t1 = J(q)^T ad1: lam = acc + dt t1
-    HIP_TRY(vjp(nullptr, ad1, nullptr, acc, lp, 0.0, 1.0));
+    // t3 = J(q+d2)^T lam: ad2 = 2/3 (lam + dt t3)
+    HIP_TRY(vjp(d2, lp, ad2, 2.0 / 3.0, false));
+    // t2 = J(q+d1)^T ad2: ad1 = 1/4 (ad2 + dt t2)
+    HIP_TRY(vjp(d1, ad2, ad1, 0.25, false));
+    // t1 = J(q)^T ad1: lam + 2/3 dt t3 + 1/4 dt t2 + dt t1
+    //                = 1/3 lam + 3/4 ad2 + ad1 + dt t1
+    HIP_TRY(vjp(nullptr, ad1, nullptr, 0.0, true));
     return NSEM_OK;
 }
 
@@ -901,13 +903,13 @@ static int rk3_adjoint(nsem_dg* d, int32_t nsteps, int32_t checkpoint, double* l
     // checkpoints; one segment's states and each step's stage increments
     // d1, d2; the adjoint and its stage work arrays
     std::vector<Field5> ws;
-    HIP_TRY(workspace(d, (size_t)nseg + 3 * (size_t)checkpoint + 4, ws));
+    HIP_TRY(workspace(d, (size_t)nseg + 3 * (size_t)checkpoint + 3, ws));
     const Field5* ck = ws.data();
     const Field5* seg = ck + nseg;
     const Field5* sd1 = seg + checkpoint;
     const Field5* sd2 = sd1 + checkpoint;
     Field5 L = sd2[checkpoint];
-    AdjointWork w{sd2[checkpoint + 1], sd2[checkpoint + 2], sd2[checkpoint + 3]};
+    AdjointWork w{sd2[checkpoint + 1], sd2[checkpoint + 2]};
     HIP_TRY(upload5(L, lam, nb, d->stream));
     if (seconds) HIP_TRY(hipEventRecord(d->ev0, d->stream));
     // one segment forward from its first state seg[0], recording each step's

@@ -57,10 +57,11 @@ hipError_t launch_mol_gradient(const DgDev& d, const double* const q[5], const S
 // Vector-Jacobian product of the method-of-lines residual at Q = q + din:
 // t = (dL/dQ)^T w, all FP64 SoA node arrays (din may be null).
 // Epilogue (the adjoint's stage algebra, fused): with t = J^T w at a node,
-//   t_out = t                        (when t_out is set)
-//   ad_out = c1 (w + dt t)           (when ad_out is set)
-//   acc_out = acc_in + c2 dt t       (when acc_out is set)
-// ad_out and acc_out must not alias w (other work-groups read w).
+//   t_out = t                                  (when t_out is set)
+//   ad_out = c1 (w + dt t)                     (when ad_out is set)
+//   comb_out = k0 y0 + k1 y1 + w + dt t        (when comb_out is set)
+// ad_out and comb_out must not alias w (other work-groups read w); comb_out
+// may alias y0 or y1 (each node reads its own entry before writing it).
 struct VjpArgs {
     int epb;      // elements per work-group (set by the launcher)
     const double* q[5];
@@ -68,9 +69,10 @@ struct VjpArgs {
     const double* w[5];
     double* t[5];
     double* ad_out[5];
-    const double* acc_in[5];
-    double* acc_out[5];
-    double dt, c1, c2;
+    const double* y0[5];
+    const double* y1[5];
+    double* comb_out[5];
+    double dt, c1, k0, k1;
 };
 hipError_t launch_rk_vjp(const DgDev& d, const VjpArgs& a, hipStream_t st);
 size_t rk_vjp_lds_bytes(int nx, int ny, int nz, int epb, bool prims_in_lds);
