@@ -138,7 +138,20 @@ __device__ __forceinline__ float pow1p_m1(float x, float gam) {
 }
 template <int K>
 __device__ __forceinline__ Dual<K> pow1p_m1(const Dual<K>& x, const Dual<K>& gam) {
-    return r_expm1(gam * r_log1p(x));
+    if (fabs(x.v) > 0.015625) return r_expm1(gam * r_log1p(x));
+    // the FP64 series and its derivative sum_k k binom(gamma, k) x^(k-1) dx
+    // (gamma a constant here, as in every caller)
+    const double g = gam.v;
+    const double c1 = g, c2 = c1 * (g - 1.0) * 0.5, c3 = c2 * (g - 2.0) * (1.0 / 3.0);
+    const double c4 = c3 * (g - 3.0) * 0.25, c5 = c4 * (g - 4.0) * 0.2, c6 = c5 * (g - 5.0) * (1.0 / 6.0);
+    const double c7 = c6 * (g - 6.0) * (1.0 / 7.0), c8 = c7 * (g - 7.0) * 0.125;
+    const double t = x.v;
+    Dual<K> r;
+    r.v = t * (c1 + t * (c2 + t * (c3 + t * (c4 + t * (c5 + t * (c6 + t * (c7 + t * c8)))))));
+    const double dv =
+        c1 + t * (2 * c2 + t * (3 * c3 + t * (4 * c4 + t * (5 * c5 + t * (6 * c6 + t * (7 * c7 + t * (8 * c8)))))));
+    for (int k = 0; k < K; k++) r.d[k] = dv * x.d[k];
+    return r;
 }
 
 __device__ __forceinline__ double value_of(double x) { return x; }
