@@ -200,9 +200,11 @@ __host__ __device__ constexpr bool vjp_prims_fit(int nx, int ny, int nz) {
             nx * nx + ny * ny + nz * nz) * 8 * 3 <= 160 * 1024;
 }
 
-// O32: node byte offsets formed in 32 bits (nb * 8 below 4 GiB).
+// O32: node byte offsets formed in 32 bits (nb * 8 below 4 GiB).  At most
+// 168 VGPRs, three wavefronts per SIMD (the compiler would otherwise take
+// up to 176 on the 2D shapes and fall to two).
 template <int NXc, int NYc, int NZc, bool DIN, bool O32>
-__global__ void __launch_bounds__(512) rk_vjp_kernel(DgDev d, VjpArgs a) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) rk_vjp_kernel(DgDev d, VjpArgs a) {
     const int NX = NXc ? NXc : d.npx, NY = NYc ? NYc : d.npy, NZ = NZc ? NZc : d.npz;
     const int NP = NX * NY * NZ;
     const int NFN = 2 * (NX * NY + NX * NZ + NY * NZ);
@@ -603,7 +605,9 @@ hipError_t launch_vjp_t(const DgDev& d, const VjpArgs& a_in, hipStream_t st) {
     VjpArgs a = a_in;
     const int NP = d.np;
     constexpr bool pl = vjp_prims_fit(X, Y, Z);
-    int epb = vjp_epb(NP);
+    // small meshes: fewer elements per work-group, so that every CU gets
+    // about four work-groups
+    int epb = std::max(1, std::min(vjp_epb(NP), (int)(d.nelem / 1024)));
     while (epb > 1 && rk_vjp_lds_bytes(d.npx, d.npy, d.npz, epb, pl) > 65536) epb--;
     const int threads = (epb * NP + 63) / 64 * 64;
     const unsigned blocks = (d.nelem + epb - 1) / epb;
