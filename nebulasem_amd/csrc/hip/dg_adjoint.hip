@@ -178,13 +178,14 @@ __device__ __forceinline__ double ghost_face_vjp(const double QA[5], const doubl
 // derived rows too; without PL they are recomputed.
 template <bool PL>
 __device__ __forceinline__ void load_prim(Prim<double>& P, const double* Ql, const double* Xl, int NP, int t,
-                                          double rref, double pref, double T0, double gam, double gR) {
+                                          double rref, double pref, double T0, double gam, double gR,
+                                          const double* ppc) {
     P.rp = Ql[t]; P.m[0] = Ql[NP + t]; P.m[1] = Ql[2 * NP + t]; P.m[2] = Ql[3 * NP + t]; P.tp = Ql[4 * NP + t];
     if constexpr (PL) {
         P.u[0] = Xl[t]; P.u[1] = Xl[NP + t]; P.u[2] = Xl[2 * NP + t]; P.thp = Xl[3 * NP + t];
         P.pp = Xl[4 * NP + t]; P.c = Xl[5 * NP + t]; P.umag = Xl[6 * NP + t];
     } else {
-        make_prim(P, rref, pref, T0, gam, gR);
+        make_prim(P, rref, pref, T0, gam, gR, ppc);
     }
 }
 
@@ -244,7 +245,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) r
         if constexpr (PL) {
             Prim<double> P;
             P.rp = Qn[0]; P.m[0] = Qn[1]; P.m[1] = Qn[2]; P.m[2] = Qn[3]; P.tp = Qn[4];
-            make_prim(P, ldn<O32>(d.rho_ref, n), ldn<O32>(d.p_ref, n), T0, gam, gR);
+            make_prim(P, ldn<O32>(d.rho_ref, n), ldn<O32>(d.p_ref, n), T0, gam, gR, d.ppc);
             const double x[7] = {P.u[0], P.u[1], P.u[2], P.thp, P.pp, P.c, P.umag};
             for (int c = 0; c < 7; c++) Xl[c * NP + q] = x[c];
         }
@@ -274,13 +275,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) r
                     const uint64_t m = face_node(ef.other, og, fa, fb, NX, NY, NZ);
                     const double cvm = ldn<O32>(d.cV, m);
                     Prim<double> Ap, Bp;
-                    load_prim<PL>(Ap, Ql, Xl, NP, fnode, rrA, prA, T0, gam, gR);
+                    load_prim<PL>(Ap, Ql, Xl, NP, fnode, rrA, prA, T0, gam, gR, d.ppc);
                     Bp.rp = ldn<O32>(a.q[0], m) + (DIN ? ldn<O32>(a.din[0], m) : 0.0);
                     Bp.m[0] = ldn<O32>(a.q[1], m) + (DIN ? ldn<O32>(a.din[1], m) : 0.0);
                     Bp.m[1] = ldn<O32>(a.q[2], m) + (DIN ? ldn<O32>(a.din[2], m) : 0.0);
                     Bp.m[2] = ldn<O32>(a.q[3], m) + (DIN ? ldn<O32>(a.din[3], m) : 0.0);
                     Bp.tp = ldn<O32>(a.q[4], m) + (DIN ? ldn<O32>(a.din[4], m) : 0.0);
-                    make_prim(Bp, ldn<O32>(d.rho_ref, m), ldn<O32>(d.p_ref, m), T0, gam, gR);
+                    make_prim(Bp, ldn<O32>(d.rho_ref, m), ldn<O32>(d.p_ref, m), T0, gam, gR, d.ppc);
                     const double s = own ? -1.0 : 1.0;
                     double coef[5];
                     for (int v = 0; v < 5; v++) coef[v] = s * (Wl[v * NP + fnode] - ldn<O32>(a.w[v], m) / cvm);
@@ -315,7 +316,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) r
         }
         const double rref = ldn<O32>(d.rho_ref, n), pref = ldn<O32>(d.p_ref, n);
         Prim<double> P;
-        load_prim<PL>(P, Ql, Xl, NP, q, rref, pref, T0, gam, gR);
+        load_prim<PL>(P, Ql, Xl, NP, q, rref, pref, T0, gam, gR, d.ppc);
         const double* D0 = Dl;
         const double* Dy = Dl + NX * NX;
         const double* Dz = Dy + NY * NY;
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(512) mortar_vjp_kernel(DgDev d, VjpArgs a) {
     auto values = [&](uint64_t n, double* v, double* PL) {
         Prim<double> P;
         P.rp = Q(n, 0); P.m[0] = Q(n, 1); P.m[1] = Q(n, 2); P.m[2] = Q(n, 3); P.tp = Q(n, 4);
-        make_prim(P, d.rho_ref[n], d.p_ref[n], T0, gam, gR);
+        make_prim(P, d.rho_ref[n], d.p_ref[n], T0, gam, gR, d.ppc);
         mortar_node_values(P, nv, T0, v);
         const double x[12] = {P.rp, P.m[0], P.m[1], P.m[2], P.tp, P.u[0], P.u[1], P.u[2], P.thp, P.pp, P.c, P.umag};
         for (int c = 0; c < 12; c++) PL[c * np + t] = x[c];

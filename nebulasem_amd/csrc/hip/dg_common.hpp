@@ -57,6 +57,9 @@ struct DgDev {
     uint32_t nelem;                   // interior elements
     uint64_t nb;                      // interior nodes
     double dt, P0, T0, R, gamma, cp;
+    // binomial(gamma, k), k = 1..8: the FP64 series of (1 + x)^gamma - 1
+    // (kernel arguments, so the kernels read them from scalar registers)
+    double ppc[8];
     double g[3];
     int buoyancy;
     const double* JcV;                // [9][nb]

@@ -424,6 +424,8 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
     dv.npx = g.npx; dv.npy = g.npy; dv.npz = g.npz; dv.np = g.np;
     dv.nelem = g.nbcs; dv.nb = nb;
     dv.dt = prm->dt; dv.P0 = prm->P0; dv.T0 = prm->T0; dv.R = R; dv.gamma = gam; dv.cp = prm->cp;
+    dv.ppc[0] = gam;
+    for (int k = 1; k < 8; k++) dv.ppc[k] = dv.ppc[k - 1] * (gam - k) / (k + 1);
     for (int c = 0; c < 3; c++) dv.g[c] = prm->gravity[c];
     dv.buoyancy = prm->buoyancy;
     dv.JcV = d->JcV.as<double>(); dv.cV = d->cV.as<double>();

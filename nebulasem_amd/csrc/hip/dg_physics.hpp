@@ -120,16 +120,14 @@ __device__ __forceinline__ Dual<K> r_div(const Dual<K>& a, const Dual<K>& b) { r
 // expm1(gamma log1p(x)) (no cancellation).  For small x the binomial series
 // sum_k binom(gamma, k) x^k is used instead, a few multiply-adds in place of
 // two libm calls: in FP64 to x^8 for |x| <= 1/64 (for gamma in (1, 2) the
-// first omitted term is below 1e-17 of the sum), in FP32 to x^6 for
-// |x| <= 1/32 (below 1e-10, under FP32 rounding).
-__device__ __forceinline__ double pow1p_m1(double x, double gam) {
-    if (fabs(x) > 0.015625) return expm1(gam * log1p(x));
-    const double c1 = gam, c2 = c1 * (gam - 1.0) * 0.5, c3 = c2 * (gam - 2.0) * (1.0 / 3.0);
-    const double c4 = c3 * (gam - 3.0) * 0.25, c5 = c4 * (gam - 4.0) * 0.2, c6 = c5 * (gam - 5.0) * (1.0 / 6.0);
-    const double c7 = c6 * (gam - 6.0) * (1.0 / 7.0), c8 = c7 * (gam - 7.0) * 0.125;
-    return x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * (c6 + x * (c7 + x * c8)))))));
+// first omitted term is below 1e-17 of the sum) with the coefficients given
+// (DgDev::ppc: uniform, so they cost no vector registers), in FP32 to x^6
+// for |x| <= 1/32 (below 1e-10, under FP32 rounding).
+__device__ __forceinline__ double pow1p_m1(double x, double gam, const double* c) {
+    if (!c || fabs(x) > 0.015625) return expm1(gam * log1p(x));
+    return x * (c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * (c[4] + x * (c[5] + x * (c[6] + x * c[7])))))));
 }
-__device__ __forceinline__ float pow1p_m1(float x, float gam) {
+__device__ __forceinline__ float pow1p_m1(float x, float gam, const double*) {
     if (fabsf(x) > 0.03125f) return expm1f(gam * log1pf(x));
     const float c1 = gam, c2 = c1 * (gam - 1.0f) * 0.5f, c3 = c2 * (gam - 2.0f) * (1.0f / 3.0f);
     const float c4 = c3 * (gam - 3.0f) * 0.25f, c5 = c4 * (gam - 4.0f) * 0.2f;
@@ -137,7 +135,7 @@ __device__ __forceinline__ float pow1p_m1(float x, float gam) {
     return x * (c1 + x * (c2 + x * (c3 + x * (c4 + x * (c5 + x * c6)))));
 }
 template <int K>
-__device__ __forceinline__ Dual<K> pow1p_m1(const Dual<K>& x, const Dual<K>& gam) {
+__device__ __forceinline__ Dual<K> pow1p_m1(const Dual<K>& x, const Dual<K>& gam, const double*) {
     if (fabs(x.v) > 0.015625) return r_expm1(gam * r_log1p(x));
     // the FP64 series and its derivative sum_k k binom(gamma, k) x^(k-1) dx
     // (gamma a constant here, as in every caller)
@@ -171,12 +169,13 @@ struct Prim {
 };
 
 template <typename R>
-__device__ __forceinline__ void make_prim(Prim<R>& P, R rho_ref, R p_ref, R T0, R gam, R gR) {
+__device__ __forceinline__ void make_prim(Prim<R>& P, R rho_ref, R p_ref, R T0, R gam, R gR,
+                                          const double* ppc = nullptr) {
     const R rho = rho_ref + P.rp;
     const R ir = r_div(R(1), rho);
     P.u[0] = P.m[0] * ir; P.u[1] = P.m[1] * ir; P.u[2] = P.m[2] * ir;
     P.thp = (P.tp - T0 * P.rp) * ir;
-    P.pp = p_ref * pow1p_m1(r_div(P.tp, rho_ref * T0), gam);
+    P.pp = p_ref * pow1p_m1(r_div(P.tp, rho_ref * T0), gam, ppc);
     P.c = r_sqrt(gR * (T0 + P.thp));
     P.umag = r_sqrt(P.u[0] * P.u[0] + P.u[1] * P.u[1] + P.u[2] * P.u[2]);
 }

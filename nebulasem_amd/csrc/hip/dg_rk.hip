@@ -267,7 +267,7 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             P.m[0] = (R)(base[1] + din[1]); P.m[1] = (R)(base[2] + din[2]); P.m[2] = (R)(base[3] + din[3]);
             P.tp = (R)(base[4] + din[4]);
         }
-        make_prim(P, rref, pref, T0, gam, gR);
+        make_prim(P, rref, pref, T0, gam, gR, d.ppc);
         R mu = 0, imu = 0;
         if constexpr (VISC) {
             mu = (rref + P.rp) * (R)d.visc;
@@ -350,7 +350,7 @@ __device__ __forceinline__ void rk_stage_body(const DgDev& d, const RkArgs<S>& a
             Prim<R> Q{};
             if (!ghost) {
                 Q.rp = F.rp; Q.m[0] = F.m[0]; Q.m[1] = F.m[1]; Q.m[2] = F.m[2]; Q.tp = F.tp;
-                make_prim(Q, F.rref, F.pref, T0, gam, gR);
+                make_prim(Q, F.rref, F.pref, T0, gam, gR, d.ppc);
             } else {
                 const BcTable& bt = *d.bc;
                 const int pi = F.meta >> 8;
@@ -842,7 +842,7 @@ __global__ void __launch_bounds__(256) mortar_flux_kernel(DgDev d, RkArgs<S> a) 
         P.m[1] = load_state<R, S, false, DIN>(a, 2, n);
         P.m[2] = load_state<R, S, false, DIN>(a, 3, n);
         P.tp = load_state<R, S, false, DIN>(a, 4, n);
-        make_prim(P, (R)d.rho_ref[n], (R)d.p_ref[n], T0, gam, gR);
+        make_prim(P, (R)d.rho_ref[n], (R)d.p_ref[n], T0, gam, gR, d.ppc);
     };
     extern __shared__ unsigned char mortar_lds[];
     R* L = reinterpret_cast<R*>(mortar_lds);   // [12][npfm]

@@ -88,6 +88,12 @@ def test_rk3_dot_product(oracle, tmp_path, case):
     lhs, rhs = np.vdot(Mv, w), np.vdot(v, MTw)
     print(f"  <Mv,w> {lhs:.16e}  <v,MTw> {rhs:.16e}", flush=True)
     assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(rhs))
+    # the same adjoint again: the buffers the VJP reuses between launches
+    # (2:1 mortar terms, boundary-side terms) start clean on every call; the
+    # mortar terms are summed with atomics, so equal to rounding
+    set_q(dg, q)
+    MTw2 = dg.rk3_adjoint(n, w, checkpoint=2)
+    assert np.max(np.abs(MTw2 - MTw)) <= 1e-12 * np.max(np.abs(MTw))
     dg.close()
 
 
