@@ -471,18 +471,47 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
             for (int r = 0; r < 4; r++) res[r] += sf[r];
         }
     __syncthreads();
-    // volume terms in two passes of two flux rows through LDS
-    if (active)
-        for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[0][c]; F[(3 + c) * NP + q] = fr[1][c]; }
-    __syncthreads();
-    if (active)
-        volume_reference_order<2, 0>(res, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
-    __syncthreads();
-    if (active)
-        for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[2][c]; F[(3 + c) * NP + q] = fr[3][c]; }
-    __syncthreads();
-    if (!active) return;
-    volume_reference_order<2, 2>(res + 2, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
+    const double* D1l = Dl + NX * NX;
+    const double* D2l = D1l + NY * NY;
+    if constexpr (VISC) {
+        // volume terms in two passes of two flux rows through LDS
+        if (active)
+            for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[0][c]; F[(3 + c) * NP + q] = fr[1][c]; }
+        __syncthreads();
+        if (active) volume_reference_order<2, 0>(res, F, J, NP, i, j, k, NX, NY, NZ, Dl, D1l, D2l);
+        __syncthreads();
+        if (active)
+            for (int c = 0; c < 3; c++) { F[c * NP + q] = fr[2][c]; F[(3 + c) * NP + q] = fr[3][c]; }
+        __syncthreads();
+        if (!active) return;
+        volume_reference_order<2, 2>(res + 2, F, J, NP, i, j, k, NX, NY, NZ, Dl, D1l, D2l);
+    } else {
+        // volume terms in one pass: the six state values (rho^n, U, theta,
+        // p') of every node in LDS, each source's four flux rows rebuilt
+        // from them with the same operations as fr above
+        if (active) {
+            F[q] = rho0; F[NP + q] = u[0]; F[2 * NP + q] = u[1]; F[3 * NP + q] = u[2];
+            F[4 * NP + q] = th; F[5 * NP + q] = pp;
+        }
+        __syncthreads();
+        if (!active) return;
+        volume_reference_walk(
+            [&](int sn, double w0, double w1, double w2) {
+                const double r0 = F[sn], us[3] = {F[NP + sn], F[2 * NP + sn], F[3 * NP + sn]};
+                const double ts = F[4 * NP + sn], ps = F[5 * NP + sn];
+                const double fc[3] = {r0 * us[0], r0 * us[1], r0 * us[2]};
+                for (int row = 0; row < 3; row++) {
+                    // tensor flux: row dot left to right
+                    const double f0 = fc[row] * us[0] + (row == 0 ? ps : 0.0 * ps);
+                    const double f1 = fc[row] * us[1] + (row == 1 ? ps : 0.0 * ps);
+                    const double f2 = fc[row] * us[2] + (row == 2 ? ps : 0.0 * ps);
+                    res[row] -= f0 * w0 + f1 * w1 + f2 * w2;
+                }
+                // vector flux: right-nested dot
+                res[3] -= dot3r(fc[0] * ts, fc[1] * ts, fc[2] * ts, w0, w1, w2);
+            },
+            J, NP, i, j, k, NX, NY, NZ, Dl, D1l, D2l);
+    }
 
     const double cv = d.cV[n];
     const double ap0 = (-1.0 / d.dt) * cv;

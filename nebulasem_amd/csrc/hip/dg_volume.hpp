@@ -20,21 +20,12 @@ constexpr int kJinRow[9] = {0, 6, 8, 3, 1, 7, 5, 4, 2};
 // Jin grad psi is one column of Jin times a derivative value, exactly as the
 // reference's 3x3 product evaluates it.  F: LDS rows x 3 x NP; J: 9 x NP in
 // the kJinRow order; D0, D1, D2: the 1-D derivative matrices.
-template <int ROWS, int VECMASK>
-__device__ __forceinline__ void volume_reference_order(double* r, const double* F, const double* J, int NP,
-                                                       int i, int j, int k, int NX, int NY, int NZ,
-                                                       const double* D0, const double* D1,
-                                                       const double* D2) {
-    auto sub = [&](int s, double w0, double w1, double w2) {
-        for (int row = 0; row < ROWS; row++) {
-            const double f0 = F[(row * 3 + 0) * NP + s];
-            const double f1 = F[(row * 3 + 1) * NP + s];
-            const double f2 = F[(row * 3 + 2) * NP + s];
-            // vector flux: right-nested dot; tensor flux: row dot left to right
-            const double c = ((VECMASK >> row) & 1) ? dot3r(f0, f1, f2, w0, w1, w2) : f0 * w0 + f1 * w1 + f2 * w2;
-            r[row] -= c;
-        }
-    };
+// The walk itself: sub(s, w0, w1, w2) is called for each source node s of
+// target (i,j,k) in the reference's order with w = Jin_s grad psi_t(x_s).
+template <typename Sub>
+__device__ __forceinline__ void volume_reference_walk(Sub&& sub, const double* J, int NP, int i, int j, int k,
+                                                      int NX, int NY, int NZ, const double* D0, const double* D1,
+                                                      const double* D2) {
     const int syz = NY * NZ;
     const int t = i * syz + j * NZ + k;
     for (int a = 0; a < i; a++) {
@@ -71,6 +62,26 @@ __device__ __forceinline__ void volume_reference_order(double* r, const double* 
         const double D = D0[a * NX + i];
         sub(s, J[s] * D, J[NP + s] * D, J[2 * NP + s] * D);
     }
+}
+
+template <int ROWS, int VECMASK>
+__device__ __forceinline__ void volume_reference_order(double* r, const double* F, const double* J, int NP,
+                                                       int i, int j, int k, int NX, int NY, int NZ,
+                                                       const double* D0, const double* D1,
+                                                       const double* D2) {
+    volume_reference_walk(
+        [&](int s, double w0, double w1, double w2) {
+            for (int row = 0; row < ROWS; row++) {
+                const double f0 = F[(row * 3 + 0) * NP + s];
+                const double f1 = F[(row * 3 + 1) * NP + s];
+                const double f2 = F[(row * 3 + 2) * NP + s];
+                // vector flux: right-nested dot; tensor flux: row dot left to right
+                const double c =
+                    ((VECMASK >> row) & 1) ? dot3r(f0, f1, f2, w0, w1, w2) : f0 * w0 + f1 * w1 + f2 * w2;
+                r[row] -= c;
+            }
+        },
+        J, NP, i, j, k, NX, NY, NZ, D0, D1, D2);
 }
 
 }  // namespace nsem
