@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     const uint32_t e = blockIdx.x * epb + le;
     const bool active = le < epb && e < d.nelem;
     extern __shared__ double lds[];
-    double* F = lds + (size_t)le * 6 * NP;  // two flux rows at a time
+    double* F = lds + (size_t)le * 6 * NP;  // node states (inviscid) or two flux rows (viscous)
     double* J = lds + (size_t)epb * 6 * NP + (size_t)le * 9 * NP;
     double* SF = split_sf_overlaid(epb, NP, NFN, 6, 4) ? lds : lds + (size_t)epb * 15 * NP;
     double* Dl = lds + split_lds_doubles(epb, NP, NFN, 6, 4);
@@ -426,17 +426,18 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     const uint64_t n = (uint64_t)e * NP + q;
 
     double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
-    double fr[4][3];  // own flux rows: U_x U_y U_z (tensor rows), theta (vector)
+    // viscous: own flux rows U_x U_y U_z (tensor rows), theta (vector)
+    double fr[VISC ? 4 : 1][3];
     if (active) {
         rho0 = s.rho[n]; rho = rho1[n];
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
         th = s.T[n] + d.T0;
         pp = o.p[n];
-        const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
-        for (int row = 0; row < 3; row++)
-            for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
-        for (int c = 0; c < 3; c++) fr[3][c] = fc[c] * th;
         if constexpr (VISC) {
+            const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
+            for (int row = 0; row < 3; row++)
+                for (int c = 0; c < 3; c++) fr[row][c] = fc[row] * u[c] + (row == c ? pp : 0.0 * pp);
+            for (int c = 0; c < 3; c++) fr[3][c] = fc[c] * th;
             double gown[12];
             for (int c = 0; c < 12; c++) gown[c] = d.grad[(size_t)c * d.nb + n];
             const double mu = rho0 * d.visc, imu = mu * d.iPr;
