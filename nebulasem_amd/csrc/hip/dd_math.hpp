@@ -27,7 +27,16 @@
 //   * log of a double: one Newton step on exp from the hardware log y0,
 //     log x = y0 + log(x e^-y0), with x e^-y0 - 1 = O(2^-52) so that
 //     log(1 + r) = r - r^2/2 to 2^-106;
-//   * x^y = exp(y log x) with the product in double-double.
+//   * x^y = exp(y log x) with the product in double-double;
+//   * Ziv's strategy (A. Ziv, "Fast evaluation of elementary mathematical
+//     functions with correctly rounded last bit", ACM TOMS 17(3), 1991): a
+//     first, cheaper evaluation of the same formulas to about 2^-73
+//     relative (exp_fast: the expm1 series to degree 7, only r and r^2 in
+//     double-double), rounded when a test shows that every value within its
+//     error bound rounds to the same double; only the rare inputs failing
+//     the test (about one in 2^16) take the 2^-90 evaluation.  Where the
+//     test passes both evaluations round to the same double, so pow_dd
+//     returns exactly what the 2^-90 evaluation alone returns.
 // The same source compiles for the host (the correctness test builds it
 // with g++ and checks it against exact decimal arithmetic and glibc).
 #pragma once
@@ -154,24 +163,91 @@ NSEM_DD_FN DD exp_dd(DD a) {
     return e;
 }
 
-// log x for positive normal finite x.
-NSEM_DD_FN DD log_dd(double x) {
-    const double y0 = std::log(x);
-    const DD t = mul_d(exp_dd(DD{-y0, 0.0}), x);  // x e^-y0 = 1 + r
-    const double r = (t.hi - 1.0) + t.lo;          // t.hi - 1 is exact (Sterbenz)
-    return fast_two_sum(y0, r - 0.5 * r * r);
+// e^a to about 2^-74 relative for |a.hi| <= 700, the first evaluation of
+// pow_dd.  The reduction of exp_dd with its one cancelling subtraction done
+// by TwoSum (exact) and the small terms (a.lo, the low parts of m ln2/64;
+// each below 2^-42) summed in double, error below 2^-92; expm1 r = r + r^2
+// (1/2 + r p(r)) with p the Taylor coefficients of degrees 3-7 in double
+// (first omitted term below 2^-75; r p below 2^-10, so its rounding is
+// below 2^-63 and reaches the result times r^2 < 2^-15), r^2 and the
+// product in double-double.
+NSEM_DD_FN DD exp_fast(DD a) {
+    const double kL64Hi = 0.010830424696249145, kL64Lo = 3.623510646634843e-19;
+    const double m = std::nearbyint(a.hi * 92.33248261689366);
+    const DD mh = two_prod(m, kL64Hi);
+    const DD s0 = two_sum(a.hi, -mh.hi);
+    const DD r = two_sum(s0.hi, ((a.lo - mh.lo) - m * kL64Lo) + s0.lo);
+    const double rh = r.hi;
+    const double p = 0.16666666666666666 +
+                     rh * (0.041666666666666664 +
+                           rh * (0.008333333333333333 + rh * (1.388888888888889e-03 + rh * 1.984126984126984e-04)));
+    const DD s = fast_two_sum(0.5, rh * p);
+    DD sq = two_prod(rh, rh);
+    sq.lo += (rh + rh) * r.lo;
+    const DD u = add_nc(r, mul(sq, s));
+    const int mi = (int)m;
+    const DD t = kExp2Tab[mi & 63];
+    DD e = add_nc(t, mul(t, u));
+    const int ki = mi >> 6;
+    e.hi = std::ldexp(e.hi, ki);
+    e.lo = std::ldexp(e.lo, ki);
+    return e;
 }
 
-// x^y rounded once from a double-double value accurate to about 2^-90.
-// Outside positive normal finite x, or where y log x leaves the range of
-// normal results, the C library's pow (its special cases: NaN, 0, Inf,
-// negative bases, overflow, underflow).
-NSEM_DD_FN double pow_dd(double x, double y) {
+// log x for positive normal finite x (FAST: through exp_fast, error below
+// 2^-74 absolute for 2^-1000 <= x <= 2^1000, where e^-y0 stays normal).
+template <bool FAST>
+NSEM_DD_FN DD log_t(double x) {
+    const double y0 = std::log(x);
+    const DD e = FAST ? exp_fast(DD{-y0, 0.0}) : exp_dd(DD{-y0, 0.0});
+    const DD t = mul_d(e, x);              // x e^-y0 = 1 + r
+    const double r = (t.hi - 1.0) + t.lo;  // t.hi - 1 is exact (Sterbenz)
+    return fast_two_sum(y0, r - 0.5 * r * r);
+}
+NSEM_DD_FN DD log_dd(double x) { return log_t<false>(x); }
+
+// x^y from the 2^-90 evaluation alone (pow_dd's second evaluation).
+NSEM_DD_FN double pow_dd_accurate(double x, double y) {
     if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308) || !std::isfinite(y))
         return std::pow(x, y);
     const DD t = mul_d(log_dd(x), y);
     if (!(std::fabs(t.hi) <= 700.0)) return std::pow(x, y);
     return exp_dd(t).hi;
+}
+
+// The first evaluation of pow_dd: e^(y log x) as a double-double e (e.hi the
+// double nearest e.hi + e.lo) with relative error below (1 + |y|) 2^-73, for
+// 2^-1000 <= x <= 2^1000 and |y log x| <= 680 (below e^-680 = 2^-981 the
+// low part would lose bits to the subnormal range); returns false outside
+// that range.
+NSEM_DD_FN bool pow_fast(double x, double y, DD& e) {
+    if (!(x >= 0x1p-1000 && x <= 0x1p1000)) return false;
+    const DD t = mul_d(log_t<true>(x), y);
+    if (!(std::fabs(t.hi) <= 680.0)) return false;
+    e = exp_fast(t);
+    return true;
+}
+
+// x^y, correctly rounded except where the exact power lies within about
+// 2^-90 of a rounding boundary; outside positive normal finite x, or where
+// y log x leaves the range of normal results, the C library's pow (its
+// special cases: NaN, 0, Inf, negative bases, overflow, underflow).  The
+// cheap evaluation comes first: with its error bound taken as
+// eps = (1 + |y|) 2^-71 (four times the bound above) and d = 2 eps e.hi,
+// e.hi + (e.lo + d) and e.hi + (e.lo - d) both round to e.hi only if every
+// value within eps e.hi of e.hi + e.lo -- the exact power among them --
+// lies inside e.hi's rounding interval; e.hi is then the correctly rounded
+// power, the double that the 2^-90 evaluation returns too.  Otherwise that
+// evaluation decides.
+NSEM_DD_FN double pow_dd(double x, double y) {
+    if (std::isfinite(y)) {
+        DD e;
+        if (pow_fast(x, y, e)) {
+            const double d = (1.0 + std::fabs(y)) * 0x1p-70 * e.hi;
+            if (e.hi + (e.lo + d) == e.hi && e.hi + (e.lo - d) == e.hi) return e.hi;
+        }
+    }
+    return pow_dd_accurate(x, y);
 }
 
 // e^x rounded once from the double-double value (C library exp outside
