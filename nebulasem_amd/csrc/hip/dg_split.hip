@@ -41,6 +41,15 @@ __host__ __device__ __forceinline__ bool split_sf_overlaid(int epb, int NP, int 
 __host__ __device__ __forceinline__ size_t split_lds_doubles(int epb, int NP, int NFN, int FR, int SFR) {
     return (size_t)epb * (FR + 9) * NP + (split_sf_overlaid(epb, NP, NFN, FR, SFR) ? 0 : (size_t)epb * NFN * SFR);
 }
+// Bit g set: node (i, j, k) of element e lies on face g and that face takes
+// a flux (the cell's face entry is active).
+__device__ __forceinline__ unsigned split_node_faces(const DgDev& d, uint32_t e, int i, int j, int k, int NX, int NY,
+                                                     int NZ) {
+    unsigned fon = 0;
+    for (int g = 0; g < 6; g++)
+        if (on_face(g, i, j, k, NX, NY, NZ) && ((d.eface[(size_t)e * 6 + g].meta >> 4) & 1)) fon |= 1u << g;
+    return fon;
+}
 // Stage D0, D1, D2 (NX^2, NY^2, NZ^2 values) in LDS at Dl.
 __device__ __forceinline__ void split_stage_d(const DgDev& d, double* Dl, int NX, int NY, int NZ) {
     const int n0 = NX * NX, n1 = NY * NY, n2 = NZ * NZ;
@@ -123,11 +132,17 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
-    double rho = 0, u0 = 0, u1 = 0, u2 = 0, th = 0;
+    double rho = 0, u0 = 0, u1 = 0, u2 = 0, th = 0, cvn = 0, pref = 0;
+    unsigned fon = 0;
     if (active) {
         rho = s.rho[n]; u0 = s.ux[n]; u1 = s.uy[n]; u2 = s.uz[n];
         th = s.T[n] + d.T0;
         for (int c = 0; c < 9; c++) J[c * NP + q] = d.JcV[(size_t)kJinRow[c] * d.nb + n];
+        // read with the state, before the first barrier, so that no global
+        // load waits after it: cV, p_ref and the faces through the node
+        cvn = d.cV[n];
+        pref = d.p_ref[n];
+        fon = split_node_faces(d, e, i, j, k, NX, NY, NZ);
     }
     // surface: one thread per face node of the work-group's elements (for
     // N = 7 one face per wavefront, so the face branch is uniform), each
@@ -146,9 +161,7 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     double surf = 0;
     if (active)
         for (int g = 0; g < 6; g++) {
-            if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
-            const EFace ef = d.eface[(size_t)e * 6 + g];
-            if (!((ef.meta >> 4) & 1)) continue;
+            if (!((fon >> g) & 1)) continue;
             int a, b;
             face_ab(g, i, j, k, a, b);
             surf += SF[le * NFN + face_task_index(g, a, b, NX, NY, NZ)];
@@ -163,12 +176,12 @@ __global__ void __launch_bounds__(512) split_rho_kernel(DgDev d, SplitState s, d
     if (!active) return;
     double r[1] = {surf};
     volume_reference_order<1, 1>(r, F, J, NP, i, j, k, NX, NY, NZ, Dl, Dl + NX * NX, Dl + NX * NX + NY * NY);
-    const double ap = (-1.0 / d.dt) * d.cV[n];
+    const double ap = (-1.0 / d.dt) * cvn;
     const double rho1 = (r[0] + rho * ap) / ap;
     rho_out[n] = rho1;
     // p = P0 (rho T R / P0)^gamma with the new rho, then p - p_ref
     // (euler.cpp:210-212), once per node for the momentum and theta pass
-    p_out[n] = d.P0 * dd::pow_dd((rho1 * th * d.R) / d.P0, d.gamma) - d.p_ref[n];
+    p_out[n] = d.P0 * dd::pow_dd((rho1 * th * d.R) / d.P0, d.gamma) - pref;
 }
 
 // ---------------------------------------------------------------------------
@@ -425,7 +438,8 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     const int i = q / (NY * NZ), j = (q / NZ) % NY, k = q % NZ;
     const uint64_t n = (uint64_t)e * NP + q;
 
-    double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0;
+    double rho0 = 0, rho = 0, u[3] = {0, 0, 0}, th = 0, pp = 0, cv = 0, rref = 0;
+    unsigned fon = 0;
     // viscous: own flux rows U_x U_y U_z (tensor rows), theta (vector)
     double fr[VISC ? 4 : 1][3];
     if (active) {
@@ -433,6 +447,10 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
         u[0] = s.ux[n]; u[1] = s.uy[n]; u[2] = s.uz[n];
         th = s.T[n] + d.T0;
         pp = o.p[n];
+        // read with the state (no global load after the first barrier)
+        cv = d.cV[n];
+        if (d.buoyancy) rref = d.rho_ref[n];
+        fon = split_node_faces(d, e, i, j, k, NX, NY, NZ);
         if constexpr (VISC) {
             const double fc[3] = {rho0 * u[0], rho0 * u[1], rho0 * u[2]};
             for (int row = 0; row < 3; row++)
@@ -463,9 +481,7 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
     double res[4] = {0, 0, 0, 0};
     if (active)
         for (int g = 0; g < 6; g++) {
-            if (!on_face(g, i, j, k, NX, NY, NZ)) continue;
-            const EFace ef = d.eface[(size_t)e * 6 + g];
-            if (!((ef.meta >> 4) & 1)) continue;
+            if (!((fon >> g) & 1)) continue;
             int a, b;
             face_ab(g, i, j, k, a, b);
             const double* sf = SF + 4 * (le * NFN + face_task_index(g, a, b, NX, NY, NZ));
@@ -514,13 +530,12 @@ __global__ void __launch_bounds__(512) split_ut_kernel(DgDev d, SplitState s, co
             J, NP, i, j, k, NX, NY, NZ, Dl, D1l, D2l);
     }
 
-    const double cv = d.cV[n];
     const double ap0 = (-1.0 / d.dt) * cv;
     const double ap = ap0 * rho;
     double* outs[3] = {o.ux, o.uy, o.uz};
     double un[3];
     for (int c = 0; c < 3; c++) {
-        const double sc = d.buoyancy ? (rho - d.rho_ref[n]) * d.g[c] : 0.0;
+        const double sc = d.buoyancy ? (rho - rref) * d.g[c] : 0.0;
         un[c] = ((res[c] - sc * cv) + (u[c] * rho0) * ap0) / ap;
         outs[c][n] = un[c];
     }
