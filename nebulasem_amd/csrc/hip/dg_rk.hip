@@ -729,11 +729,13 @@ __global__ void __launch_bounds__(512) rk_stage_kernel(DgDev d, RkArgs<S> a) {
 // per work-group, whose element index is not uniform and costs registers, so
 // they take one wave less (FP64 at 4^3 nodes two less: at 3 waves it spills
 // 56 bytes per lane and ran 8% slower than at 2); the viscous variant's extra
-// gradient state takes two less.
+// gradient state takes two less.  FP32 at 6^3 nodes (N = 5) spills a few
+// dwords at 6 and runs 5-6% faster at 5.  Measured and not taken: FP32 4^3
+// at 4 (2% slower than 5), FP64 5^3 at 2 (18% slower than 3).
 template <typename R, int X, int Y, int Z, bool VISC>
 constexpr int rk_waves_per_simd() {
     constexpr int np = X * Y * Z;
-    if constexpr (sizeof(R) == 4) return VISC ? 4 : (np > 128 ? 6 : 5);
+    if constexpr (sizeof(R) == 4) return VISC ? 4 : (np > 128 ? (np == 216 ? 5 : 6) : 5);
     return VISC ? 2 : (np > 128 ? 4 : np > 64 ? 3 : 2);
 }
 
