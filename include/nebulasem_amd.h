@@ -96,9 +96,16 @@ void nsem_mesh_destroy(nsem_mesh* m);
  * (every node, ghosts included, xyz per node for ncomp 3); bc_kind/bc_value
  * per mesh boundary patch in nsem_mesh_boundary_name order (kind -1 when the
  * file has no entry for the patch).  gravity is used by `hydrostatic`.     */
-enum { NSEM_BC_NEUMANN = 0, NSEM_BC_SYMMETRY = 1, NSEM_BC_DIRICHLET = 2 };
+enum { NSEM_BC_NEUMANN = 0, NSEM_BC_SYMMETRY = 1, NSEM_BC_DIRICHLET = 2, NSEM_BC_ROBIN = 3 };
 int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
                     double* values, int32_t* bc_kind, double* bc_value);
+/* As nsem_field_read, and ROBIN patches too (BCondition's value / shape /
+ * tvalue, field.h:156-240): bc_shape[patch], bc_tvalue[ncomp*patch]; a
+ * ROBIN ghost node is shape value + (1 - shape)(owner + tvalue |x_ghost -
+ * x_owner|) (applyExplicitBCs, field.h:2677-2680).  nsem_field_read fails
+ * with NSEM_EUNSUPPORTED on a ROBIN patch, as before. */
+int nsem_field_read_bc(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
+                       double* values, int32_t* bc_kind, double* bc_value, double* bc_shape, double* bc_tvalue);
 /* Write a field file as MeshField::write_ does (field.h:1588-1663, boundary
  * blocks as BCondition's operator<<, field.h:187-216): ".bin" paths in the
  * binary token stream (util.h:191-222), others as text with 12 significant
@@ -107,6 +114,11 @@ int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const d
  * value from bc_value[ncomp*patch], written when its magnitude exceeds 1e-7. */
 int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
                      const int32_t* bc_kind, const double* bc_value);
+/* As nsem_field_write, with ROBIN patches (shape written when it exceeds
+ * 1e-7, tvalue when its magnitude does, as BCondition's operator<<). */
+int nsem_field_write_bc(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
+                        const int32_t* bc_kind, const double* bc_value, const double* bc_shape,
+                        const double* bc_tvalue);
 /* The refinement tree amrTree_<step>.bin / .txt that the reference's prepare
  * tool writes beside a refined grid (Mesh::Node, mesh.h:49-72, as a
  * std::vector, util.h:129-148): nodes[4*i..] = id (the cell of a leaf),

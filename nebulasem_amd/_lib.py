@@ -149,6 +149,24 @@ def read_field(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
     return vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp)
 
 
+def read_field_bc(mesh, path, ncomp, gravity=(0.0, 0.0, 0.0)):
+    """read_field with ROBIN patches (nsem_field_read_bc): (values[nall, ncomp],
+    bc_kind[nb], bc_value[nb, ncomp], bc_shape[nb], bc_tvalue[nb, ncomp])."""
+    n = mesh.info.nall_field
+    nb = mesh.info.nboundaries
+    vals = np.empty(n * ncomp)
+    kinds = np.empty(max(nb, 1), dtype=np.int32)
+    bv = np.empty(max(nb, 1) * ncomp)
+    sh = np.empty(max(nb, 1))
+    tv = np.empty(max(nb, 1) * ncomp)
+    check(lib().nsem_field_read_bc(mesh.handle, path.encode(), ncomp, (C.c_double * 3)(*gravity),
+                                   vals.ctypes.data_as(C.c_void_p), kinds.ctypes.data_as(C.c_void_p),
+                                   bv.ctypes.data_as(C.c_void_p), sh.ctypes.data_as(C.c_void_p),
+                                   tv.ctypes.data_as(C.c_void_p)))
+    return (vals.reshape(n, ncomp), kinds[:nb], bv[:nb * ncomp].reshape(nb, ncomp), sh[:nb],
+            tv[:nb * ncomp].reshape(nb, ncomp))
+
+
 def read_amr_tree(path):
     """amrTree_<step>.bin/.txt: (n, 4) uint32 nodes (id, nchildren, cid, level)."""
     n = C.c_uint64()
@@ -175,13 +193,27 @@ def write_field(mesh, path, values, bc_kind, bc_value):
                                  k.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p)))
 
 
+def write_field_bc(mesh, path, values, bc_kind, bc_value, bc_shape, bc_tvalue):
+    """write_field with ROBIN patches (nsem_field_write_bc): bc_shape (nb,),
+    bc_tvalue (nb, ncomp)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    ncomp = 1 if v.ndim == 1 else v.shape[1]
+    k = np.ascontiguousarray(bc_kind, dtype=np.int32).reshape(-1)
+    bv = np.ascontiguousarray(bc_value, dtype=np.float64).reshape(-1)
+    sh = np.ascontiguousarray(bc_shape, dtype=np.float64).reshape(-1)
+    tv = np.ascontiguousarray(bc_tvalue, dtype=np.float64).reshape(-1)
+    check(lib().nsem_field_write_bc(mesh.handle, path.encode(), ncomp, v.ctypes.data_as(C.c_void_p),
+                                    k.ctypes.data_as(C.c_void_p), bv.ctypes.data_as(C.c_void_p),
+                                    sh.ctypes.data_as(C.c_void_p), tv.ctypes.data_as(C.c_void_p)))
+
+
 class EulerParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("P0", C.c_double), ("T0", C.c_double), ("cp", C.c_double),
                 ("cv", C.c_double), ("gravity", C.c_double * 3), ("buoyancy", C.c_int32),
                 ("diffusion", C.c_int32), ("viscosity", C.c_double), ("Pr", C.c_double)]
 
 
-BC_NEUMANN, BC_SYMMETRY, BC_DIRICHLET = 0, 1, 2
+BC_NEUMANN, BC_SYMMETRY, BC_DIRICHLET, BC_ROBIN = 0, 1, 2, 3
 
 
 def euler_params(dt, P0=101325.0, T0=300.0, cp=1004.67, cv=715.5, gravity=(0.0, 0.0, -9.80606),

@@ -35,8 +35,9 @@ void reflect(const double* p, const double* n, double* r) {
 
 // Ghost nodes from the boundary conditions just read, as a field read does
 // (MeshField::read_, field.h:1557-1564, then applyExplicitBCs 2585-2727):
-// NEUMANN: owner + value |x_ghost - x_owner|; SYMMETRY: a vector reflected
-// about the face, a scalar copied; DIRICHLET: the value.
+// NEUMANN: owner + value |x_ghost - x_owner|; ROBIN: shape value + (1 -
+// shape)(owner + tvalue |x_ghost - x_owner|) (field.h:2677-2680); SYMMETRY:
+// a vector reflected about the face, a scalar copied; DIRICHLET: the value.
 void apply_explicit_bcs(const nsem::Geometry& g, const nsem::LoadedMesh& lm, int ncomp,
                         const std::vector<nsem::BoundaryCondition>& bcs, double* v) {
     for (const auto& bc : bcs) {
@@ -54,6 +55,12 @@ void apply_explicit_bcs(const nsem::Geometry& g, const nsem::LoadedMesh& lm, int
                     for (int c = 0; c < 3; c++) dv[c] = g.cC[c2][c] - g.cC[c1][c];
                     const double md = std::sqrt(dot_r(dv, dv));
                     for (int c = 0; c < ncomp; c++) dst[c] = src[c] + bc.value[c] * md;
+                } else if (bc.type == "ROBIN") {
+                    double dv[3];
+                    for (int c = 0; c < 3; c++) dv[c] = g.cC[c2][c] - g.cC[c1][c];
+                    const double md = std::sqrt(dot_r(dv, dv));
+                    for (int c = 0; c < ncomp; c++)
+                        dst[c] = bc.shape * bc.value[c] + (1 - bc.shape) * (src[c] + bc.tvalue[c] * md);
                 } else if (bc.type == "SYMMETRY") {
                     if (ncomp == 3) {
                         const double nv[3] = {g.fN[k][0], g.fN[k][1], g.fN[k][2]};
@@ -68,14 +75,11 @@ void apply_explicit_bcs(const nsem::Geometry& g, const nsem::LoadedMesh& lm, int
     }
 }
 
-}  // namespace
-
-extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp,
-                               const double gravity[3], double* values, int32_t* bc_kind,
-                               double* bc_value) {
-    if (!m || !path || !values || !bc_kind || !bc_value)
-        return nsem_fail(NSEM_EINVAL, "null argument");
+int field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3], double* values,
+               int32_t* bc_kind, double* bc_value, double* bc_shape, double* bc_tvalue) {
+    if (!m || !path || !values || !bc_kind || !bc_value) return nsem_fail(NSEM_EINVAL, "null argument");
     if (ncomp != 1 && ncomp != 3) return nsem_fail(NSEM_EINVAL, "ncomp must be 1 or 3");
+    const bool robin = bc_shape && bc_tvalue;
     const nsem::Geometry& g = nsem_mesh_geometry(m);
     std::vector<double> cc(g.nall_field * 3);
     for (uint64_t i = 0; i < g.nall_field; i++)
@@ -93,13 +97,22 @@ extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t nco
     for (size_t b = 0; b < names.size(); b++) {
         bc_kind[b] = -1;
         for (int c = 0; c < ncomp; c++) bc_value[b * ncomp + c] = 0;
+        if (robin) {
+            bc_shape[b] = 0;
+            for (int c = 0; c < ncomp; c++) bc_tvalue[b * ncomp + c] = 0;
+        }
         for (const auto& bc : fd.bcs) {
             if (bc.patch != names[b]) continue;
             if (bc.type == "NEUMANN") bc_kind[b] = NSEM_BC_NEUMANN;
             else if (bc.type == "SYMMETRY") bc_kind[b] = NSEM_BC_SYMMETRY;
             else if (bc.type == "DIRICHLET") bc_kind[b] = NSEM_BC_DIRICHLET;
+            else if (bc.type == "ROBIN" && robin) bc_kind[b] = NSEM_BC_ROBIN;
             else return nsem_fail(NSEM_EUNSUPPORTED, "boundary type " + bc.type + " is not supported");
             for (int c = 0; c < ncomp; c++) bc_value[b * ncomp + c] = bc.value[c];
+            if (robin) {
+                bc_shape[b] = bc.shape;
+                for (int c = 0; c < ncomp; c++) bc_tvalue[b * ncomp + c] = bc.tvalue[c];
+            }
         }
     }
     apply_explicit_bcs(g, nsem_mesh_loaded(m), ncomp, fd.bcs, fd.values.data());
@@ -107,10 +120,11 @@ extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t nco
     return NSEM_OK;
 }
 
-extern "C" int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
-                                const int32_t* bc_kind, const double* bc_value) {
+int field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values, const int32_t* bc_kind,
+                const double* bc_value, const double* bc_shape, const double* bc_tvalue) {
     if (!m || !path || !values || !bc_kind || !bc_value) return nsem_fail(NSEM_EINVAL, "null argument");
     if (ncomp != 1 && ncomp != 3) return nsem_fail(NSEM_EINVAL, "ncomp must be 1 or 3");
+    const bool robin = bc_shape && bc_tvalue;
     const nsem::Geometry& g = nsem_mesh_geometry(m);
     const std::vector<std::string>& names = nsem_mesh_boundary_names(m);
     std::vector<nsem::BoundaryCondition> bcs;
@@ -121,13 +135,42 @@ extern "C" int nsem_field_write(const nsem_mesh* m, const char* path, int32_t nc
         if (bc_kind[b] == NSEM_BC_NEUMANN) bc.type = "NEUMANN";
         else if (bc_kind[b] == NSEM_BC_SYMMETRY) bc.type = "SYMMETRY";
         else if (bc_kind[b] == NSEM_BC_DIRICHLET) bc.type = "DIRICHLET";
+        else if (bc_kind[b] == NSEM_BC_ROBIN && robin) bc.type = "ROBIN";
         else return nsem_fail(NSEM_EINVAL, "unknown boundary kind for " + names[b]);
         bc.value.assign(bc_value + b * ncomp, bc_value + (b + 1) * ncomp);
+        if (robin) {
+            bc.shape = bc_shape[b];
+            bc.tvalue.assign(bc_tvalue + b * ncomp, bc_tvalue + (b + 1) * ncomp);
+        }
         bcs.push_back(bc);
     }
     std::string err;
     if (!nsem::write_field_file(path, ncomp, g.nbcs_field, values, bcs, &err)) return nsem_fail(NSEM_EIO, err);
     return NSEM_OK;
+}
+
+}  // namespace
+
+extern "C" int nsem_field_read(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
+                               double* values, int32_t* bc_kind, double* bc_value) {
+    return field_read(m, path, ncomp, gravity, values, bc_kind, bc_value, nullptr, nullptr);
+}
+extern "C" int nsem_field_read_bc(const nsem_mesh* m, const char* path, int32_t ncomp, const double gravity[3],
+                                  double* values, int32_t* bc_kind, double* bc_value, double* bc_shape,
+                                  double* bc_tvalue) {
+    if (!bc_shape || !bc_tvalue) return nsem_fail(NSEM_EINVAL, "null argument");
+    return field_read(m, path, ncomp, gravity, values, bc_kind, bc_value, bc_shape, bc_tvalue);
+}
+
+extern "C" int nsem_field_write(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
+                                const int32_t* bc_kind, const double* bc_value) {
+    return field_write(m, path, ncomp, values, bc_kind, bc_value, nullptr, nullptr);
+}
+extern "C" int nsem_field_write_bc(const nsem_mesh* m, const char* path, int32_t ncomp, const double* values,
+                                   const int32_t* bc_kind, const double* bc_value, const double* bc_shape,
+                                   const double* bc_tvalue) {
+    if (!bc_shape || !bc_tvalue) return nsem_fail(NSEM_EINVAL, "null argument");
+    return field_write(m, path, ncomp, values, bc_kind, bc_value, bc_shape, bc_tvalue);
 }
 
 extern "C" int nsem_amr_tree_read(const char* path, uint32_t* nodes, uint64_t capacity, uint64_t* n_out) {

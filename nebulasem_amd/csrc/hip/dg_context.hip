@@ -73,6 +73,9 @@ int nsem_dg_create(const nsem_mesh* mesh, const nsem_euler_params* prm, const in
         for (int f = 0; f < 4; f++) {
             int k = bc_kind[4 * b + f];
             if (k < 0) return nsem_fail(NSEM_EINVAL, "no boundary condition for patch " + names[b]);
+            if (k > BC_DIRICHLET)
+                return nsem_fail(NSEM_EUNSUPPORTED, "boundary kind " + std::to_string(k) + " on patch " + names[b] +
+                                                        " is not supported by the DG path");
             if (f != 1 && k == BC_DIRICHLET)
                 return nsem_fail(NSEM_EUNSUPPORTED, "DIRICHLET on rho/T/p is not supported by the DG path");
             bt.kind[f][b] = k;

@@ -173,6 +173,7 @@ bool read_field_file(const std::string& path, const FieldContext& ctx, int ncomp
     for (uint32_t b = 0; b < nb; b++) {
         BoundaryCondition bc;
         bc.value.assign(ncomp, 0.0);
+        bc.tvalue.assign(ncomp, 0.0);
         if (!in.str(bc.patch)) return fail("bad patch name");
         if (!in.bin) {
             // text: the name may carry the brace ("top{")
@@ -193,6 +194,8 @@ bool read_field_file(const std::string& path, const FieldContext& ctx, int ncomp
                 for (int c = 0; c < ncomp; c++) in.f64(bc.value[c]);
             } else if (K == "SHAPE") {
                 in.f64(bc.shape);
+            } else if (K == "TVALUE") {
+                for (int c = 0; c < ncomp; c++) in.f64(bc.tvalue[c]);
             } else {
                 return fail("unsupported boundary keyword " + key);
             }
@@ -268,6 +271,11 @@ bool write_field_file(const std::string& path, int ncomp, uint64_t n, const doub
         }
         if (!(std::fabs(bc.shape) <= 1e-7)) {
             o.str("\tshape "); o.f64(bc.shape); o.str("\n");
+        }
+        double t2 = 0;
+        for (int c = 0; c < (int)bc.tvalue.size() && c < ncomp; c++) t2 += bc.tvalue[c] * bc.tvalue[c];
+        if (!(std::sqrt(t2) <= 1e-7)) {
+            o.str("\ttvalue "); put_value(o, bc.tvalue.data(), ncomp); o.str("\n");
         }
         o.str("}\n");
     }

@@ -11,6 +11,7 @@ struct BoundaryCondition {
     std::string type = "NEUMANN";  // upper-case BC name
     std::vector<double> value;     // ncomp entries
     double shape = 0;
+    std::vector<double> tvalue;    // ncomp entries (ROBIN)
 };
 
 struct FieldData {

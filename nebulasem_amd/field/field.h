@@ -156,10 +156,12 @@ class BaseField {
 namespace nsem_field {
 // boundary conditions of one named field, per mesh boundary patch (the
 // reference's BCondition list for that field, field.h:150-240)
-enum BcType : int { BC_NONE = -1, BC_NEUMANN = 0, BC_SYMMETRY = 1, BC_DIRICHLET = 2, BC_CALC_DIRICHLET = 3 };
+enum BcType : int { BC_NONE = -1, BC_NEUMANN = 0, BC_SYMMETRY = 1, BC_DIRICHLET = 2, BC_ROBIN = 3, BC_CALC_DIRICHLET = 4 };
 struct FieldBCs {
     std::vector<int> kind;         // per patch
     std::vector<Scalar> value;     // ncomp per patch
+    std::vector<Scalar> shape;     // per patch (ROBIN)
+    std::vector<Scalar> tvalue;    // ncomp per patch (ROBIN)
     std::vector<std::vector<Scalar>> fixed;  // CALC_DIRICHLET: per patch, ncomp per face node
 };
 FieldBCs& bcs_of(Int fIndex, int ncomp);

@@ -276,7 +276,8 @@ struct PatchArgs {
 };
 
 template <class T>
-__global__ void explicit_bc_kernel(DevTables t, PatchArgs pa, int kind, T value, T* f, T* fixed, int capture) {
+__global__ void explicit_bc_kernel(DevTables t, PatchArgs pa, int kind, T value, Scalar shape, T tvalue, T* f,
+                                   T* fixed, int capture) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= pa.n) return;
     const uint64_t k = pa.k[i];
@@ -285,6 +286,11 @@ __global__ void explicit_bc_kernel(DevTables t, PatchArgs pa, int kind, T value,
         const Vector dv = Vector(t.cC[3 * c2], t.cC[3 * c2 + 1], t.cC[3 * c2 + 2]) -
                           Vector(t.cC[3 * c1], t.cC[3 * c1 + 1], t.cC[3 * c1 + 2]);
         f[c2] = f[c1] + value * mag(dv);
+    } else if (kind == BC_ROBIN) {
+        // field.h:2677-2680
+        const Vector dv = Vector(t.cC[3 * c2], t.cC[3 * c2 + 1], t.cC[3 * c2 + 2]) -
+                          Vector(t.cC[3 * c1], t.cC[3 * c1 + 1], t.cC[3 * c1 + 2]);
+        f[c2] = shape * value + (1 - shape) * (f[c1] + tvalue * mag(dv));
     } else if (kind == BC_SYMMETRY) {
         f[c2] = sym(f[c1], fnormal(t, k));
     } else if (kind == BC_DIRICHLET) {
@@ -313,7 +319,8 @@ __global__ void ghost_value_kernel(DevTables t, PatchArgs pa, T value, T* f) {
 // applyImplicitBCs on one patch's face nodes, sequentially in the
 // reference's order (a node can appear on several faces of a patch)
 template <class T>
-__global__ void implicit_bc_kernel(DevTables t, PatchArgs pa, int kind, Scalar* ap, Scalar* ann, T* Su, const T* cF) {
+__global__ void implicit_bc_kernel(DevTables t, PatchArgs pa, int kind, Scalar shape, Scalar* ap, Scalar* ann, T* Su,
+                                   const T* cF) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (uint32_t i = 0; i < pa.n; i++) {
         const uint64_t k = pa.k[i];
@@ -321,6 +328,10 @@ __global__ void implicit_bc_kernel(DevTables t, PatchArgs pa, int kind, Scalar* 
         if (kind == BC_NEUMANN) {
             ap[c1] -= ann[k];
             Su[c1] += ann[k] * (cF[c2] - cF[c1]);
+        } else if (kind == BC_ROBIN) {
+            // field.h:2571-2573
+            ap[c1] -= (1 - shape) * ann[k];
+            Su[c1] += ann[k] * (cF[c2] - (1 - shape) * cF[c1]);
         } else {
             Su[c1] += ann[k] * cF[c2];
         }
@@ -416,8 +427,11 @@ void explicit_bcs_t(T* f, Int fIndex) {
         if (kind == BC_NONE) continue;
         const PatchArgs pa = patch(p);
         if (!pa.n) continue;
-        T value;
-        for (int c = 0; c < NC; c++) reinterpret_cast<Scalar*>(&value)[c] = b.value[p * NC + c];
+        T value, tvalue;
+        for (int c = 0; c < NC; c++) {
+            reinterpret_cast<Scalar*>(&value)[c] = b.value[p * NC + c];
+            reinterpret_cast<Scalar*>(&tvalue)[c] = b.tvalue[p * NC + c];
+        }
         T* fixed = nullptr;
         int capture = 0;
         if (kind == BC_CALC_DIRICHLET) {
@@ -430,7 +444,7 @@ void explicit_bcs_t(T* f, Int fIndex) {
             fixed = static_cast<T*>(slot.first);
         }
         hipLaunchKernelGGL((explicit_bc_kernel<T>), dim3(blocks_for(pa.n)), dim3(kThreads), 0, stream(), dev(), pa, kind,
-                           value, f, fixed, capture);
+                           value, b.shape[p], tvalue, f, fixed, capture);
     }
     check(hipGetLastError(), "boundary conditions");
 }
@@ -585,7 +599,8 @@ void implicit_bcs(Scalar* ap, Scalar* ann, Scalar* Su, const Scalar* cF, int nc,
         const PatchArgs pa = patch(p);
         if (!pa.n) continue;
         NSEM_DISPATCH(nc, hipLaunchKernelGGL((implicit_bc_kernel<T>), dim3(1), dim3(64), 0, stream(), dev(), pa, b.kind[p],
-                                             ap, ann, reinterpret_cast<T*>(Su), reinterpret_cast<const T*>(cF)));
+                                             b.shape[p], ap, ann, reinterpret_cast<T*>(Su),
+                                             reinterpret_cast<const T*>(cF)));
     }
     check(hipGetLastError(), "applyImplicitBCs");
 }

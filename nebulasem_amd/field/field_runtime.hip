@@ -572,6 +572,8 @@ FieldBCs& bcs_of(Int fIndex, int ncomp) {
     if (b.kind.size() != np) {
         b.kind.assign(np, BC_NONE);
         b.value.assign(np * ncomp, 0.0);
+        b.shape.assign(np, 0.0);
+        b.tvalue.assign(np * ncomp, 0.0);
         b.fixed.assign(np, {});
     }
     return b;
@@ -589,10 +591,19 @@ void copy_bcs(Int src, Int dst, int ncomp, int mode, Scalar psi) {
             d.fixed[b].clear();
         } else {  // scaleBCs
             d.kind[b] = s.kind[b];
-            for (int c = 0; c < ncomp; c++) d.value[b * ncomp + c] = s.value[b * ncomp + c];
-            if (s.kind[b] == BC_NEUMANN)
+            d.shape[b] = s.shape[b];
+            for (int c = 0; c < ncomp; c++) {
+                d.value[b * ncomp + c] = s.value[b * ncomp + c];
+                d.tvalue[b * ncomp + c] = s.tvalue[b * ncomp + c];
+            }
+            if (s.kind[b] == BC_NEUMANN) {
                 for (int c = 0; c < ncomp; c++) d.value[b * ncomp + c] *= psi;
-            else if (s.kind[b] != BC_SYMMETRY) {
+            } else if (s.kind[b] == BC_ROBIN) {
+                for (int c = 0; c < ncomp; c++) {
+                    d.value[b * ncomp + c] *= psi;
+                    d.tvalue[b * ncomp + c] *= psi;
+                }
+            } else if (s.kind[b] != BC_SYMMETRY) {
                 d.kind[b] = BC_CALC_DIRICHLET;
                 d.fixed[b].clear();
             }
@@ -606,14 +617,19 @@ void read_field_file(const std::string& name, Int step, int ncomp, Scalar* host_
     if (!exists(path)) fail("cannot read field file " + base + ".txt or .bin");
     if (ncomp != 1 && ncomp != 3) fail("field files of " + std::to_string(ncomp) + " components are not read here");
     std::vector<int32_t> kind(g_mesh.npatch + 1);
-    std::vector<double> val((g_mesh.npatch + 1) * ncomp);
+    std::vector<double> val((g_mesh.npatch + 1) * ncomp), shape(g_mesh.npatch + 1), tval((g_mesh.npatch + 1) * ncomp);
     const double g[3] = {Controls::gravity[0], Controls::gravity[1], Controls::gravity[2]};
-    if (nsem_field_read(g_handle, path.c_str(), ncomp, g, host_all, kind.data(), val.data()))
+    if (nsem_field_read_bc(g_handle, path.c_str(), ncomp, g, host_all, kind.data(), val.data(), shape.data(),
+                           tval.data()))
         fail("reading " + path + ": " + nsem_last_error());
     FieldBCs& b = bcs_of(fIndex, ncomp);
     for (size_t p = 0; p < g_mesh.npatch; p++) {
         b.kind[p] = kind[p] < 0 ? BC_NONE : kind[p];
-        for (int c = 0; c < ncomp; c++) b.value[p * ncomp + c] = val[p * ncomp + c];
+        b.shape[p] = shape[p];
+        for (int c = 0; c < ncomp; c++) {
+            b.value[p * ncomp + c] = val[p * ncomp + c];
+            b.tvalue[p * ncomp + c] = tval[p * ncomp + c];
+        }
         b.fixed[p].clear();
     }
     MP::printH("Reading %s\n", base.c_str());
@@ -623,15 +639,21 @@ void write_field_file(const std::string& name, Int step, int ncomp, const Scalar
     const std::string path =
         name + std::to_string(step) + (Controls::write_format == Controls::BINARY ? ".bin" : ".txt");
     std::vector<int32_t> kind(g_mesh.npatch + 1, -1);
-    std::vector<double> val((g_mesh.npatch + 1) * ncomp, 0.0);
+    std::vector<double> val((g_mesh.npatch + 1) * ncomp, 0.0), shape(g_mesh.npatch + 1, 0.0),
+        tval((g_mesh.npatch + 1) * ncomp, 0.0);
     if (has_bcs(fIndex)) {
         FieldBCs& b = bcs_of(fIndex, ncomp);
         for (size_t p = 0; p < g_mesh.npatch; p++) {
-            kind[p] = (b.kind[p] >= BC_NEUMANN && b.kind[p] <= BC_DIRICHLET) ? b.kind[p] : -1;
-            for (int c = 0; c < ncomp; c++) val[p * ncomp + c] = b.value[p * ncomp + c];
+            kind[p] = (b.kind[p] >= BC_NEUMANN && b.kind[p] <= BC_ROBIN) ? b.kind[p] : -1;
+            shape[p] = b.shape[p];
+            for (int c = 0; c < ncomp; c++) {
+                val[p * ncomp + c] = b.value[p * ncomp + c];
+                tval[p * ncomp + c] = b.tvalue[p * ncomp + c];
+            }
         }
     }
-    if (nsem_field_write(g_handle, path.c_str(), ncomp, host_all, kind.data(), val.data()))
+    if (nsem_field_write_bc(g_handle, path.c_str(), ncomp, host_all, kind.data(), val.data(), shape.data(),
+                            tval.data()))
         fail("writing " + path + ": " + nsem_last_error());
 }
 }  // namespace nsem_field

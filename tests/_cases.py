@@ -86,10 +86,12 @@ refinement
 
 
 def field_file(ncomp, gens, bcs):
+    """bcs: (patch, type) or (patch, type, [extra lines such as "value 1"])."""
     body = ["size %d" % ncomp, "internal %d" % len(gens), "{"] + ["    " + g for g in gens] + ["}"]
     body += ["boundary %d" % len(bcs), "{"]
-    for name, kind in bcs:
-        body += ["    %s {" % name, "        type %s" % kind, "    }"]
+    for bc in bcs:
+        body += ["    %s {" % bc[0], "        type %s" % bc[1]] + ["        " + x for x in (bc[2] if len(bc) > 2 else [])]
+        body += ["    }"]
     body += ["}"]
     return "\n".join(body) + "\n"
 
@@ -148,9 +150,10 @@ potential
         for u in U:
             f.write("%.17g %.17g %.17g\n" % tuple(u))
         f.write("}\n" + field_file(3, [], [(nm, "NEUMANN") for nm in names]).split("}\n", 1)[1])
+    robin = case.get("robin", (None, []))
     with open(os.path.join(d, "p0.txt"), "w") as f:
-        f.write(field_file(1, ["uniform 0"], [(nm, "DIRICHLET" if nm == "xmin" else "NEUMANN")
-                                              for nm in names]))
+        f.write(field_file(1, ["uniform 0"], [(nm, "ROBIN", robin[1]) if nm == robin[0] else
+                                              (nm, "DIRICHLET" if nm == "xmin" else "NEUMANN") for nm in names]))
 
 
 def potential_velocity(case):
@@ -235,6 +238,15 @@ POTENTIAL_CASES = [
     dict(name="pot_8", ndiv=(8, 8, 8), keep_fields=True),
     dict(name="pot_16", ndiv=(16, 16, 16)),
     dict(name="pot_12x8x6", ndiv=(12, 8, 6), seed=7),
+]
+
+# a ROBIN patch (applyImplicitBCs / applyExplicitBCs, field.h:2571-2573,
+# 2677-2680): p on xmax mixes the value 1 and the slope 0.2 half and half.
+# Run through the field API only (the C-ABI pressure solver and the oracle
+# take NEUMANN and DIRICHLET patches).
+POTENTIAL_ROBIN_CASES = [
+    dict(name="pot_robin", ndiv=(10, 8, 6), seed=3, keep_fields=True,
+         robin=("xmax", ["value 1", "shape 0.5", "tvalue 0.2"])),
 ]
 
 # a sheared unit cube (x shifted by 0.3 z, y by 0.2 x): every face is

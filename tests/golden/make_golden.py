@@ -28,7 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 from _cases import (AMR_CASES, EULER_CASES, GEOM_CASES, PISO_CASES, POTENTIAL_CASES,  # noqa: E402
-                    POTENTIAL_SKEW_CASES, write_euler_case, write_piso_case, write_potential_case)
+                    POTENTIAL_ROBIN_CASES, POTENTIAL_SKEW_CASES, write_euler_case, write_piso_case, write_potential_case)
 from _fieldio import read_field_bin  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
@@ -227,7 +227,7 @@ def main():
             rec = euler_record(case, d, arrays)
             fixtures["euler"][case["name"]] = rec
             print("amr", case["name"], passes, rec["log_tail"][-1])
-        for case in pick(POTENTIAL_CASES + POTENTIAL_SKEW_CASES):
+        for case in pick(POTENTIAL_CASES + POTENTIAL_SKEW_CASES + POTENTIAL_ROBIN_CASES):
             d = os.path.join(tmp, "p_" + case["name"])
             write_potential_case(d, case)
             run([os.path.join(REF, "mesh"), "spec", "-o", "grid_0.bin"], d)

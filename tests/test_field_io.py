@@ -96,3 +96,47 @@ def test_amr_tree_round_trip_and_structure(tmp_path, case):
         assert np.all(level[kids] == level[k] + 1)
     assert level.max() == case["refine"]["max_level"]
 
+
+
+@pytest.mark.parametrize("ext", ["txt", "bin"])
+def test_robin_patches_read_write(tmp_path, ext):
+    """ROBIN patches (BCondition's value / shape / tvalue, field.h:156-240):
+    written with their shape and tvalue lines, read back by
+    nsem_field_read_bc with every ghost node set as applyExplicitBCs sets it
+    (shape value + (1 - shape)(owner + tvalue |x_ghost - x_owner|),
+    field.h:2677-2680), written again byte for byte; nsem_field_read,
+    which has no place for shape and tvalue, refuses the file."""
+    from nebulasem_amd import BC_ROBIN, NsemError, read_field_bc, write_field_bc
+    case = [c for c in EULER_CASES if c["name"] == "e3_n3"][0]
+    mesh, _ = load_case(tmp_path, case)
+    nb = len(mesh.boundaries)
+    nbcs, nall = mesh.info.nbcs_field, mesh.info.nall_field
+    rng = np.random.default_rng(11)
+    vals = rng.uniform(250.0, 350.0, nbcs)
+    kinds = np.full(nb, BC_ROBIN, dtype=np.int32)
+    bv = np.full((nb, 1), 300.0)
+    sh = np.full(nb, 0.25)
+    tv = np.full((nb, 1), -0.0065)
+    out = str(tmp_path / f"T0.{ext}")
+    write_field_bc(mesh, out, vals, kinds, bv, sh, tv)
+    if ext == "txt":
+        text = open(out).read()
+        assert text.count("\ttype ROBIN\n") == nb and text.count("\tshape 0.25\n") == nb
+        assert text.count("\ttvalue -0.0065\n") == nb
+    v2, k2, b2, s2, t2 = read_field_bc(mesh, out, 1)
+    assert np.array_equal(k2, kinds) and np.array_equal(b2, bv) and np.array_equal(s2, sh)
+    assert np.array_equal(t2, tv)
+    v = v2[:, 0]
+    assert np.array_equal(v[:nbcs], vals) if ext == "bin" else np.allclose(v[:nbcs], vals, rtol=1e-11)
+    FO, FN, cC = mesh.array("FO"), mesh.array("FN"), mesh.array("cC").reshape(-1, 3)
+    k = np.nonzero((FN >= nbcs) & (FN < nall))[0]
+    c1, c2 = FO[k].astype(np.int64), FN[k].astype(np.int64)
+    dv = cC[c2] - cC[c1]
+    md = np.sqrt(((dv[:, 2] * dv[:, 2] + 0.0) + dv[:, 1] * dv[:, 1]) + dv[:, 0] * dv[:, 0])
+    want = 0.25 * 300.0 + (1 - 0.25) * (v[c1] + -0.0065 * md)
+    assert k.size > 0 and np.array_equal(v[c2], want)
+    out2 = str(tmp_path / f"T0b.{ext}")
+    write_field_bc(mesh, out2, v[:nbcs], k2, b2, s2, t2)
+    assert open(out2, "rb").read() == open(out, "rb").read()
+    with pytest.raises(NsemError):
+        read_field(mesh, out, 1)

@@ -28,8 +28,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from _cases import (EULER_CASES, PISO_CASES, POTENTIAL_CASES, POTENTIAL_SKEW_CASES, write_euler_case, write_piso_case,
-                    write_potential_case)
+from _cases import (EULER_CASES, PISO_CASES, POTENTIAL_CASES, POTENTIAL_ROBIN_CASES, POTENTIAL_SKEW_CASES,
+                    write_euler_case, write_piso_case, write_potential_case)
 from _fieldio import read_field_bin
 from nebulasem_amd import _lib
 
@@ -96,11 +96,12 @@ def test_reference_euler_app_on_field_api(tmp_path, case):
     print(f"  max |api - ref| / max|ref| (fields kept as arrays): {worst}", flush=True)
 
 
-POTENTIAL_ALL = POTENTIAL_CASES + POTENTIAL_SKEW_CASES
+POTENTIAL_ALL = POTENTIAL_CASES + POTENTIAL_SKEW_CASES + POTENTIAL_ROBIN_CASES
 
 
 # pot_skew: a sheared grid on which lap's non-orthogonal correction
-# (field.h:3676-3699) moves the reference's solution by 7%
+# (field.h:3676-3699) moves the reference's solution by 7%; pot_robin: a
+# ROBIN patch (57 iterations against 72 with NEUMANN there)
 @pytest.mark.parametrize("case", POTENTIAL_ALL, ids=[c["name"] for c in POTENTIAL_ALL])
 def test_reference_potential_app_on_field_api(tmp_path, case):
     print(f"[gpu] field API potential {case['name']}", flush=True)
