@@ -573,6 +573,17 @@ def fv_bench(n, tol=1e-10):
         t_iter = out[name]["seconds"] / max(out[name]["iterations"], 1)
         out[name]["GBps_per_iter"] = b_iter * n ** 3 / t_iter / 1e9
         out[name]["frac_hbm_peak"] = out[name]["GBps_per_iter"] * 1e9 / HBM_PEAK
+    # the bytes the kernels move per cell per iteration when the couplings
+    # are uniform (taken as constants, so the SpMV reads only p and the
+    # diagonal): SpMV 16 read + 8 written, x/r/z update 40 + 24, p update
+    # 16 + 8 = 112 B in FP64, 56 B for the FP32 inner iterations (the
+    # solver's counter profile measures them to 0.5%); the FP32 figure's
+    # time also holds the FP64 refinement steps
+    if fv.uniform:
+        for name, b_moved in (("fp64", 112.0), ("fp32_refined", 56.0)):
+            t_iter = out[name]["seconds"] / max(out[name]["iterations"], 1)
+            out[name]["moved_GBps_per_iter"] = b_moved * n ** 3 / t_iter / 1e9
+            out[name]["moved_frac_hbm_peak"] = out[name]["moved_GBps_per_iter"] * 1e9 / HBM_PEAK
     out["operator"] = ("structured, uniform couplings" if fv.uniform else
                        "structured" if fv.structured else "general")
     out["cells"] = n ** 3
