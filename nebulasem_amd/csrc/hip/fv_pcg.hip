@@ -32,6 +32,7 @@ namespace nsem {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMarch = 4;   // x-planes per pass of the structured SpMV march
 
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
@@ -161,20 +162,48 @@ __global__ void __launch_bounds__(kThreads) spmv_dot_march_kernel(FvOp<R> A, Mar
             uint64_t n = (uint64_t)i0 * sx + col;
             R pm = i0 > 0 ? p[n - sx] : R(0), c0m = UNI ? A.cu[0] : (i0 > 0 ? c0[n - sx] : R(0));
             R pc = p[n];
-            for (int i = i0; i < i1; i++, n += sx) {
-                const bool hx = i + 1 < A.nx;
-                const R pn = hx ? p[n + sx] : R(0);
-                const R c0c = UNI ? A.cu[0] : c0[n];
-                R acc = pc * A.diag[n];
-                if (hx) acc -= pn * c0c;
-                if (i > 0) acc -= pm * c0m;
-                if (j + 1 < A.ny) acc -= p[n + A.nz] * (UNI ? A.cu[1] : c1[n]);
-                if (j > 0) acc -= p[n - A.nz] * (UNI ? A.cu[1] : c1[n - A.nz]);
-                if (k + 1 < A.nz) acc -= p[n + 1] * (UNI ? A.cu[2] : c2[n]);
-                if (k > 0) acc -= p[n - 1] * (UNI ? A.cu[2] : c2[n - 1]);
-                Ap[n] = acc;
-                v[0] += (double)pc * (double)acc;
-                pm = pc; pc = pn; c0m = c0c;
+            const bool hyp = j + 1 < A.ny, hym = j > 0, hzp = k + 1 < A.nz, hzm = k > 0;
+            // kMarch planes per pass: every load of the pass is issued before
+            // the first row is formed, so a wave keeps kMarch planes of reads
+            // in flight (one plane at a time leaves the march latency-bound);
+            // each row is then formed exactly as one plane at a time forms it
+            for (int i = i0; i < i1; i += kMarch, n += (uint64_t)kMarch * sx) {
+                R pn[kMarch], dg[kMarch], yp[kMarch], ym[kMarch], zp[kMarch], zm[kMarch];
+                R cx[kMarch], cyp[kMarch], cym[kMarch], czp[kMarch], czm[kMarch];
+#pragma unroll
+                for (int u = 0; u < kMarch; u++) {
+                    const uint64_t m = n + (uint64_t)u * sx;
+                    const bool in = i + u < i1;
+                    pn[u] = (in && i + u + 1 < A.nx) ? p[m + sx] : R(0);
+                    dg[u] = in ? A.diag[m] : R(0);
+                    yp[u] = (in && hyp) ? p[m + A.nz] : R(0);
+                    ym[u] = (in && hym) ? p[m - A.nz] : R(0);
+                    zp[u] = (in && hzp) ? p[m + 1] : R(0);
+                    zm[u] = (in && hzm) ? p[m - 1] : R(0);
+                    if constexpr (UNI) {
+                        cx[u] = A.cu[0]; cyp[u] = cym[u] = A.cu[1]; czp[u] = czm[u] = A.cu[2];
+                    } else {
+                        cx[u] = in ? c0[m] : R(0);
+                        cyp[u] = (in && hyp) ? c1[m] : R(0);
+                        cym[u] = (in && hym) ? c1[m - A.nz] : R(0);
+                        czp[u] = (in && hzp) ? c2[m] : R(0);
+                        czm[u] = (in && hzm) ? c2[m - 1] : R(0);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kMarch; u++) {
+                    if (i + u >= i1) break;
+                    R acc = pc * dg[u];
+                    if (i + u + 1 < A.nx) acc -= pn[u] * cx[u];
+                    if (i + u > 0) acc -= pm * c0m;
+                    if (hyp) acc -= yp[u] * cyp[u];
+                    if (hym) acc -= ym[u] * cym[u];
+                    if (hzp) acc -= zp[u] * czp[u];
+                    if (hzm) acc -= zm[u] * czm[u];
+                    Ap[n + (uint64_t)u * sx] = acc;
+                    v[0] += (double)pc * (double)acc;
+                    pm = pc; pc = pn[u]; c0m = cx[u];
+                }
             }
         }
     }

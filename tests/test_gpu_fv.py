@@ -63,6 +63,30 @@ def test_fv_modes_reach_true_residual(oracle, tmp_path):
     fv.close()
 
 
+def test_fv_march_with_partial_passes_reaches_true_residual(tmp_path):
+    """256 x 256 cells per x-plane make the structured SpMV march 8 chunks of
+    5 planes along x (40 planes): each chunk takes one full pass of 4 planes
+    and one pass of a single plane.  The true residual is formed by the
+    separate row-per-thread apply kernel, so a plane the march skips or
+    repeats shows there."""
+    print("[gpu] fv march 40x256x256", flush=True)
+    case = dict(name="pot_march", ndiv=(40, 256, 256), seed=17)
+    mesh, U, p, kp, vp = potential_inputs(tmp_path, case)
+    n = mesh.info.nbcs
+    fv = FVPoisson(mesh, kp, vp)
+    assert fv.structured
+    fv.set_rhs_divergence(U[:n], np.zeros(len(mesh.boundaries), dtype=np.int32),
+                          np.zeros((len(mesh.boundaries), 3)))
+    b = fv.rhs()
+    for mode in (1, 2):
+        x, r, rc = fv.solve(mode=mode, tol=1e-10)
+        true = np.linalg.norm(b - fv.apply(x)) / np.linalg.norm(b)
+        print(f"  mode {mode} iters {r.iterations} true {true:.2e} uniform {fv.uniform}", flush=True)
+        assert rc == 0
+        assert true <= 1e-10 * 1.01
+    fv.close()
+
+
 def test_fv_nonuniform_coefficient_keeps_stored_couplings(oracle, tmp_path):
     """A variable coefficient mu (as piso's rmu) makes the couplings differ,
     so the SpMV reads them; the solution still matches a dense solve of the
